@@ -1,0 +1,842 @@
+// amt.hip — host orchestration and the C-ABI of libamt.so (declared in include/amt.h).
+//
+// One amt_problem owns every device buffer of one SIMP problem and one HIP stream. An OC
+// iteration (topology_optimizer.cpp:294-384) runs entirely on the device:
+//   k_simp -> k_diag -> k_dinv -> k_bc_rhs            ("assembly": operator coefficients)
+//   PCG: k_pcg_init, then k_pcg_apply/k_pcg_update     ("solve"; state stays on the device,
+//        pairs launched in chunks, host polls one flag  the host polls a done flag per chunk)
+//   k_dot (compliance), k_sens                         ("sensitivity")
+//   k_filter, k_oc_sweep x ~16, k_oc_final             ("update")
+// and one small device->host copy of scalars ends it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/amt.h"
+#include "kernels.hpp"
+
+namespace amt {
+void hex8_stiffness(double E, double nu, double h, double* Ke);
+}
+
+using namespace amt;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+constexpr int kOcLevels = 5;
+constexpr int kOcCand = (1 << kOcLevels) - 1;
+constexpr int kMaxReduceBlocks = 1024;
+constexpr int kChunk = 32;  // PCG iterations launched between two host polls
+
+enum Ticket {
+  T_DIAG = 0, T_INIT, T_APPLY, T_UPDATE, T_RESID, T_DOT, T_OC_SWEEP, T_OC_FINAL0, T_OC_FINAL1,
+  T_MASK, T_DOT2, T_INIT32, T_APPLY32, T_UPDATE32, T_COUNT
+};
+
+// Device scalar slots
+enum Scal { S_MAXDIAG = 0, S_P, S_COMPL, S_NDESIGN, S_RR, S_BB, S_COUNT };
+
+inline double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+}  // namespace
+
+struct amt_problem {
+  amt_config cfg;
+  GridDims g;
+  int64_t n_e = 0, n_n = 0, n_dof = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  double* Ke64 = nullptr;
+  float* Ke32 = nullptr;
+  double *rho = nullptr, *s64 = nullptr;
+  float* s32 = nullptr;
+  uint8_t *design = nullptr, *fixcnt = nullptr;
+  double *F = nullptr, *Fb = nullptr, *diag = nullptr, *dinv = nullptr;
+  float* dinv32 = nullptr;
+  double *u = nullptr, *r = nullptr, *z = nullptr, *p[2] = {nullptr, nullptr}, *q = nullptr;
+  float *r32 = nullptr, *z32 = nullptr, *p32[2] = {nullptr, nullptr}, *q32 = nullptr,
+        *d32 = nullptr;
+  double *dc = nullptr, *dcf = nullptr;
+  StencilEntry* stencil = nullptr;
+  double* stencil_w = nullptr;
+  int n_stencil = 0;
+  double* partials = nullptr;
+  unsigned* tickets = nullptr;
+  double* scal = nullptr;
+  PcgState* pcg = nullptr;
+  OcState* oc = nullptr;
+  int64_t* dbg_idx = nullptr;
+  double* dbg_w = nullptr;
+  int* dbg_cnt = nullptr;
+
+  bool loads_set = false, design_ready = false, coeffs_ready = false, have_u = false;
+  double n_design = 0;
+
+  // operator timing (HIP events around k_pcg_apply launches)
+  bool prof_on = false;
+  std::vector<hipEvent_t> ev_pool;
+  int ev_used = 0;
+  int64_t prof_launches = 0;
+  double prof_ms = 0.0;
+  int prof_every = 8, prof_tick = 0;
+
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+int fail(amt_problem* p, int code, const std::string& msg) {
+  if (p)
+    p->err = msg;
+  else
+    g_create_error = msg;
+  return code;
+}
+
+#define HIP_TRY(p, call)                                                              \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail((p), AMT_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define AMT_CHECK(call)            \
+  do {                             \
+    int rc_ = (call);              \
+    if (rc_ != AMT_OK) return rc_; \
+  } while (0)
+
+template <typename T>
+int dalloc(amt_problem* p, T** ptr, size_t count) {
+  void* v = nullptr;
+  hipError_t e = hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess)
+    return fail(p, AMT_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  p->allocs.push_back(v);
+  *ptr = (T*)v;
+  return AMT_OK;
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+inline unsigned nred(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(nblk(n), kMaxReduceBlocks));
+}
+
+int check_launch(amt_problem* p, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(p, AMT_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return AMT_OK;
+}
+
+// Filter stencil (topology_optimizer.cpp:60-102): offsets in dz, dy, dx order with the
+// reference's distance and weight arithmetic; the per-element list is this table with
+// out-of-grid offsets skipped.
+void build_stencil(double r, std::vector<StencilEntry>& st, std::vector<double>& w) {
+  const int R = (int)std::ceil(r);
+  for (int dz = -R; dz <= R; ++dz)
+    for (int dy = -R; dy <= R; ++dy)
+      for (int dx = -R; dx <= R; ++dx) {
+        const double dist = std::sqrt((double)(dx * dx + dy * dy + dz * dz));
+        if (dist <= r) {
+          st.push_back({dx, dy, dz, 0});
+          w.push_back(std::max(0.0, r - dist));
+        }
+      }
+}
+
+int copy_in(amt_problem* p, void* dst, const void* src, size_t bytes) {
+  HIP_TRY(p, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, p->stream));
+  HIP_TRY(p, hipStreamSynchronize(p->stream));
+  return AMT_OK;
+}
+int copy_out(amt_problem* p, void* dst, const void* src, size_t bytes) {
+  HIP_TRY(p, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(p, hipStreamSynchronize(p->stream));
+  return AMT_OK;
+}
+
+// ---- operator coefficients for the current density ("assembly") ----
+int prepare_coeffs(amt_problem* p) {
+  const GridDims g = p->g;
+  hipLaunchKernelGGL(k_simp, dim3(nblk(p->n_e)), dim3(kBlock), 0, p->stream, p->rho, p->s64,
+                     p->s32, g, p->cfg.density_index == AMT_INDEX_REF_COMPAT ? 1 : 0,
+                     p->cfg.penalty, p->cfg.E_min, p->cfg.rho_min);
+  hipLaunchKernelGGL(k_diag, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, p->diag, p->s64,
+                     p->Ke64, g, p->partials, p->tickets + T_DIAG, p->scal + S_MAXDIAG);
+  hipLaunchKernelGGL(k_dinv, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->diag,
+                     p->fixcnt, p->scal + S_MAXDIAG, p->cfg.penalty_factor, p->scal + S_P,
+                     p->dinv, p->dinv32, p->n_dof);
+  hipLaunchKernelGGL(k_bc_rhs, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->F,
+                     p->fixcnt, p->Fb, p->n_dof);
+  p->coeffs_ready = true;
+  return check_launch(p, "prepare_coeffs");
+}
+
+void prof_begin(amt_problem* p, hipEvent_t* a) {
+  *a = nullptr;
+  if (!p->prof_on) return;
+  if (++p->prof_tick % p->prof_every) return;
+  if (p->ev_used + 2 > (int)p->ev_pool.size()) return;
+  *a = p->ev_pool[p->ev_used];
+  (void)hipEventRecord(*a, p->stream);
+}
+void prof_end(amt_problem* p, hipEvent_t a) {
+  if (!a) return;
+  (void)hipEventRecord(p->ev_pool[p->ev_used + 1], p->stream);
+  p->ev_used += 2;
+}
+void prof_harvest(amt_problem* p) {
+  if (!p->prof_on || p->ev_used == 0) return;
+  (void)hipStreamSynchronize(p->stream);
+  for (int i = 0; i < p->ev_used; i += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p->ev_pool[i], p->ev_pool[i + 1]) == hipSuccess) {
+      p->prof_ms += ms;
+      p->prof_launches += 1;
+    }
+  }
+  p->ev_used = 0;
+}
+
+// Launch PCG pairs until the device reports done. T = vector type, X = solution pointer.
+template <typename T, typename TK, typename TX>
+int pcg_loop(amt_problem* p, TX* x, T* r, T* z, T** pp, T* q, const T* s, const TK* Ke,
+             const T* dinv, int t_apply, int t_update, PcgState* host_state) {
+  const GridDims g = p->g;
+  int t = 0;
+  for (;;) {
+    for (int c = 0; c < kChunk; ++c, ++t) {
+      hipEvent_t ev;
+      prof_begin(p, &ev);
+      hipLaunchKernelGGL((k_pcg_apply<T, TK>), dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream,
+                         z, pp[t & 1], pp[(t + 1) & 1], q, s, Ke, p->fixcnt, p->scal + S_P, g,
+                         p->pcg, p->partials, p->tickets + t_apply);
+      prof_end(p, ev);
+      hipLaunchKernelGGL((k_pcg_update<T, TX>), dim3(nred(p->n_dof)), dim3(kBlock), 0,
+                         p->stream, x, r, z, pp[(t + 1) & 1], q, dinv, p->n_dof, p->pcg,
+                         p->partials, p->tickets + t_update);
+    }
+    AMT_CHECK(check_launch(p, "pcg chunk"));
+    HIP_TRY(p, hipMemcpyAsync(host_state, p->pcg, sizeof(PcgState), hipMemcpyDeviceToHost,
+                              p->stream));
+    HIP_TRY(p, hipStreamSynchronize(p->stream));
+    if (host_state->done) break;
+  }
+  prof_harvest(p);
+  return AMT_OK;
+}
+
+// FP64 residual r_out = Fb - K x and |r|^2 into scal[S_RR] (blocking read).
+int residual64(amt_problem* p, const double* x, double* rout, double* rr_host) {
+  hipLaunchKernelGGL(k_residual, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, x, p->Fb,
+                     rout, p->s64, p->Ke64, p->fixcnt, p->scal + S_P, p->g, p->partials,
+                     p->tickets + T_RESID, p->scal + S_RR);
+  AMT_CHECK(check_launch(p, "k_residual"));
+  if (rr_host) {
+    HIP_TRY(p, hipMemcpyAsync(rr_host, p->scal + S_RR, sizeof(double), hipMemcpyDeviceToHost,
+                              p->stream));
+    HIP_TRY(p, hipStreamSynchronize(p->stream));
+  }
+  return AMT_OK;
+}
+
+__global__ void k_cast_scale(const double* __restrict__ a, float* __restrict__ b, double sc,
+                             int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) b[i] = (float)(a[i] * sc);
+}
+__global__ void k_axpy_cast(double* __restrict__ x, const float* __restrict__ d, double sc,
+                            int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = fma((double)d[i], sc, x[i]);
+}
+
+// Plain FP64 Jacobi-PCG (Eigen loop), from x = 0 or from the previous displacement.
+int solve_fp64(amt_problem* p, amt_solve_stats* st) {
+  const bool warm = p->cfg.warm_start && p->have_u;
+  if (warm) {
+    AMT_CHECK(residual64(p, p->u, p->r, nullptr));
+  } else {
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->u,
+                       0.0, p->n_dof);
+    HIP_TRY(p, hipMemcpyAsync(p->r, p->Fb, sizeof(double) * p->n_dof, hipMemcpyDeviceToDevice,
+                              p->stream));
+  }
+  hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->p[0],
+                     0.0, p->n_dof);
+  hipLaunchKernelGGL((k_pcg_init<double>), dim3(nred(p->n_dof)), dim3(kBlock), 0, p->stream,
+                     p->Fb, p->r, p->z, p->dinv, p->n_dof, p->cfg.pcg_tol, p->cfg.pcg_maxit,
+                     p->pcg, p->partials, p->tickets + T_INIT);
+  AMT_CHECK(check_launch(p, "k_pcg_init"));
+  PcgState hs;
+  HIP_TRY(p, hipMemcpyAsync(&hs, p->pcg, sizeof(PcgState), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(p, hipStreamSynchronize(p->stream));
+  if (hs.rhs2 == 0.0) {  // Eigen: zero rhs -> x = 0
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream,
+                       p->u, 0.0, p->n_dof);
+  } else if (!hs.done) {
+    AMT_CHECK((pcg_loop<double, double, double>(p, p->u, p->r, p->z, p->p, p->q, p->s64,
+                                                p->Ke64, p->dinv, T_APPLY, T_UPDATE, &hs)));
+  }
+  double rr_true = 0.0;
+  AMT_CHECK(residual64(p, p->u, p->q, &rr_true));
+  st->iterations = hs.it;
+  st->converged = hs.converged;
+  st->outer_iterations = 1;
+  st->rel_residual = hs.rhs2 > 0 ? std::sqrt(hs.rr / hs.rhs2) : 0.0;
+  st->true_rel_residual = hs.rhs2 > 0 ? std::sqrt(rr_true / hs.rhs2) : 0.0;
+  return AMT_OK;
+}
+
+// FP64 iterative refinement around FP32 Jacobi-PCG inner solves:
+//   r = F_bc - K x (FP64);  solve K d = r/|r| in FP32 to inner_tol;  x += |r| d.
+// The stopping test is the FP64 true residual |r| <= pcg_tol |F_bc|.
+int solve_mixed(amt_problem* p, amt_solve_stats* st) {
+  const bool warm = p->cfg.warm_start && p->have_u;
+  if (!warm)
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream,
+                       p->u, 0.0, p->n_dof);
+  hipLaunchKernelGGL(k_dot, dim3(nred(p->n_dof)), dim3(kBlock), 0, p->stream, p->Fb, p->Fb,
+                     p->n_dof, p->partials, p->tickets + T_DOT2, p->scal + S_BB);
+  double bb = 0.0, rr = 0.0;
+  HIP_TRY(p, hipMemcpyAsync(&bb, p->scal + S_BB, sizeof(double), hipMemcpyDeviceToHost,
+                            p->stream));
+  AMT_CHECK(residual64(p, p->u, p->r, &rr));
+  const double thresh = std::max(p->cfg.pcg_tol * p->cfg.pcg_tol * bb, 2.2250738585072014e-308);
+  int inner_total = 0, outer = 0;
+  const int max_outer = 60;
+  PcgState hs;
+  std::memset(&hs, 0, sizeof(hs));
+  while (bb > 0.0 && rr >= thresh && outer < max_outer && inner_total < p->cfg.pcg_maxit) {
+    const double nr = std::sqrt(rr);
+    hipLaunchKernelGGL(k_cast_scale, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->r,
+                       p->r32, 1.0 / nr, p->n_dof);
+    hipLaunchKernelGGL(k_fill<float>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->d32,
+                       0.0f, p->n_dof);
+    hipLaunchKernelGGL(k_fill<float>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream,
+                       p->p32[0], 0.0f, p->n_dof);
+    const int budget = std::min(p->cfg.inner_maxit, p->cfg.pcg_maxit - inner_total);
+    hipLaunchKernelGGL((k_pcg_init<float>), dim3(nred(p->n_dof)), dim3(kBlock), 0, p->stream,
+                       (const double*)nullptr, p->r32, p->z32, p->dinv32, p->n_dof,
+                       p->cfg.inner_tol, budget, p->pcg, p->partials, p->tickets + T_INIT32);
+    AMT_CHECK(check_launch(p, "k_pcg_init<float>"));
+    AMT_CHECK((pcg_loop<float, float, float>(p, p->d32, p->r32, p->z32, p->p32, p->q32, p->s32,
+                                             p->Ke32, p->dinv32, T_APPLY32, T_UPDATE32, &hs)));
+    inner_total += std::max(hs.it, 1);
+    hipLaunchKernelGGL(k_axpy_cast, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->u,
+                       p->d32, nr, p->n_dof);
+    AMT_CHECK(residual64(p, p->u, p->r, &rr));
+    ++outer;
+  }
+  st->iterations = inner_total;
+  st->outer_iterations = outer;
+  st->converged = (bb == 0.0) || (rr < thresh);
+  st->rel_residual = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+  st->true_rel_residual = st->rel_residual;
+  return AMT_OK;
+}
+
+int do_solve(amt_problem* p, amt_solve_stats* st) {
+  if (!p->loads_set) return fail(p, AMT_EINVAL, "loads not set (amt_set_loads)");
+  if (!p->coeffs_ready) AMT_CHECK(prepare_coeffs(p));
+  int rc = p->cfg.precision == AMT_PREC_MIXED_FP32 ? solve_mixed(p, st) : solve_fp64(p, st);
+  if (rc == AMT_OK) p->have_u = true;
+  return rc;
+}
+
+int build_design(amt_problem* p) {
+  hipLaunchKernelGGL(k_design_mask, dim3(nred(p->n_e)), dim3(kBlock), 0, p->stream, p->rho,
+                     p->design, p->n_e, p->cfg.rho_min, p->partials, p->tickets + T_MASK,
+                     p->scal + S_NDESIGN);
+  AMT_CHECK(check_launch(p, "k_design_mask"));
+  HIP_TRY(p, hipMemcpyAsync(&p->n_design, p->scal + S_NDESIGN, sizeof(double),
+                            hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(p, hipStreamSynchronize(p->stream));
+  p->design_ready = true;
+  return AMT_OK;
+}
+
+// OC update on the device density with p->dcf; writes state into p->oc.
+int oc_update(amt_problem* p, OcState* hs) {
+  if (!p->design_ready) AMT_CHECK(build_design(p));
+  OcState init;
+  std::memset(&init, 0, sizeof(init));
+  init.l1 = 1e-20;
+  init.l2 = 1e20;
+  init.vf = p->cfg.volume_fraction;
+  init.n_design = p->n_design > 0 ? p->n_design : (double)p->n_e;  // :222-225 fallback
+  HIP_TRY(p, hipMemcpyAsync(p->oc, &init, sizeof(OcState), hipMemcpyHostToDevice, p->stream));
+  const unsigned nb = nred(p->n_e);
+  // ceil(100 / LEV) sweeps always suffice: each one advances LEV passes or finishes
+  for (int s = 0; s < (100 + kOcLevels - 1) / kOcLevels + 1; ++s)
+    hipLaunchKernelGGL((k_oc_sweep<kOcLevels>), dim3(nb), dim3(kBlock), 0, p->stream, p->rho,
+                       p->dcf, p->design, p->n_e, p->cfg.move_limit, p->cfg.rho_min, p->oc,
+                       p->partials, p->tickets + T_OC_SWEEP);
+  hipLaunchKernelGGL(k_oc_final, dim3(nb), dim3(kBlock), 0, p->stream, p->rho, p->dcf,
+                     p->design, p->n_e, p->cfg.move_limit, p->cfg.rho_min, p->oc, p->partials,
+                     p->tickets + T_OC_FINAL0);
+  AMT_CHECK(check_launch(p, "oc"));
+  p->coeffs_ready = false;
+  if (hs) {
+    HIP_TRY(p, hipMemcpyAsync(hs, p->oc, sizeof(OcState), hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(p, hipStreamSynchronize(p->stream));
+  }
+  return AMT_OK;
+}
+
+int run_iteration(amt_problem* p, amt_iteration_result* out) {
+  amt_iteration_result R;
+  std::memset(&R, 0, sizeof(R));
+  if (!p->loads_set) return fail(p, AMT_EINVAL, "loads not set (amt_set_loads)");
+  if (!p->design_ready) AMT_CHECK(build_design(p));
+  const double t0 = now_s();
+  AMT_CHECK(prepare_coeffs(p));
+  if (p->cfg.verbose) HIP_TRY(p, hipStreamSynchronize(p->stream));
+  const double t1 = now_s();
+  amt_solve_stats ss;
+  std::memset(&ss, 0, sizeof(ss));
+  AMT_CHECK(do_solve(p, &ss));
+  const double t2 = now_s();
+  // compliance with the unmodified forces (topology_optimizer.cpp:338)
+  hipLaunchKernelGGL(k_dot, dim3(nred(p->n_dof)), dim3(kBlock), 0, p->stream, p->u, p->F,
+                     p->n_dof, p->partials, p->tickets + T_DOT, p->scal + S_COMPL);
+  hipLaunchKernelGGL(k_sens, dim3(nblk(p->n_e)), dim3(kBlock), 0, p->stream, p->u, p->rho,
+                     p->Ke64, p->g, p->cfg.penalty, p->dc);
+  AMT_CHECK(check_launch(p, "sensitivities"));
+  if (p->cfg.verbose) HIP_TRY(p, hipStreamSynchronize(p->stream));
+  const double t3 = now_s();
+  hipLaunchKernelGGL(k_filter, dim3(nblk(p->n_e)), dim3(kBlock), 0, p->stream, p->dc, p->rho,
+                     p->stencil, p->stencil_w, p->n_stencil, p->g, p->dcf);
+  OcState hs;
+  AMT_CHECK(oc_update(p, &hs));
+  double compl_ = 0.0;
+  AMT_CHECK(copy_out(p, &compl_, p->scal + S_COMPL, sizeof(double)));
+  const double t4 = now_s();
+  R.compliance = compl_;
+  R.max_density_change = hs.change;
+  R.bisect_passes = hs.passes;
+  R.volume_fraction = p->n_design > 0 ? hs.vol_sum / p->n_design : hs.vol_sum / p->n_e;
+  R.solver_iterations = ss.iterations;
+  R.solver_converged = ss.converged;
+  R.outer_iterations = ss.outer_iterations;
+  R.true_rel_residual = ss.true_rel_residual;
+  R.assembly_time_s = t1 - t0;
+  R.solve_time_s = t2 - t1;
+  R.sensitivity_time_s = t3 - t2;
+  R.update_time_s = t4 - t3;
+  if (!ss.converged && p->cfg.verbose)
+    std::fprintf(stderr, "WARNING: Solver did not converge at iteration!\n");
+  if (out) *out = R;
+  return AMT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void amt_config_default(amt_config* c) {
+  std::memset(c, 0, sizeof(*c));
+  c->nx = c->ny = c->nz = 1;
+  c->element_size = 1.0;
+  c->E = 113.8e9;
+  c->nu = 0.342;
+  c->volume_fraction = 0.25;
+  c->penalty = 3.0;
+  c->filter_radius = 2.0;
+  c->move_limit = 0.2;
+  c->rho_min = 0.01;
+  c->E_min = 1e-6;
+  c->max_iterations = 100;
+  c->convergence_tol = 0.01;
+  c->pcg_tol = 1e-6;
+  c->pcg_maxit = 5000;
+  c->penalty_factor = 1e8;
+  c->density_index = AMT_INDEX_REF_COMPAT;
+  c->precision = AMT_PREC_FP64;
+  c->precond = AMT_PRECOND_JACOBI;
+  c->warm_start = 0;
+  c->inner_tol = 1e-4;
+  c->inner_maxit = 20000;
+  c->verbose = 0;
+}
+
+int amt_element_stiffness(double E, double nu, double h, double* Ke) {
+  if (!Ke) return fail(nullptr, AMT_EINVAL, "null output");
+  if (!(h > 0) || !(E > 0) || !(nu > -1.0 && nu < 0.5))
+    return fail(nullptr, AMT_EINVAL, "invalid element size or material");
+  hex8_stiffness(E, nu, h, Ke);
+  return AMT_OK;
+}
+
+const char* amt_last_error(const amt_problem* p) {
+  return p ? p->err.c_str() : g_create_error.c_str();
+}
+
+int amt_create(const amt_config* cfg, amt_problem** out) {
+  if (!cfg || !out) return fail(nullptr, AMT_EINVAL, "null argument");
+  *out = nullptr;
+  if (cfg->nx <= 0 || cfg->ny <= 0 || cfg->nz <= 0)
+    return fail(nullptr, AMT_EINVAL, "grid dimensions must be positive");
+  if (!(cfg->element_size > 0) || !(cfg->E > 0) || !(cfg->nu > -1.0 && cfg->nu < 0.5))
+    return fail(nullptr, AMT_EINVAL, "invalid element size or material");
+  if (!(cfg->filter_radius >= 0) || cfg->filter_radius > 64)
+    return fail(nullptr, AMT_EINVAL, "filter radius must be in [0, 64]");
+  if (cfg->precond != AMT_PRECOND_JACOBI)
+    return fail(nullptr, AMT_ENOTSUP, "only the Jacobi preconditioner is built in this path");
+  const int64_t n_n = (int64_t)(cfg->nx + 1) * (cfg->ny + 1) * (cfg->nz + 1);
+  if (3 * n_n > (int64_t)INT32_MAX) return fail(nullptr, AMT_EINVAL, "grid too large");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(nullptr, AMT_EHIP, "no HIP device available");
+
+  amt_problem* p = new amt_problem();
+  p->cfg = *cfg;
+  p->g.nx = cfg->nx;
+  p->g.ny = cfg->ny;
+  p->g.nz = cfg->nz;
+  p->g.nny = cfg->ny + 1;
+  p->g.nnz = cfg->nz + 1;
+  p->n_e = (int64_t)cfg->nx * cfg->ny * cfg->nz;
+  p->n_n = n_n;
+  p->n_dof = 3 * n_n;
+  p->g.n_nodes = p->n_n;
+  p->g.n_elem = p->n_e;
+
+  auto bail = [&](int rc) {
+    g_create_error = p->err;
+    amt_destroy(p);
+    return rc;
+  };
+#define CREATE_TRY(x)              \
+  do {                             \
+    int rc_ = (x);                 \
+    if (rc_ != AMT_OK) return bail(rc_); \
+  } while (0)
+
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(p, AMT_EHIP, "hipStreamCreate failed"));
+  const int64_t nd = p->n_dof, ne = p->n_e;
+  CREATE_TRY(dalloc(p, &p->Ke64, 576));
+  CREATE_TRY(dalloc(p, &p->Ke32, 576));
+  CREATE_TRY(dalloc(p, &p->rho, ne));
+  CREATE_TRY(dalloc(p, &p->s64, ne));
+  CREATE_TRY(dalloc(p, &p->s32, ne));
+  CREATE_TRY(dalloc(p, &p->design, ne));
+  CREATE_TRY(dalloc(p, &p->fixcnt, nd));
+  CREATE_TRY(dalloc(p, &p->F, nd));
+  CREATE_TRY(dalloc(p, &p->Fb, nd));
+  CREATE_TRY(dalloc(p, &p->diag, nd));
+  CREATE_TRY(dalloc(p, &p->dinv, nd));
+  CREATE_TRY(dalloc(p, &p->dinv32, nd));
+  CREATE_TRY(dalloc(p, &p->u, nd));
+  CREATE_TRY(dalloc(p, &p->r, nd));
+  CREATE_TRY(dalloc(p, &p->z, nd));
+  CREATE_TRY(dalloc(p, &p->p[0], nd));
+  CREATE_TRY(dalloc(p, &p->p[1], nd));
+  CREATE_TRY(dalloc(p, &p->q, nd));
+  if (cfg->precision == AMT_PREC_MIXED_FP32) {
+    CREATE_TRY(dalloc(p, &p->r32, nd));
+    CREATE_TRY(dalloc(p, &p->z32, nd));
+    CREATE_TRY(dalloc(p, &p->p32[0], nd));
+    CREATE_TRY(dalloc(p, &p->p32[1], nd));
+    CREATE_TRY(dalloc(p, &p->q32, nd));
+    CREATE_TRY(dalloc(p, &p->d32, nd));
+  }
+  CREATE_TRY(dalloc(p, &p->dc, ne));
+  CREATE_TRY(dalloc(p, &p->dcf, ne));
+  const int64_t npart = std::max<int64_t>({(int64_t)kOcCand * kMaxReduceBlocks,
+                                           4 * (int64_t)nblk(p->n_n), 4 * (int64_t)nblk(ne),
+                                           4 * (int64_t)kMaxReduceBlocks});
+  CREATE_TRY(dalloc(p, &p->partials, npart));
+  CREATE_TRY(dalloc(p, &p->tickets, T_COUNT));
+  CREATE_TRY(dalloc(p, &p->scal, S_COUNT));
+  CREATE_TRY(dalloc(p, &p->pcg, 1));
+  CREATE_TRY(dalloc(p, &p->oc, 1));
+
+  std::vector<StencilEntry> st;
+  std::vector<double> sw;
+  build_stencil(cfg->filter_radius, st, sw);
+  p->n_stencil = (int)st.size();
+  CREATE_TRY(dalloc(p, &p->stencil, st.size()));
+  CREATE_TRY(dalloc(p, &p->stencil_w, sw.size()));
+  CREATE_TRY(dalloc(p, &p->dbg_idx, 1));
+  CREATE_TRY(dalloc(p, &p->dbg_w, 1));
+  CREATE_TRY(dalloc(p, &p->dbg_cnt, 1));
+
+  double Ke[576];
+  float Kf[576];
+  hex8_stiffness(cfg->E, cfg->nu, cfg->element_size, Ke);
+  for (int i = 0; i < 576; ++i) Kf[i] = (float)Ke[i];
+  CREATE_TRY(copy_in(p, p->Ke64, Ke, sizeof(Ke)));
+  CREATE_TRY(copy_in(p, p->Ke32, Kf, sizeof(Kf)));
+  if (!st.empty()) {
+    CREATE_TRY(copy_in(p, p->stencil, st.data(), st.size() * sizeof(StencilEntry)));
+    CREATE_TRY(copy_in(p, p->stencil_w, sw.data(), sw.size() * sizeof(double)));
+  }
+  if (hipMemsetAsync(p->tickets, 0, sizeof(unsigned) * T_COUNT, p->stream) != hipSuccess ||
+      hipMemsetAsync(p->fixcnt, 0, nd, p->stream) != hipSuccess ||
+      hipMemsetAsync(p->scal, 0, sizeof(double) * S_COUNT, p->stream) != hipSuccess)
+    return bail(fail(p, AMT_EHIP, "hipMemset failed"));
+  // initial density: uniform volume fraction (topology_optimizer.cpp:403-405)
+  hipLaunchKernelGGL(k_fill<double>, dim3(nblk(ne)), dim3(kBlock), 0, p->stream, p->rho,
+                     cfg->volume_fraction, ne);
+  if (hipStreamSynchronize(p->stream) != hipSuccess)
+    return bail(fail(p, AMT_EHIP, "initialisation failed"));
+#undef CREATE_TRY
+  *out = p;
+  return AMT_OK;
+}
+
+int amt_destroy(amt_problem* p) {
+  if (!p) return AMT_OK;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  for (auto e : p->ev_pool) (void)hipEventDestroy(e);
+  for (void* a : p->allocs) (void)hipFree(a);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return AMT_OK;
+}
+
+int amt_set_loads(amt_problem* p, const double* F, int64_t n_dof, const int64_t* fixed,
+                  int64_t n_fixed) {
+  if (!p) return AMT_EINVAL;
+  if (!F || n_dof != p->n_dof)
+    return fail(p, AMT_EINVAL,
+                "force vector size mismatch: expected " + std::to_string(p->n_dof) + ", got " +
+                    std::to_string(n_dof));
+  if (n_fixed < 0 || (n_fixed > 0 && !fixed)) return fail(p, AMT_EINVAL, "bad fixed list");
+  std::vector<uint8_t> cnt(p->n_dof, 0);
+  for (int64_t t = 0; t < n_fixed; ++t) {
+    const int64_t i = fixed[t];
+    if (i < 0 || i >= p->n_dof)
+      return fail(p, AMT_ERANGE, "Fixed DOF index out of range: " + std::to_string(i));
+    if (cnt[i] == 255) return fail(p, AMT_EINVAL, "DOF listed more than 255 times");
+    cnt[i]++;
+  }
+  for (int64_t i = 0; i < n_dof; ++i)
+    if (!std::isfinite(F[i])) return fail(p, AMT_ENAN, "non-finite force");
+  AMT_CHECK(copy_in(p, p->F, F, sizeof(double) * n_dof));
+  AMT_CHECK(copy_in(p, p->fixcnt, cnt.data(), n_dof));
+  p->loads_set = true;
+  p->coeffs_ready = false;
+  p->have_u = false;
+  return AMT_OK;
+}
+
+int amt_set_density(amt_problem* p, const double* rho, int64_t n_e) {
+  if (!p) return AMT_EINVAL;
+  if (!rho || n_e != p->n_e)
+    return fail(p, AMT_EINVAL,
+                "Density vector size mismatch: expected " + std::to_string(p->n_e) + ", got " +
+                    std::to_string(n_e));
+  for (int64_t i = 0; i < n_e; ++i)
+    if (!std::isfinite(rho[i])) return fail(p, AMT_ENAN, "non-finite density");
+  AMT_CHECK(copy_in(p, p->rho, rho, sizeof(double) * n_e));
+  p->coeffs_ready = false;
+  return AMT_OK;
+}
+
+int amt_get_density(amt_problem* p, double* rho, int64_t n_e) {
+  if (!p) return AMT_EINVAL;
+  if (!rho || n_e != p->n_e) return fail(p, AMT_EINVAL, "density buffer size mismatch");
+  return copy_out(p, rho, p->rho, sizeof(double) * n_e);
+}
+
+int amt_get_displacement(amt_problem* p, double* u, int64_t n_dof) {
+  if (!p) return AMT_EINVAL;
+  if (!u || n_dof != p->n_dof) return fail(p, AMT_EINVAL, "displacement buffer size mismatch");
+  return copy_out(p, u, p->u, sizeof(double) * n_dof);
+}
+
+int amt_begin_run(amt_problem* p) {
+  if (!p) return AMT_EINVAL;
+  return build_design(p);
+}
+
+int amt_run_iteration(amt_problem* p, amt_iteration_result* out) {
+  if (!p) return AMT_EINVAL;
+  return run_iteration(p, out);
+}
+
+int amt_run(amt_problem* p, int max_it, amt_iteration_result* hist, int* iterations,
+            int* converged, double* elapsed_s, amt_iter_cb cb, void* user) {
+  if (!p) return AMT_EINVAL;
+  if (max_it < 0 || (max_it > 0 && !hist)) return fail(p, AMT_EINVAL, "bad history buffer");
+  const double t0 = now_s();
+  AMT_CHECK(build_design(p));
+  int done = 0, conv = 0;
+  const int cap = std::min(max_it, p->cfg.max_iterations);
+  std::vector<double> host_rho;
+  for (int it = 0; it < cap; ++it) {
+    amt_iteration_result R;
+    AMT_CHECK(run_iteration(p, &R));
+    hist[it] = R;
+    done = it + 1;
+    if (p->cfg.verbose)
+      std::printf("%6d %14.4e %10.3f %12.2e %10d\n", it, R.compliance, R.volume_fraction,
+                  R.max_density_change, R.solver_iterations);
+    if (cb) {
+      host_rho.resize(p->n_e);
+      AMT_CHECK(copy_out(p, host_rho.data(), p->rho, sizeof(double) * p->n_e));
+      if (!cb(it, host_rho.data(), p->n_e, &R, user)) break;
+    }
+    if (R.max_density_change < p->cfg.convergence_tol) {
+      conv = 1;
+      break;
+    }
+  }
+  if (iterations) *iterations = done;
+  if (converged) *converged = conv;
+  if (elapsed_s) *elapsed_s = now_s() - t0;
+  return AMT_OK;
+}
+
+int amt_apply_operator(amt_problem* p, const double* x, double* y, int with_bc) {
+  if (!p || !x || !y) return AMT_EINVAL;
+  if (with_bc && !p->loads_set) return fail(p, AMT_EINVAL, "loads not set (amt_set_loads)");
+  if (!p->coeffs_ready) AMT_CHECK(prepare_coeffs(p));
+  AMT_CHECK(copy_in(p, p->z, x, sizeof(double) * p->n_dof));
+  hipLaunchKernelGGL((k_apply<double, double>), dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream,
+                     p->z, p->q, p->s64, p->Ke64, p->fixcnt,
+                     with_bc ? (const double*)(p->scal + S_P) : nullptr, p->g);
+  AMT_CHECK(check_launch(p, "k_apply"));
+  return copy_out(p, y, p->q, sizeof(double) * p->n_dof);
+}
+
+int amt_get_diagonal(amt_problem* p, double* diag, int with_bc, double* penalty) {
+  if (!p || !diag) return AMT_EINVAL;
+  if (!p->coeffs_ready) AMT_CHECK(prepare_coeffs(p));
+  std::vector<double> dinv(p->n_dof);
+  if (with_bc) {
+    AMT_CHECK(copy_out(p, dinv.data(), p->dinv, sizeof(double) * p->n_dof));
+    for (int64_t i = 0; i < p->n_dof; ++i) diag[i] = 1.0 / dinv[i];
+  } else {
+    AMT_CHECK(copy_out(p, diag, p->diag, sizeof(double) * p->n_dof));
+  }
+  if (penalty) AMT_CHECK(copy_out(p, penalty, p->scal + S_P, sizeof(double)));
+  return AMT_OK;
+}
+
+int amt_solve(amt_problem* p, double* u, amt_solve_stats* stats) {
+  if (!p) return AMT_EINVAL;
+  amt_solve_stats st;
+  std::memset(&st, 0, sizeof(st));
+  const double t0 = now_s();
+  AMT_CHECK(do_solve(p, &st));
+  HIP_TRY(p, hipStreamSynchronize(p->stream));
+  st.solve_time_s = now_s() - t0;
+  if (stats) *stats = st;
+  if (u) return copy_out(p, u, p->u, sizeof(double) * p->n_dof);
+  return AMT_OK;
+}
+
+int amt_sensitivities(amt_problem* p, const double* u, double* dc) {
+  if (!p || !u || !dc) return AMT_EINVAL;
+  AMT_CHECK(copy_in(p, p->u, u, sizeof(double) * p->n_dof));
+  p->have_u = true;
+  hipLaunchKernelGGL(k_sens, dim3(nblk(p->n_e)), dim3(kBlock), 0, p->stream, p->u, p->rho,
+                     p->Ke64, p->g, p->cfg.penalty, p->dc);
+  AMT_CHECK(check_launch(p, "k_sens"));
+  return copy_out(p, dc, p->dc, sizeof(double) * p->n_e);
+}
+
+int amt_filter(amt_problem* p, const double* dc, double* dcf) {
+  if (!p || !dc || !dcf) return AMT_EINVAL;
+  AMT_CHECK(copy_in(p, p->dc, dc, sizeof(double) * p->n_e));
+  hipLaunchKernelGGL(k_filter, dim3(nblk(p->n_e)), dim3(kBlock), 0, p->stream, p->dc, p->rho,
+                     p->stencil, p->stencil_w, p->n_stencil, p->g, p->dcf);
+  AMT_CHECK(check_launch(p, "k_filter"));
+  return copy_out(p, dcf, p->dcf, sizeof(double) * p->n_e);
+}
+
+int amt_filter_neighbors(amt_problem* p, int64_t e, int64_t* idx, double* w, int cap,
+                         int* count) {
+  if (!p || !count || cap < 0 || (cap > 0 && (!idx || !w))) return AMT_EINVAL;
+  if (e < 0 || e >= p->n_e) return fail(p, AMT_ERANGE, "element index out of range");
+  if (cap > 0) {
+    // scratch large enough for this request
+    (void)hipFree(p->dbg_idx);
+    (void)hipFree(p->dbg_w);
+    p->allocs.erase(std::remove(p->allocs.begin(), p->allocs.end(), (void*)p->dbg_idx),
+                    p->allocs.end());
+    p->allocs.erase(std::remove(p->allocs.begin(), p->allocs.end(), (void*)p->dbg_w),
+                    p->allocs.end());
+    p->dbg_idx = nullptr;
+    p->dbg_w = nullptr;
+    AMT_CHECK(dalloc(p, &p->dbg_idx, cap));
+    AMT_CHECK(dalloc(p, &p->dbg_w, cap));
+  }
+  hipLaunchKernelGGL(k_filter_list, dim3(1), dim3(64), 0, p->stream, e, p->stencil,
+                     p->stencil_w, p->n_stencil, p->g, p->dbg_idx, p->dbg_w, cap, p->dbg_cnt);
+  AMT_CHECK(check_launch(p, "k_filter_list"));
+  AMT_CHECK(copy_out(p, count, p->dbg_cnt, sizeof(int)));
+  const int m = std::min(cap, *count);
+  if (m > 0) {
+    AMT_CHECK(copy_out(p, idx, p->dbg_idx, sizeof(int64_t) * m));
+    AMT_CHECK(copy_out(p, w, p->dbg_w, sizeof(double) * m));
+  }
+  return AMT_OK;
+}
+
+int amt_oc_update(amt_problem* p, const double* dcf, double* max_change, int* passes) {
+  if (!p || !dcf) return AMT_EINVAL;
+  AMT_CHECK(copy_in(p, p->dcf, dcf, sizeof(double) * p->n_e));
+  OcState hs;
+  AMT_CHECK(oc_update(p, &hs));
+  if (max_change) *max_change = hs.change;
+  if (passes) *passes = hs.passes;
+  return AMT_OK;
+}
+
+int amt_sync(amt_problem* p) {
+  if (!p) return AMT_EINVAL;
+  HIP_TRY(p, hipStreamSynchronize(p->stream));
+  return AMT_OK;
+}
+
+int amt_device_info(char* name, int cap, int* n_cu) {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return fail(nullptr, AMT_EHIP, "no HIP device");
+  if (name && cap > 0) {
+    std::snprintf(name, cap, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (n_cu) *n_cu = prop.multiProcessorCount;
+  return AMT_OK;
+}
+
+int amt_profile_enable(amt_problem* p, int on) {
+  if (!p) return AMT_EINVAL;
+  p->prof_on = on != 0;
+  p->prof_ms = 0.0;
+  p->prof_launches = 0;
+  p->prof_tick = 0;
+  if (p->prof_on && p->ev_pool.empty()) {
+    p->ev_pool.resize(2 * 4096);
+    for (auto& e : p->ev_pool) HIP_TRY(p, hipEventCreate(&e));
+  }
+  return AMT_OK;
+}
+
+int amt_profile_read(amt_problem* p, int64_t* launches, double* total_ms) {
+  if (!p) return AMT_EINVAL;
+  prof_harvest(p);
+  if (launches) *launches = p->prof_launches;
+  if (total_ms) *total_ms = p->prof_ms;
+  return AMT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,746 @@
+// kernels.hpp — gfx950 HIP kernels of the SIMP hot path (device code only).
+//
+// Data layout in HBM (one problem):
+//   nodal vectors  : double/float [3*N_n], DOF = 3*node + c, node z-fastest
+//                    (element_kernel.cpp:186-216)
+//   element scale s: [N_e] in C order (i*ny*nz + j*nz + k), so the eight elements around a
+//                    node are three contiguous pairs along k
+//   slot arrays    : rho, dc, dcf, design mask [N_e] in the optimizer's x-fastest slot order
+//                    (topology_optimizer.hpp:218-232)
+//
+// Reductions are deterministic: each block reduces in a fixed tree order and writes one
+// partial; the last block to finish (atomic ticket) sums the partials in a fixed order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace amt {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+// element_kernel.cpp:196-199: local node offsets of the hex8 element
+__device__ constexpr int kOX[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+__device__ constexpr int kOY[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+__device__ constexpr int kOZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+
+struct GridDims {
+  int nx, ny, nz;  // elements
+  int nny, nnz;    // ny+1, nz+1
+  int64_t n_nodes;
+  int64_t n_elem;
+};
+
+// Device-resident solver state: the host never needs these values to keep iterating.
+struct PcgState {
+  double rz, alpha, beta, rr, rhs2, thresh, pq, pad0;
+  int it, done, converged, maxit;
+};
+
+struct OcState {
+  double l1, l2, lmid, vf, n_design, change, vol_sum, pad;
+  int passes, done, pad1, pad2;
+};
+
+// ---------------------------------------------------------------------------------------
+// Deterministic reductions
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Block-reduces NV values (sum, or max when IS_MAX) and writes partials[v*nblk + blockIdx.x].
+// Returns true in every thread of the block that finished last; that block must then call
+// finish_partials. All threads of the block must call this.
+template <int NV, bool IS_MAX = false>
+__device__ bool block_partials(double (&v)[NV], double* partials, unsigned* ticket) {
+  __shared__ double red[NV][kWaves];
+  __shared__ bool last;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double t = IS_MAX ? wave_max(v[q]) : wave_sum(v[q]);
+    if (lane == 0) red[q][w] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double t = red[q][0];
+#pragma unroll
+      for (int k = 1; k < kWaves; ++k) t = IS_MAX ? fmax(t, red[q][k]) : t + red[q][k];
+      partials[(size_t)q * gridDim.x + blockIdx.x] = t;
+    }
+    __threadfence();
+    const unsigned prev = atomicAdd(ticket, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  return last;
+}
+
+// Called by all threads of the last block: out[q] = fixed-order reduction of the partials.
+template <int NV, bool IS_MAX = false>
+__device__ void finish_partials(const double* partials, unsigned* ticket, double (&out)[NV]) {
+  __shared__ double red[NV][kWaves];
+  __threadfence();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nb = gridDim.x;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double t = IS_MAX ? 0.0 : 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) {
+      const double x = __hip_atomic_load(partials + (size_t)q * nb + b, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+      t = IS_MAX ? fmax(t, x) : t + x;
+    }
+    t = IS_MAX ? wave_max(t) : wave_sum(t);
+    if (lane == 0) red[q][w] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double t = red[q][0];
+#pragma unroll
+    for (int k = 1; k < kWaves; ++k) t = IS_MAX ? fmax(t, red[q][k]) : t + red[q][k];
+    out[q] = t;
+  }
+  if (threadIdx.x == 0) atomicExch(ticket, 0u);
+}
+
+// ---------------------------------------------------------------------------------------
+// SIMP element scale (global_assembler.cpp:141-158 + global_assembler.hpp:171-173)
+// ---------------------------------------------------------------------------------------
+// s[C-order (i,j,k)] = 0 if rho < rho_min else Emin + rho^p (1-Emin), with rho read at
+// C-order (reference) or x-fastest (consistent) position of element (i,j,k).
+__global__ void k_simp(const double* __restrict__ rho, double* __restrict__ s64,
+                       float* __restrict__ s32, GridDims g, int ref_compat, double penal,
+                       double Emin, double rho_min) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= g.n_elem) return;
+  int64_t src = e;
+  if (!ref_compat) {
+    const int k = (int)(e % g.nz);
+    const int64_t t = e / g.nz;
+    const int j = (int)(t % g.ny);
+    const int i = (int)(t / g.ny);
+    src = (int64_t)k * g.nx * g.ny + (int64_t)j * g.nx + i;
+  }
+  const double r = rho[src];
+  const double s = (r < rho_min) ? 0.0 : Emin + pow(r, penal) * (1.0 - Emin);
+  s64[e] = s;
+  s32[e] = (float)s;
+}
+
+// ---------------------------------------------------------------------------------------
+// Matrix-free hex8 operator, node-centric gather.
+//
+// One thread owns one node n=(i,j,k) and its three rows. It holds the 27 neighbour nodal
+// vectors in registers and, for each of the (up to) eight elements e around n (n at local
+// position a in e), adds s_e * Ke0[3a+c, :] . x_e. Ke0 is read through uniform (scalar)
+// loads. The sum over elements runs in a fixed order, so the result is deterministic.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+struct Nbr {
+  T v[27][3];
+};
+
+__device__ __forceinline__ void node_decode(int64_t n, const GridDims& g, int& i, int& j,
+                                            int& k) {
+  k = (int)(n % g.nnz);
+  const int64_t t = n / g.nnz;
+  j = (int)(t % g.nny);
+  i = (int)(t / g.nny);
+}
+
+template <typename T>
+__device__ __forceinline__ void load_element_scales(const T* __restrict__ s, const GridDims& g,
+                                                    int i, int j, int k, T (&s8)[8]) {
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int ei = i - kOX[a], ej = j - kOY[a], ek = k - kOZ[a];
+    const bool ok = ei >= 0 && ei < g.nx && ej >= 0 && ej < g.ny && ek >= 0 && ek < g.nz;
+    s8[a] = ok ? s[((int64_t)ei * g.ny + ej) * g.nz + ek] : T(0);
+  }
+}
+
+// y_n = sum_a s8[a] * Ke0[3a.., :] x_e(a)
+template <typename T, typename TK>
+__device__ __forceinline__ void apply_node(const Nbr<T>& nb, const T (&s8)[8],
+                                           const TK* __restrict__ Ke, T (&y)[3]) {
+  y[0] = y[1] = y[2] = T(0);
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    T t0 = T(0), t1 = T(0), t2 = T(0);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int di = kOX[b] - kOX[a] + 1, dj = kOY[b] - kOY[a] + 1, dk = kOZ[b] - kOZ[a] + 1;
+      const int m = di * 9 + dj * 3 + dk;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const T xv = nb.v[m][d];
+        t0 = fma((T)Ke[(3 * a + 0) * 24 + 3 * b + d], xv, t0);
+        t1 = fma((T)Ke[(3 * a + 1) * 24 + 3 * b + d], xv, t1);
+        t2 = fma((T)Ke[(3 * a + 2) * 24 + 3 * b + d], xv, t2);
+      }
+    }
+    y[0] = fma(s8[a], t0, y[0]);
+    y[1] = fma(s8[a], t1, y[1]);
+    y[2] = fma(s8[a], t2, y[2]);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void gather_nbr(const T* __restrict__ x, const GridDims& g,
+                                           int64_t n, int i, int j, int k, Nbr<T>& nb) {
+  const int64_t sj = g.nnz, si = (int64_t)g.nny * g.nnz;
+#pragma unroll
+  for (int di = -1; di <= 1; ++di)
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+      for (int dk = -1; dk <= 1; ++dk) {
+        const int m = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
+        const bool ok = (i + di) >= 0 && (i + di) <= g.nx && (j + dj) >= 0 &&
+                        (j + dj) <= g.ny && (k + dk) >= 0 && (k + dk) <= g.nz;
+        const int64_t q = 3 * (n + di * si + dj * sj + dk);
+        nb.v[m][0] = ok ? x[q + 0] : T(0);
+        nb.v[m][1] = ok ? x[q + 1] : T(0);
+        nb.v[m][2] = ok ? x[q + 2] : T(0);
+      }
+}
+
+// Same gather, but the value read is p = z + beta*p_old (the CG direction update done on
+// the fly, so the next operator launch needs no separate p-update kernel). Every thread
+// evaluates the identical fma for a given DOF, so the owner writes exactly what its
+// neighbours used.
+template <typename T>
+__device__ __forceinline__ void gather_dir(const T* __restrict__ z, const T* __restrict__ pold,
+                                           T beta, const GridDims& g, int64_t n, int i, int j,
+                                           int k, Nbr<T>& nb) {
+  const int64_t sj = g.nnz, si = (int64_t)g.nny * g.nnz;
+#pragma unroll
+  for (int di = -1; di <= 1; ++di)
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+      for (int dk = -1; dk <= 1; ++dk) {
+        const int m = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
+        const bool ok = (i + di) >= 0 && (i + di) <= g.nx && (j + dj) >= 0 &&
+                        (j + dj) <= g.ny && (k + dk) >= 0 && (k + dk) <= g.nz;
+        const int64_t q = 3 * (n + di * si + dj * sj + dk);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nb.v[m][c] = ok ? fma(beta, pold[q + c], z[q + c]) : T(0);
+      }
+}
+
+// y = K x (+ penalty rows when P != null): fixcnt[dof] * P * x[dof]
+template <typename T, typename TK>
+__global__ __launch_bounds__(kBlock) void k_apply(const T* __restrict__ x, T* __restrict__ y,
+                                                  const T* __restrict__ s,
+                                                  const TK* __restrict__ Ke,
+                                                  const uint8_t* __restrict__ fixcnt,
+                                                  const double* __restrict__ P, GridDims g) {
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  int i, j, k;
+  node_decode(n, g, i, j, k);
+  Nbr<T> nb;
+  gather_nbr(x, g, n, i, j, k, nb);
+  T s8[8];
+  load_element_scales(s, g, i, j, k, s8);
+  T yv[3];
+  apply_node(nb, s8, Ke, yv);
+  if (P) {
+    const T pen = (T)(*P);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) yv[c] = fma((T)fixcnt[3 * n + c] * pen, nb.v[13][c], yv[c]);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) y[3 * n + c] = yv[c];
+}
+
+// r = b - K x (with penalty), partial |r|^2 -> out[0] via the last block.
+__global__ __launch_bounds__(kBlock) void k_residual(const double* __restrict__ x,
+                                                     const double* __restrict__ b,
+                                                     double* __restrict__ r,
+                                                     const double* __restrict__ s,
+                                                     const double* __restrict__ Ke,
+                                                     const uint8_t* __restrict__ fixcnt,
+                                                     const double* __restrict__ P, GridDims g,
+                                                     double* partials, unsigned* ticket,
+                                                     double* out) {
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double acc[1] = {0.0};
+  if (n < g.n_nodes) {
+    int i, j, k;
+    node_decode(n, g, i, j, k);
+    Nbr<double> nb;
+    gather_nbr(x, g, n, i, j, k, nb);
+    double s8[8];
+    load_element_scales(s, g, i, j, k, s8);
+    double yv[3];
+    apply_node(nb, s8, Ke, yv);
+    const double pen = *P;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      yv[c] = fma((double)fixcnt[3 * n + c] * pen, nb.v[13][c], yv[c]);
+      const double rv = b[3 * n + c] - yv[c];
+      r[3 * n + c] = rv;
+      acc[0] = fma(rv, rv, acc[0]);
+    }
+  }
+  if (block_partials<1>(acc, partials, ticket)) {
+    double tot[1];
+    finish_partials<1>(partials, ticket, tot);
+    if (threadIdx.x == 0) out[0] = tot[0];
+  }
+}
+
+// Diagonal of K without penalty: d = sum_a s8[a] Ke0[3a+c,3a+c]; partial max|d|.
+__global__ __launch_bounds__(kBlock) void k_diag(double* __restrict__ diag,
+                                                 const double* __restrict__ s,
+                                                 const double* __restrict__ Ke, GridDims g,
+                                                 double* partials, unsigned* ticket,
+                                                 double* maxdiag) {
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double m[1] = {0.0};
+  if (n < g.n_nodes) {
+    int i, j, k;
+    node_decode(n, g, i, j, k);
+    double s8[8];
+    load_element_scales(s, g, i, j, k, s8);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double d = 0.0;
+#pragma unroll
+      for (int a = 0; a < 8; ++a) d = fma(s8[a], Ke[(3 * a + c) * 24 + 3 * a + c], d);
+      diag[3 * n + c] = d;
+      m[0] = fmax(m[0], fabs(d));
+    }
+  }
+  if (block_partials<1, true>(m, partials, ticket)) {
+    double tot[1];
+    finish_partials<1, true>(partials, ticket, tot);
+    if (threadIdx.x == 0) maxdiag[0] = tot[0];
+  }
+}
+
+// P = pf * max|K_ii| (linear_solver.cpp:46-47); Jacobi inverse of the penalised diagonal.
+// A zero diagonal gets 1, as Eigen's DiagonalPreconditioner does; such rows carry r = 0.
+__global__ void k_dinv(const double* __restrict__ diag, const uint8_t* __restrict__ fixcnt,
+                       const double* __restrict__ maxdiag, double pf, double* __restrict__ P,
+                       double* __restrict__ dinv, float* __restrict__ dinv32, int64_t n_dof) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const double pen = pf * maxdiag[0];
+  if (i == 0) P[0] = pen;
+  if (i >= n_dof) return;
+  double d = diag[i];
+  for (int t = 0; t < fixcnt[i]; ++t) d += pen;  // one K_ii += P per listed occurrence
+  const double v = (d != 0.0) ? 1.0 / d : 1.0;
+  dinv[i] = v;
+  dinv32[i] = (float)v;
+}
+
+// F_bc: forces with fixed DOFs zeroed (linear_solver.cpp:83).
+__global__ void k_bc_rhs(const double* __restrict__ F, const uint8_t* __restrict__ fixcnt,
+                         double* __restrict__ Fb, int64_t n_dof) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n_dof) return;
+  Fb[i] = fixcnt[i] ? 0.0 : F[i];
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ v, T val, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) v[i] = val;
+}
+
+// Grid-stride helper with a fixed grid: element i of thread t is t + q*stride.
+#define AMT_GRID_STRIDE(i, n) \
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < (n); i += (int64_t)gridDim.x * kBlock)
+
+// ---------------------------------------------------------------------------------------
+// PCG (Eigen conjugate_gradient loop with a Jacobi preconditioner)
+// ---------------------------------------------------------------------------------------
+
+// z = Dinv r; |b|^2, |r|^2, r.z -> state (Eigen: rhsNorm2, residualNorm2, absNew).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_init(const double* __restrict__ b,
+                                                     const T* __restrict__ r,
+                                                     T* __restrict__ z,
+                                                     const T* __restrict__ dinv, int64_t n,
+                                                     double tol, int maxit, PcgState* st,
+                                                     double* partials, unsigned* ticket) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  AMT_GRID_STRIDE(i, n) {
+    const T rv = r[i];
+    const T zv = dinv[i] * rv;
+    z[i] = zv;
+    if (b) acc[0] = fma(b[i], b[i], acc[0]);
+    acc[1] = fma((double)rv, (double)rv, acc[1]);
+    acc[2] = fma((double)rv, (double)zv, acc[2]);
+  }
+  if (block_partials<3>(acc, partials, ticket)) {
+    double tot[3];
+    finish_partials<3>(partials, ticket, tot);
+    if (threadIdx.x == 0) {
+      // without b (inner solve of a refinement step) the tolerance is relative to |r0|
+      const double bb = b ? tot[0] : tot[1];
+      st->rhs2 = bb;
+      st->thresh = fmax(tol * tol * bb, 2.2250738585072014e-308);
+      st->rr = tot[1];
+      st->rz = tot[2];
+      st->beta = 0.0;
+      st->alpha = 0.0;
+      st->it = 0;
+      st->maxit = maxit;
+      const int conv = (bb == 0.0) || (tot[1] < st->thresh);
+      st->done = conv || maxit <= 0;
+      st->converged = conv;
+    }
+  }
+}
+
+// p = z + beta p_old (gathered on the fly), q = K p, partial p.q -> alpha = rz / pq.
+template <typename T, typename TK>
+__global__ __launch_bounds__(kBlock) void k_pcg_apply(
+    const T* __restrict__ z, const T* __restrict__ pold, T* __restrict__ pnew,
+    T* __restrict__ q, const T* __restrict__ s, const TK* __restrict__ Ke,
+    const uint8_t* __restrict__ fixcnt, const double* __restrict__ P, GridDims g,
+    PcgState* st, double* partials, unsigned* ticket) {
+  if (st->done) return;
+  const T beta = (T)st->beta;
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double acc[1] = {0.0};
+  if (n < g.n_nodes) {
+    int i, j, k;
+    node_decode(n, g, i, j, k);
+    Nbr<T> nb;
+    gather_dir(z, pold, beta, g, n, i, j, k, nb);
+    T s8[8];
+    load_element_scales(s, g, i, j, k, s8);
+    T yv[3];
+    apply_node(nb, s8, Ke, yv);
+    const T pen = (T)(*P);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      yv[c] = fma((T)fixcnt[3 * n + c] * pen, nb.v[13][c], yv[c]);
+      pnew[3 * n + c] = nb.v[13][c];
+      q[3 * n + c] = yv[c];
+      acc[0] = fma((double)nb.v[13][c], (double)yv[c], acc[0]);
+    }
+  }
+  if (block_partials<1>(acc, partials, ticket)) {
+    double tot[1];
+    finish_partials<1>(partials, ticket, tot);
+    if (threadIdx.x == 0) {
+      st->pq = tot[0];
+      st->alpha = st->rz / tot[0];
+    }
+  }
+}
+
+// x += alpha p; r -= alpha q; |r|^2 test; z = Dinv r; beta = r.z / rz_old.
+template <typename T, typename TX>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(TX* __restrict__ x, T* __restrict__ r,
+                                                       T* __restrict__ z,
+                                                       const T* __restrict__ p,
+                                                       const T* __restrict__ q,
+                                                       const T* __restrict__ dinv, int64_t n,
+                                                       PcgState* st, double* partials,
+                                                       unsigned* ticket) {
+  if (st->done) return;
+  const double alpha = st->alpha;
+  const T a = (T)alpha;
+  double acc[2] = {0.0, 0.0};
+  AMT_GRID_STRIDE(i, n) {
+    const T pv = p[i];
+    x[i] = x[i] + (TX)(a * pv);
+    const T rv = r[i] - a * q[i];
+    r[i] = rv;
+    const T zv = dinv[i] * rv;
+    z[i] = zv;
+    acc[0] = fma((double)rv, (double)rv, acc[0]);
+    acc[1] = fma((double)rv, (double)zv, acc[1]);
+  }
+  if (block_partials<2>(acc, partials, ticket)) {
+    double tot[2];
+    finish_partials<2>(partials, ticket, tot);
+    if (threadIdx.x == 0) {
+      st->rr = tot[0];
+      if (tot[0] < st->thresh) {
+        st->done = 1;
+        st->converged = 1;
+      } else {
+        st->beta = tot[1] / st->rz;
+        st->rz = tot[1];
+        st->it += 1;
+        if (st->it >= st->maxit) st->done = 1;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Dot product (compliance u.F, topology_optimizer.hpp:273-278)
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_dot(const double* __restrict__ a,
+                                                const double* __restrict__ b, int64_t n,
+                                                double* partials, unsigned* ticket,
+                                                double* out) {
+  double acc[1] = {0.0};
+  AMT_GRID_STRIDE(i, n) acc[0] = fma(a[i], b[i], acc[0]);
+  if (block_partials<1>(acc, partials, ticket)) {
+    double tot[1];
+    finish_partials<1>(partials, ticket, tot);
+    if (threadIdx.x == 0) out[0] = tot[0];
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Sensitivities (topology_optimizer.cpp:126-161): per slot e (x-fastest decode),
+// ce = u_e^T Ke0 u_e, dc = -p rho^(p-1) ce.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_sens(const double* __restrict__ u,
+                                                 const double* __restrict__ rho,
+                                                 const double* __restrict__ Ke, GridDims g,
+                                                 double penal, double* __restrict__ dc) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= g.n_elem) return;
+  const int64_t nxy = (int64_t)g.nx * g.ny;
+  const int iz = (int)(e / nxy);
+  const int64_t rem = e % nxy;
+  const int iy = (int)(rem / g.nx), ix = (int)(rem % g.nx);
+  double ue[24];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const int64_t node = ((int64_t)(ix + kOX[b]) * g.nny + (iy + kOY[b])) * g.nnz + (iz + kOZ[b]);
+    ue[3 * b + 0] = u[3 * node + 0];
+    ue[3 * b + 1] = u[3 * node + 1];
+    ue[3 * b + 2] = u[3 * node + 2];
+  }
+  double ce = 0.0;
+#pragma unroll
+  for (int b = 0; b < 24; ++b) {
+    double t = 0.0;
+#pragma unroll
+    for (int a = 0; a < 24; ++a) t = fma(ue[a], Ke[a * 24 + b], t);
+    ce = fma(t, ue[b], ce);
+  }
+  dc[e] = -penal * pow(rho[e], penal - 1.0) * ce;
+}
+
+// ---------------------------------------------------------------------------------------
+// Sensitivity filter (topology_optimizer.cpp:60-119 lists, :168-198 apply).
+// The neighbour list of slot e is the interior stencil table (built on the host in
+// dz, dy, dx order with the reference's sqrt/weight arithmetic) with out-of-grid entries
+// skipped, which is exactly the list buildFilter stores.
+// ---------------------------------------------------------------------------------------
+struct StencilEntry {
+  int dx, dy, dz, pad;
+};
+
+__global__ __launch_bounds__(kBlock) void k_filter(const double* __restrict__ dc,
+                                                   const double* __restrict__ rho,
+                                                   const StencilEntry* __restrict__ st,
+                                                   const double* __restrict__ w, int n_st,
+                                                   GridDims g, double* __restrict__ dcf) {
+#pragma clang fp contract(off)
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= g.n_elem) return;
+  const int64_t nxy = (int64_t)g.nx * g.ny;
+  const int iz = (int)(e / nxy);
+  const int64_t rem = e % nxy;
+  const int iy = (int)(rem / g.nx), ix = (int)(rem % g.nx);
+  double num = 0.0, den = 0.0;
+  for (int t = 0; t < n_st; ++t) {
+    const StencilEntry o = st[t];
+    const int cx = ix + o.dx, cy = iy + o.dy, cz = iz + o.dz;
+    if (cx < 0 || cx >= g.nx || cy < 0 || cy >= g.ny || cz < 0 || cz >= g.nz) continue;
+    const int64_t m = (int64_t)cz * nxy + (int64_t)cy * g.nx + cx;
+    const double wr = w[t] * rho[m];
+    num = num + wr * dc[m];
+    den = den + wr;
+  }
+  dcf[e] = den > 1e-15 ? num / den : dc[e];
+}
+
+// Writes the neighbour list of slot e exactly as k_filter walks it.
+__global__ void k_filter_list(int64_t e, const StencilEntry* __restrict__ st,
+                              const double* __restrict__ w, int n_st, GridDims g,
+                              int64_t* __restrict__ idx, double* __restrict__ wout, int cap,
+                              int* count) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int64_t nxy = (int64_t)g.nx * g.ny;
+  const int iz = (int)(e / nxy);
+  const int64_t rem = e % nxy;
+  const int iy = (int)(rem / g.nx), ix = (int)(rem % g.nx);
+  int c = 0;
+  for (int t = 0; t < n_st; ++t) {
+    const StencilEntry o = st[t];
+    const int cx = ix + o.dx, cy = iy + o.dy, cz = iz + o.dz;
+    if (cx < 0 || cx >= g.nx || cy < 0 || cy >= g.ny || cz < 0 || cz >= g.nz) continue;
+    if (c < cap) {
+      idx[c] = (int64_t)cz * nxy + (int64_t)cy * g.nx + cx;
+      wout[c] = w[t];
+    }
+    ++c;
+  }
+  *count = c;
+}
+
+// ---------------------------------------------------------------------------------------
+// Optimality Criteria (topology_optimizer.cpp:205-287)
+//
+// The reference runs up to 100 dependent bisection passes, each a full reduction. Here one
+// launch evaluates the next LEV levels of the bisection tree at once: the 2^LEV-1 midpoints
+// every path can visit, computed with the same (l1+l2)/2 arithmetic. The last block then
+// walks the tree with the reference's break test, so the visited lambdas and the l1/l2
+// updates are the reference's, in ~LEV times fewer grid-wide reductions.
+// ---------------------------------------------------------------------------------------
+// std::max / std::min semantics: max(a,b) = (a<b)?b:a, min(a,b) = (b<a)?b:a (NaN rules kept)
+__device__ __forceinline__ double std_max(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double std_min(double a, double b) { return (b < a) ? b : a; }
+
+__device__ __forceinline__ double oc_candidate(double r, double dcf, double lmid, double move,
+                                               double rho_min) {
+  const double Be = sqrt(-dcf / (lmid * 1.0 + 1e-20));
+  const double c = r * Be;
+  return std_max(rho_min, std_max(r - move, std_min(1.0, std_min(r + move, c))));
+}
+
+template <int LEV>
+__device__ __forceinline__ void oc_tree(double l1, double l2, double (&lam)[1 << LEV]) {
+  // heap order: node 1 = [l1,l2]; node 2j = left half (l2 := mid), 2j+1 = right (l1 := mid)
+  double lo[1 << LEV], hi[1 << LEV];
+  lo[1] = l1;
+  hi[1] = l2;
+#pragma unroll
+  for (int nd = 1; nd < (1 << LEV); ++nd) {
+    lam[nd] = 0.5 * (lo[nd] + hi[nd]);
+    if (2 * nd < (1 << LEV)) {
+      lo[2 * nd] = lo[nd];
+      hi[2 * nd] = lam[nd];
+      lo[2 * nd + 1] = lam[nd];
+      hi[2 * nd + 1] = hi[nd];
+    }
+  }
+  lam[0] = 0.0;
+}
+
+template <int LEV>
+__global__ __launch_bounds__(kBlock) void k_oc_sweep(const double* __restrict__ rho,
+                                                     const double* __restrict__ dcf,
+                                                     const uint8_t* __restrict__ design,
+                                                     int64_t n, double move, double rho_min,
+                                                     OcState* st, double* partials,
+                                                     unsigned* ticket) {
+  if (st->done) return;
+  constexpr int NC = (1 << LEV) - 1;
+  double lam[1 << LEV];
+  oc_tree<LEV>(st->l1, st->l2, lam);
+  double vol[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) vol[c] = 0.0;
+  AMT_GRID_STRIDE(e, n) {
+    if (!design[e]) continue;
+    const double r = rho[e], d = dcf[e];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) vol[c] += oc_candidate(r, d, lam[c + 1], move, rho_min);
+  }
+  if (block_partials<NC>(vol, partials, ticket)) {
+    double tot[NC];
+    finish_partials<NC>(partials, ticket, tot);
+    if (threadIdx.x == 0) {
+      double l1 = st->l1, l2 = st->l2;
+      int nd = 1, passes = st->passes, done = 0;
+      for (int lev = 0; lev < LEV; ++lev) {
+        if (passes >= 100 || (l2 - l1) / (l1 + l2) < 1e-4) {
+          done = 1;
+          break;
+        }
+        const double lm = 0.5 * (l1 + l2);  // == lam[nd]
+        st->lmid = lm;
+        ++passes;
+        if (tot[nd - 1] / st->n_design > st->vf) {
+          l1 = lm;
+          nd = 2 * nd + 1;
+        } else {
+          l2 = lm;
+          nd = 2 * nd;
+        }
+      }
+      if (!done && (passes >= 100 || (l2 - l1) / (l1 + l2) < 1e-4)) done = 1;
+      st->l1 = l1;
+      st->l2 = l2;
+      st->passes = passes;
+      st->done = done;
+    }
+  }
+}
+
+// rho <- candidate at the last evaluated lambda (design) / unchanged (non-design);
+// max|drho| and the design-space volume sum (topology_optimizer.cpp:274-286, :364-381).
+__global__ __launch_bounds__(kBlock) void k_oc_final(double* __restrict__ rho,
+                                                     const double* __restrict__ dcf,
+                                                     const uint8_t* __restrict__ design,
+                                                     int64_t n, double move, double rho_min,
+                                                     OcState* st, double* partials,
+                                                     unsigned* ticket) {
+  const double lm = st->lmid;
+  const int any = st->passes > 0;
+  double acc[1] = {0.0}, mx[1] = {0.0};
+  AMT_GRID_STRIDE(e, n) {
+    const double r = rho[e];
+    double rn = r;
+    if (design[e]) {
+      rn = any ? oc_candidate(r, dcf[e], lm, move, rho_min) : 0.0;
+      acc[0] += rn;
+    }
+    mx[0] = fmax(mx[0], fabs(rn - r));
+    rho[e] = rn;
+  }
+  // two reductions share one ticket: first the max, then the sum
+  if (block_partials<1, true>(mx, partials, ticket)) {
+    double t[1];
+    finish_partials<1, true>(partials, ticket, t);
+    if (threadIdx.x == 0) st->change = t[0];
+  }
+  __syncthreads();
+  if (block_partials<1>(acc, partials + gridDim.x, ticket + 1)) {
+    double t[1];
+    finish_partials<1>(partials + gridDim.x, ticket + 1, t);
+    if (threadIdx.x == 0) st->vol_sum = t[0];
+  }
+}
+
+// Design mask from the initial density (topology_optimizer.cpp:419-428); count via partials.
+__global__ __launch_bounds__(kBlock) void k_design_mask(const double* __restrict__ rho,
+                                                        uint8_t* __restrict__ design,
+                                                        int64_t n, double rho_min,
+                                                        double* partials, unsigned* ticket,
+                                                        double* count) {
+  double acc[1] = {0.0};
+  AMT_GRID_STRIDE(e, n) {
+    const double r = rho[e];
+    const uint8_t d = (r > rho_min + 1e-6) && (r < 0.99);
+    design[e] = d;
+    acc[0] += d;
+  }
+  if (block_partials<1>(acc, partials, ticket)) {
+    double t[1];
+    finish_partials<1>(partials, ticket, t);
+    if (threadIdx.x == 0) count[0] = t[0];
+  }
+}
+
+}  // namespace amt

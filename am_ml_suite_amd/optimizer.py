@@ -1,0 +1,349 @@
+"""Python mirror of the reference optimizer API over libamt.so.
+
+Names follow the reference's C++ types (topology_optimizer.hpp:49-279,
+element_kernel.hpp:61-87) and the Python binding the reference notebook imports
+(notebooks/visualize_results.ipynb cell 23 lines 9-12: CPPTopologyOptimizer, SIMPConfig,
+MaterialProperties, get_max_threads, is_available; run / run_iteration, cell 23-24).
+Every call goes to the GPU through the C-ABI; nothing here computes on the CPU.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional
+
+import numpy as np
+
+from . import _ffi
+
+
+class AmtError(RuntimeError):
+    pass
+
+
+def _raise(code, msg):
+    if code in (_ffi.AMT_EINVAL, _ffi.AMT_ENAN):
+        raise ValueError(msg)  # std::invalid_argument
+    if code == _ffi.AMT_ERANGE:
+        raise IndexError(msg)  # std::out_of_range
+    if code == _ffi.AMT_ENOTSUP:
+        raise NotImplementedError(msg)
+    raise AmtError(msg)
+
+
+@dataclass
+class MaterialProperties:
+    """element_kernel.hpp:61-87 (Ti6Al4V defaults)."""
+    E: float = 113.8e9
+    nu: float = 0.342
+    rho: float = 4430.0
+
+    def lam(self):
+        return self.E * self.nu / ((1.0 + self.nu) * (1.0 - 2.0 * self.nu))
+
+    def mu(self):
+        return self.E / (2.0 * (1.0 + self.nu))
+
+
+@dataclass
+class SIMPConfig:
+    """topology_optimizer.hpp:49-59 plus the solver settings topology_optimizer.cpp:35-40
+    fixes, and the device options."""
+    volume_fraction: float = 0.25
+    penalty: float = 3.0
+    filter_radius: float = 2.0
+    move_limit: float = 0.2
+    rho_min: float = 0.01
+    E_min: float = 1e-6
+    max_iterations: int = 100
+    convergence_tol: float = 0.01
+    verbose: bool = True
+    solver_tolerance: float = 1e-6
+    solver_max_iterations: int = 5000
+    penalty_factor: float = 1e8
+    density_index: str = "reference"  # or "consistent"
+    precision: str = "fp64"           # or "mixed_fp32"
+    warm_start: bool = False
+    inner_tolerance: float = 1e-4
+    inner_max_iterations: int = 20000
+
+
+@dataclass
+class IterationResult:
+    """topology_optimizer.hpp:65-74 (+ solver detail)."""
+    compliance: float = 0.0
+    volume_fraction: float = 0.0
+    max_density_change: float = 0.0
+    solver_iterations: int = 0
+    assembly_time_s: float = 0.0
+    solve_time_s: float = 0.0
+    sensitivity_time_s: float = 0.0
+    update_time_s: float = 0.0
+    solver_converged: bool = False
+    outer_iterations: int = 0
+    true_rel_residual: float = 0.0
+    bisect_passes: int = 0
+
+
+@dataclass
+class OptimizationResult:
+    """topology_optimizer.hpp:80-87."""
+    final_density: np.ndarray = None
+    compliance_history: List[float] = field(default_factory=list)
+    volume_history: List[float] = field(default_factory=list)
+    iterations: int = 0
+    converged: bool = False
+    elapsed_time_s: float = 0.0
+    iteration_results: List[IterationResult] = field(default_factory=list)
+
+
+def _convert(r):
+    return IterationResult(
+        compliance=r.compliance, volume_fraction=r.volume_fraction,
+        max_density_change=r.max_density_change, solver_iterations=r.solver_iterations,
+        assembly_time_s=r.assembly_time_s, solve_time_s=r.solve_time_s,
+        sensitivity_time_s=r.sensitivity_time_s, update_time_s=r.update_time_s,
+        solver_converged=bool(r.solver_converged), outer_iterations=r.outer_iterations,
+        true_rel_residual=r.true_rel_residual, bisect_passes=r.bisect_passes)
+
+
+def make_amt_config(nx, ny, nz, element_size, material, config):
+    L = _ffi.load()
+    c = _ffi.AmtConfig()
+    L.amt_config_default(C.byref(c))
+    c.nx, c.ny, c.nz = int(nx), int(ny), int(nz)
+    c.element_size = float(element_size)
+    c.E, c.nu = float(material.E), float(material.nu)
+    c.volume_fraction = config.volume_fraction
+    c.penalty = config.penalty
+    c.filter_radius = config.filter_radius
+    c.move_limit = config.move_limit
+    c.rho_min = config.rho_min
+    c.E_min = config.E_min
+    c.max_iterations = int(config.max_iterations)
+    c.convergence_tol = config.convergence_tol
+    c.pcg_tol = config.solver_tolerance
+    c.pcg_maxit = int(config.solver_max_iterations)
+    c.penalty_factor = config.penalty_factor
+    c.density_index = {"reference": _ffi.AMT_INDEX_REF_COMPAT,
+                       "consistent": _ffi.AMT_INDEX_CONSISTENT}[config.density_index]
+    c.precision = {"fp64": _ffi.AMT_PREC_FP64,
+                   "mixed_fp32": _ffi.AMT_PREC_MIXED_FP32}[config.precision]
+    c.precond = _ffi.AMT_PRECOND_JACOBI
+    c.warm_start = int(bool(config.warm_start))
+    c.inner_tol = config.inner_tolerance
+    c.inner_maxit = int(config.inner_max_iterations)
+    c.verbose = 0
+    return c
+
+
+class TopologyOptimizer:
+    """topology_optimizer.hpp:120-279 on one GPU."""
+
+    def __init__(self, nx, ny, nz, element_size, material=None, config=None):
+        self._L = _ffi.load()
+        self.material = material or MaterialProperties()
+        self.config = config or SIMPConfig()
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.n_elements = self.nx * self.ny * self.nz
+        self.n_dofs = 3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
+        self._h = C.c_void_p()
+        cfg = make_amt_config(nx, ny, nz, element_size, self.material, self.config)
+        rc = self._L.amt_create(C.byref(cfg), C.byref(self._h))
+        if rc != _ffi.AMT_OK:
+            _raise(rc, self._L.amt_last_error(None).decode())
+        self._loads = None
+        self._mask_built = False
+
+    # -- plumbing --
+    def _check(self, rc):
+        if rc != _ffi.AMT_OK:
+            _raise(rc, self._L.amt_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self._L.amt_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_loads(self, forces, fixed_dofs):
+        F = np.ascontiguousarray(forces, dtype=np.float64)
+        fixed = np.ascontiguousarray(fixed_dofs, dtype=np.int64)
+        key = (F.tobytes(), fixed.tobytes())
+        if key == self._loads:
+            return
+        self._check(self._L.amt_set_loads(self._h, F, F.size, fixed, fixed.size))
+        self._loads = key
+
+    def set_density(self, rho):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        self._check(self._L.amt_set_density(self._h, rho, rho.size))
+
+    def get_density(self):
+        out = np.zeros(self.n_elements)
+        self._check(self._L.amt_get_density(self._h, out, out.size))
+        return out
+
+    def get_displacement(self):
+        out = np.zeros(self.n_dofs)
+        self._check(self._L.amt_get_displacement(self._h, out, out.size))
+        return out
+
+    def begin_run(self):
+        self._check(self._L.amt_begin_run(self._h))
+        self._mask_built = True
+
+    # -- reference API --
+    def run_iteration(self, density, forces, fixed_dofs):
+        """runIteration (topology_optimizer.cpp:294-384); density (numpy array) is updated
+        in place when it is a float64 array, and returned."""
+        self.set_loads(forces, fixed_dofs)
+        self.set_density(density)
+        if not self._mask_built:
+            self.begin_run()
+        r = _ffi.AmtIterationResult()
+        self._check(self._L.amt_run_iteration(self._h, C.byref(r)))
+        new = self.get_density()
+        if isinstance(density, np.ndarray) and density.dtype == np.float64:
+            density[...] = new.reshape(density.shape)
+        elif isinstance(density, list):
+            density[:] = new.tolist()
+        return _convert(r)
+
+    runIteration = run_iteration
+
+    def run(self, initial_density, forces, fixed_dofs,
+            callback: Optional[Callable] = None) -> OptimizationResult:
+        """run (topology_optimizer.cpp:391-520)."""
+        rho = np.asarray(initial_density, dtype=np.float64).ravel()
+        if rho.size != self.n_elements:
+            rho = np.full(self.n_elements, self.config.volume_fraction)
+        self.set_loads(forces, fixed_dofs)
+        self.set_density(rho)
+        n = int(self.config.max_iterations)
+        hist = (_ffi.AmtIterationResult * max(n, 1))()
+        iters, conv, el = C.c_int(), C.c_int(), C.c_double()
+        err = []
+
+        def thunk(it, dptr, ne, rptr, user):
+            try:
+                d = np.ctypeslib.as_array(dptr, shape=(ne,)).copy()
+                return 1 if callback(it, d, _convert(rptr.contents)) else 0
+            except Exception as e:  # stop the run, re-raise below
+                err.append(e)
+                return 0
+
+        cb = _ffi.ITER_CB(thunk) if callback else _ffi.ITER_CB()
+        self._check(self._L.amt_run(self._h, n, hist, C.byref(iters), C.byref(conv),
+                                    C.byref(el), cb, None))
+        self._mask_built = True
+        if err:
+            raise err[0]
+        res = [_convert(hist[i]) for i in range(iters.value)]
+        return OptimizationResult(
+            final_density=self.get_density(),
+            compliance_history=[r.compliance for r in res],
+            volume_history=[r.volume_fraction for r in res],
+            iterations=iters.value, converged=bool(conv.value), elapsed_time_s=el.value,
+            iteration_results=res)
+
+    # -- phase entry points --
+    def apply_operator(self, x, with_bc=True):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n_dofs)
+        self._check(self._L.amt_apply_operator(self._h, x, y, int(bool(with_bc))))
+        return y
+
+    def diagonal(self, with_bc=True):
+        d = np.zeros(self.n_dofs)
+        P = C.c_double()
+        self._check(self._L.amt_get_diagonal(self._h, d, int(bool(with_bc)), C.byref(P)))
+        return d, P.value
+
+    def solve(self):
+        u = np.zeros(self.n_dofs)
+        st = _ffi.AmtSolveStats()
+        self._check(self._L.amt_solve(self._h, u, C.byref(st)))
+        return u, dict(iterations=st.iterations, converged=bool(st.converged),
+                       outer_iterations=st.outer_iterations, rel_residual=st.rel_residual,
+                       true_rel_residual=st.true_rel_residual, solve_time_s=st.solve_time_s)
+
+    def sensitivities(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        dc = np.zeros(self.n_elements)
+        self._check(self._L.amt_sensitivities(self._h, u, dc))
+        return dc
+
+    def filter(self, dc):
+        dc = np.ascontiguousarray(dc, dtype=np.float64)
+        out = np.zeros(self.n_elements)
+        self._check(self._L.amt_filter(self._h, dc, out))
+        return out
+
+    def filter_neighbors(self, e):
+        cnt = C.c_int()
+        self._check(self._L.amt_filter_neighbors(self._h, int(e), None, None, 0, C.byref(cnt)))
+        idx = np.zeros(max(cnt.value, 1), dtype=np.int64)
+        w = np.zeros(max(cnt.value, 1))
+        self._check(self._L.amt_filter_neighbors(self._h, int(e), idx.ctypes.data,
+                                                 w.ctypes.data, cnt.value, C.byref(cnt)))
+        return idx[:cnt.value], w[:cnt.value]
+
+    def oc_update(self, dcf):
+        dcf = np.ascontiguousarray(dcf, dtype=np.float64)
+        ch, npass = C.c_double(), C.c_int()
+        self._check(self._L.amt_oc_update(self._h, dcf, C.byref(ch), C.byref(npass)))
+        return ch.value, npass.value
+
+    def sync(self):
+        self._check(self._L.amt_sync(self._h))
+
+    def profile_enable(self, on=True):
+        self._check(self._L.amt_profile_enable(self._h, int(bool(on))))
+
+    def profile_read(self):
+        n, ms = C.c_int64(), C.c_double()
+        self._check(self._L.amt_profile_read(self._h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+
+def element_stiffness(E=1.0, nu=0.3, h=1.0):
+    """ElementKernel::computeStiffness (element_kernel.cpp:155-179) -> (24, 24) array."""
+    K = np.zeros(576)
+    rc = _ffi.load().amt_element_stiffness(float(E), float(nu), float(h), K)
+    if rc != _ffi.AMT_OK:
+        _raise(rc, _ffi.load().amt_last_error(None).decode())
+    return K.reshape(24, 24)
+
+
+# notebook binding names (cell 23 lines 9-12)
+CPPTopologyOptimizer = TopologyOptimizer
+
+
+def is_available():
+    """True when libamt.so loads and a HIP device is present."""
+    try:
+        L = _ffi.load()
+    except Exception:
+        return False
+    n = C.c_int()
+    return L.amt_device_info(None, 0, C.byref(n)) == _ffi.AMT_OK
+
+
+def get_max_threads():
+    """The notebook prints the engine's parallel width; here: the GPU's compute units."""
+    n = C.c_int()
+    if _ffi.load().amt_device_info(None, 0, C.byref(n)) != _ffi.AMT_OK:
+        return 0
+    return n.value
+
+
+def device_name():
+    buf = C.create_string_buffer(256)
+    n = C.c_int()
+    if _ffi.load().amt_device_info(buf, 256, C.byref(n)) != _ffi.AMT_OK:
+        return None
+    return buf.value.decode()

@@ -1,0 +1,189 @@
+"""OC iterations/s on the 120x60x60 bracket-like problem (1,350,723 DOFs), every solve to a
+1e-8 relative FP64 true residual, on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+
+A step is one OC iteration (operator coefficients, PCG solve, compliance, sensitivities,
+filter, OC bisection) of the SIMP loop, with all inputs resident in HBM. Two solver paths run
+the same W + K OC iterations:
+  - plain:  FP64 Jacobi-PCG from u = 0 (the reference's solve, Jacobi in place of IC);
+  - fast:   FP32 Jacobi-PCG inner solves inside FP64 iterative refinement, warm-started from
+            the previous OC iteration's displacement, stopped on the same FP64 true residual.
+`value` is the fast path's OC iterations/s; the plain path and the speed-up are reported
+beside it, with the fast path's compliance/density error against the plain FP64 path.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_path(problem, warmup, steps, precision, warm_start, profile):
+    import am_ml_suite_amd as A
+    from am_ml_suite_amd.problems import make_config
+    p = problem
+    cfg = make_config(p, precision=precision, warm_start=warm_start,
+                      max_iterations=warmup + steps)
+    opt = A.TopologyOptimizer(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3), cfg)
+    rho = p.rho0.copy()
+    opt.set_loads(p.F, p.fixed)
+    opt.set_density(rho)
+    opt.begin_run()
+    res = []
+    for it in range(warmup):
+        r = opt.run_iteration(rho, p.F, p.fixed)
+        res.append(r)
+        log("  [%s] warmup %d: c=%.9e pcg=%d outer=%d solve=%.3fs" % (
+            precision, it, r.compliance, r.solver_iterations, r.outer_iterations, r.solve_time_s))
+    if profile:
+        opt.profile_enable(True)
+    opt.sync()
+    t0 = time.perf_counter()
+    for it in range(steps):
+        r = opt.run_iteration(rho, p.F, p.fixed)
+        res.append(r)
+        log("  [%s] step %d: c=%.9e pcg=%d outer=%d res=%.2e solve=%.3fs" % (
+            precision, it, r.compliance, r.solver_iterations, r.outer_iterations,
+            r.true_rel_residual, r.solve_time_s))
+    opt.sync()
+    dt = time.perf_counter() - t0
+    prof = opt.profile_read() if profile else (0, 0.0)
+    dens = opt.get_density()
+    opt.close()
+    return dict(time=dt, results=res, density=dens, prof=prof)
+
+
+def cpu_baseline(problem, gpu_iters_per_oc, max_pcg=150):
+    """Oracle (C++/OpenMP restatement of the reference path, assembled CSR) on the host: one
+    OC iteration with PCG capped at max_pcg passes; the solve time is scaled to the PCG
+    iteration count the plain FP64 GPU path needed per OC iteration."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+    p = problem
+    cfg = O.make_config(p.nx, p.ny, p.nz, volume_fraction=p.params["volume_fraction"],
+                        filter_radius=p.params["filter_radius"], max_iterations=1,
+                        pcg_tol=1e-8, pcg_maxit=max_pcg, ref_compat=0)
+    t0 = time.perf_counter()
+    _, hist, _, _ = O.run(cfg, p.rho0, p.F, p.fixed)
+    wall = time.perf_counter() - t0
+    h = hist[0]
+    per_pcg = h["solve_time_s"] / max(h["solver_iterations"], 1)
+    t_iter = (h["assembly_time_s"] + per_pcg * gpu_iters_per_oc + h["sensitivity_time_s"]
+              + h["update_time_s"])
+    return dict(value=1.0 / t_iter, unit="OC iter/s", cores=O.lib().orc_max_threads(),
+                kind="port",
+                sample=("oracle C++/OpenMP FP64 (assembled CSR per OC iteration, Jacobi-PCG), "
+                        "1 OC iteration of the same 120x60x60 problem with PCG capped at %d "
+                        "passes (%.1f s wall); solve time scaled to %d PCG passes/OC iteration "
+                        "(plain GPU path); host %s"
+                        % (h["solver_iterations"], wall, gpu_iters_per_oc,
+                           platform.processor() or platform.machine())),
+                phases_s=dict(assembly=h["assembly_time_s"], per_pcg_iter=per_pcg,
+                              sensitivity=h["sensitivity_time_s"], update=h["update_time_s"]))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--paths", default="plain,fast")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import am_ml_suite_amd as A
+    p = A.bracket_like_problem()
+    log("problem %s: %d elements, %d DOFs, device %s"
+        % (p.name, p.n_elements, p.n_dofs, A.device_name()))
+    paths = args.paths.split(",")
+    out = {}
+    if "plain" in paths:
+        log("plain FP64 Jacobi-PCG")
+        out["plain"] = run_path(p, args.warmup, args.steps, "fp64", False, False)
+    if "fast" in paths:
+        log("fast: mixed FP32/FP64 refinement + warm start")
+        out["fast"] = run_path(p, args.warmup, args.steps, "mixed_fp32", True, True)
+
+    main_key = "fast" if "fast" in out else "plain"
+    m = out[main_key]
+    ops = args.steps / m["time"]
+    n_dof, n_e, n_n = p.n_dofs, p.n_elements, p.n_dofs // 3
+    timed = m["results"][args.warmup:]
+    pcg_per_oc = float(np.mean([r.solver_iterations for r in timed]))
+
+    # dominant kernel: the fused PCG operator launch (p = z + beta p_old gathered, q = K p)
+    vb = 4 if main_key == "fast" else 8
+    bytes_per_launch = n_dof * (4 * vb + 1) + n_e * vb
+    flops_per_launch = 1152.0 * n_n
+    launches, ms = m["prof"]
+    if launches:
+        avg_s = ms / launches / 1e3
+        achieved = bytes_per_launch / avg_s / 1e9
+        roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=achieved / HBM_PEAK_GBS, traffic=None,
+                        kernel="k_pcg_apply<%s>" % ("float" if vb == 4 else "double"),
+                        avg_launch_us=avg_s * 1e6, timed_launches=launches,
+                        algorithmic_bytes_per_launch=bytes_per_launch,
+                        flops_per_launch=flops_per_launch,
+                        achieved_tflops=flops_per_launch / avg_s / 1e12)
+    else:
+        roofline = None
+
+    extra = {}
+    if "plain" in out:
+        pl = out["plain"]
+        extra["plain_fp64_oc_iter_per_s"] = args.steps / pl["time"]
+        extra["plain_pcg_iters_per_oc"] = float(np.mean(
+            [r.solver_iterations for r in pl["results"][args.warmup:]]))
+        if main_key == "fast":
+            extra["speedup_vs_plain_fp64"] = ops / extra["plain_fp64_oc_iter_per_s"]
+            cf = np.array([r.compliance for r in m["results"]])
+            cp = np.array([r.compliance for r in pl["results"]])
+            extra["max_rel_compliance_err_vs_fp64"] = float(np.abs(cf / cp - 1).max())
+            extra["density_maxabs_vs_fp64"] = float(np.abs(m["density"] - pl["density"]).max())
+    extra["fast_pcg_iters_per_oc"] = pcg_per_oc
+    extra["dof_per_s"] = ops * n_dof
+    extra["max_true_rel_residual"] = float(max(r.true_rel_residual for r in timed))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        iters = int(round(extra.get("plain_pcg_iters_per_oc", pcg_per_oc)))
+        log("cpu baseline (oracle, %d PCG passes extrapolated)" % iters)
+        cpu = cpu_baseline(p, iters)
+
+    line = {
+        "metric": "OC iter/s at 1.3M DOF @1e-8 residual; learned+mixed vs plain FP64 PCG",
+        "value": ops, "unit": "OC iter/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * m["time"] / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if main_key == "fast" else "f64", "data": "synthetic",
+        "config": {"workload": "bracket_like_120x60x60_seed2", "n_dofs": n_dof,
+                   "n_elements": n_e, "pcg_tol": 1e-8, "path": main_key,
+                   "solver": ("FP32 Jacobi-PCG in FP64 refinement, warm start"
+                              if main_key == "fast" else "FP64 Jacobi-PCG")},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

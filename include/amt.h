@@ -1,0 +1,159 @@
+/*
+ * amt.h — C-ABI of the MI355X SIMP topology-optimization engine (libamt.so).
+ *
+ * Plain pointers and sizes only; no exceptions cross this boundary. Every entry point
+ * returns AMT_OK or an AMT_E* status; amt_last_error() gives the message.
+ *
+ * One handle = one problem on one GPU. The handle owns every device buffer and one HIP
+ * stream; calls on one handle are not thread-safe. Host buffers passed in are copied.
+ *
+ * The reference interface each entry point replaces is cited as
+ * cpp_engine/<file>:<line> of ettoremiglioranza1012/AM_ML_suite.
+ *
+ * Index conventions (same as the reference):
+ *   node id (z fastest) = i*(ny+1)*(nz+1) + j*(nz+1) + k; dof = 3*node + c
+ *     (element_kernel.cpp:186-216)
+ *   density slot e      = iz*nx*ny + iy*nx + ix (x fastest) for sensitivities, filter, OC
+ *     (topology_optimizer.hpp:218-232)
+ *   the stiffness of element (i,j,k) reads density[i*ny*nz + j*nz + k] when
+ *   density_index == AMT_INDEX_REF_COMPAT (global_assembler.hpp:162-164, as the reference),
+ *   or density[k*nx*ny + j*nx + i] when AMT_INDEX_CONSISTENT.
+ */
+#ifndef AMT_H
+#define AMT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMT_OK 0
+#define AMT_EINVAL 1   /* std::invalid_argument in the reference */
+#define AMT_ERANGE 2   /* std::out_of_range in the reference */
+#define AMT_EHIP 3     /* HIP runtime error (no device, launch failure, OOM) */
+#define AMT_ENAN 4     /* non-finite input */
+#define AMT_ENOTSUP 5  /* option not available in this build */
+
+enum { AMT_INDEX_CONSISTENT = 0, AMT_INDEX_REF_COMPAT = 1 };
+enum { AMT_PREC_FP64 = 0, AMT_PREC_MIXED_FP32 = 1 };
+enum { AMT_PRECOND_JACOBI = 0, AMT_PRECOND_UNET = 1 };
+
+typedef struct amt_problem amt_problem;
+
+/* Problem + SIMPConfig (topology_optimizer.hpp:49-59) + MaterialProperties
+ * (element_kernel.hpp:61-87) + the solver settings the optimizer hard-codes
+ * (topology_optimizer.cpp:35-40), in one flat struct. */
+typedef struct amt_config {
+  int nx, ny, nz;
+  double element_size;
+  double E, nu;
+  double volume_fraction, penalty, filter_radius, move_limit, rho_min, E_min;
+  int max_iterations;
+  double convergence_tol;
+  double pcg_tol;        /* relative residual |r|/|F_bc| */
+  int pcg_maxit;
+  double penalty_factor; /* P = penalty_factor * max|K_ii| */
+  int density_index;     /* AMT_INDEX_* */
+  int precision;         /* AMT_PREC_* */
+  int precond;           /* AMT_PRECOND_* */
+  int warm_start;        /* 1: start each solve from the previous displacement */
+  double inner_tol;      /* mixed precision: relative reduction per FP32 inner solve */
+  int inner_maxit;
+  int verbose;
+} amt_config;
+
+/* IterationResult (topology_optimizer.hpp:65-74) plus solver detail. */
+typedef struct amt_iteration_result {
+  double compliance, volume_fraction, max_density_change;
+  int solver_iterations; /* PCG passes (Eigen count: matvecs - 1 when converged) */
+  int solver_converged;
+  int bisect_passes;
+  int outer_iterations;  /* FP64 refinement steps (1 for plain FP64) */
+  double true_rel_residual; /* |F_bc - K u| / |F_bc|, recomputed in FP64 */
+  double assembly_time_s, solve_time_s, sensitivity_time_s, update_time_s;
+} amt_iteration_result;
+
+/* Fills the reference defaults: SIMPConfig{} (topology_optimizer.hpp:49-59), Ti6Al4V
+ * (element_kernel.hpp:67-69), solver tol 1e-6 / maxit 5000 / penalty 1e8
+ * (topology_optimizer.cpp:35-40), reference density index, FP64, Jacobi. */
+void amt_config_default(amt_config* cfg);
+
+/* ElementKernel::computeStiffness (element_kernel.cpp:155-179): hex8 Ke0 of a cube of edge
+ * h, 24x24 row-major. Host computation; needs no device. */
+int amt_element_stiffness(double E, double nu, double h, double* Ke);
+
+/* TopologyOptimizer ctor (topology_optimizer.cpp:16-53): Ke0, filter stencil, buffers. */
+int amt_create(const amt_config* cfg, amt_problem** out);
+int amt_destroy(amt_problem* p);
+/* Message of the last failed call on p (p may be NULL for amt_create failures). */
+const char* amt_last_error(const amt_problem* p);
+
+/* forces (n_dof) and fixed DOF list, as passed to run()/runIteration()
+ * (topology_optimizer.hpp:149-169). Out-of-range DOF -> AMT_ERANGE
+ * (linear_solver.cpp:69-73). Duplicated DOFs add the penalty once per occurrence. */
+int amt_set_loads(amt_problem* p, const double* F, int64_t n_dof, const int64_t* fixed,
+                  int64_t n_fixed);
+int amt_set_density(amt_problem* p, const double* rho, int64_t n_e);
+int amt_get_density(amt_problem* p, double* rho, int64_t n_e);
+int amt_get_displacement(amt_problem* p, double* u, int64_t n_dof);
+
+/* run() prologue (topology_optimizer.cpp:419-428): design mask from the current density. */
+int amt_begin_run(amt_problem* p);
+
+/* runIteration (topology_optimizer.cpp:294-384) on the device-resident density. If no
+ * design mask exists yet it is built from the current density first. */
+int amt_run_iteration(amt_problem* p, amt_iteration_result* out);
+
+/* IterationCallback (topology_optimizer.hpp:98-102); return 0 to stop. */
+typedef int (*amt_iter_cb)(int iteration, const double* density, int64_t n_e,
+                           const amt_iteration_result* result, void* user);
+
+/* run (topology_optimizer.cpp:391-520) from the density set by amt_set_density.
+ * hist must hold max_it entries. */
+int amt_run(amt_problem* p, int max_it, amt_iteration_result* hist, int* iterations,
+            int* converged, double* elapsed_s, amt_iter_cb cb, void* user);
+
+/* ---- phase entry points (the reference's private phases; used by parity tests) ---- */
+
+/* y = K x with the current density (global_assembler.cpp:61-222 + SpMV). with_bc=1 adds
+ * the penalty rows (linear_solver.cpp:35-91) exactly as the solve sees them. */
+int amt_apply_operator(amt_problem* p, const double* x, double* y, int with_bc);
+/* Diagonal of K (+ penalty if with_bc) and P = penalty_factor * max|K_ii|. */
+int amt_get_diagonal(amt_problem* p, double* diag, int with_bc, double* penalty);
+
+typedef struct amt_solve_stats {
+  int iterations, converged, outer_iterations;
+  double rel_residual, true_rel_residual, solve_time_s;
+} amt_solve_stats;
+/* solveWithBCs (linear_solver.cpp:186-203) for the current density and loads; u (n_dof)
+ * receives the displacement. */
+int amt_solve(amt_problem* p, double* u, amt_solve_stats* stats);
+
+/* computeSensitivities (topology_optimizer.cpp:126-161) for u (n_dof) and the current
+ * density; dc has n_e entries (slot order). */
+int amt_sensitivities(amt_problem* p, const double* u, double* dc);
+/* filterSensitivities (topology_optimizer.cpp:168-198) with the current density. */
+int amt_filter(amt_problem* p, const double* dc, double* dcf);
+/* Neighbour list of slot e as the filter kernel walks it (topology_optimizer.cpp:60-119).
+ * Writes up to cap entries; *count = list length. */
+int amt_filter_neighbors(amt_problem* p, int64_t e, int64_t* idx, double* w, int cap,
+                         int* count);
+/* updateDensitiesOC (topology_optimizer.cpp:205-287) on the device density with dcf (n_e)
+ * and the current design mask. */
+int amt_oc_update(amt_problem* p, const double* dcf, double* max_change, int* passes);
+
+/* Wait for all work of this handle. */
+int amt_sync(amt_problem* p);
+/* Device name / CU count of the handle's GPU. */
+int amt_device_info(char* name, int cap, int* n_cu);
+
+/* Kernel timing of the operator (K x) launches, measured with HIP events on the handle's
+ * stream: enable, then read the number of timed launches and their summed time. */
+int amt_profile_enable(amt_problem* p, int on);
+int amt_profile_read(amt_problem* p, int64_t* launches, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
