@@ -42,6 +42,8 @@ def build(verbose=False):
         cmd = [HIPCC] + COMMON
         if src.endswith(".hip"):
             cmd += ["-x", "hip", "--offload-arch=" + ARCH] + DEVICE_FLAGS
+        else:
+            cmd += ["-x", "c++"]
         cmd += ["-c", path, "-o", obj]
         _run(cmd, verbose)
         objs.append(obj)

@@ -36,10 +36,8 @@ constexpr int kOcCand = (1 << kOcLevels) - 1;
 constexpr int kMaxReduceBlocks = 1024;
 constexpr int kChunk = 32;  // PCG iterations launched between two host polls
 
-enum Ticket {
-  T_DIAG = 0, T_INIT, T_APPLY, T_UPDATE, T_RESID, T_DOT, T_OC_SWEEP, T_OC_FINAL0, T_OC_FINAL1,
-  T_MASK, T_DOT2, T_INIT32, T_APPLY32, T_UPDATE32, T_COUNT
-};
+// Partials regions (doubles) inside one buffer; see amt_problem::part()
+enum Region { R_APPLY = 0, R_UPD, R_INIT, R_OC, R_MISC, R_COUNT };
 
 // Device scalar slots
 enum Scal { S_MAXDIAG = 0, S_P, S_COMPL, S_NDESIGN, S_RR, S_BB, S_COUNT };
@@ -73,9 +71,12 @@ struct amt_problem {
   double* stencil_w = nullptr;
   int n_stencil = 0;
   double* partials = nullptr;
-  unsigned* tickets = nullptr;
+  int64_t region_off[R_COUNT] = {0};
+  int stride_apply = 0, stride_red = 0;
+  double* part(Region r) { return partials + region_off[r]; }
   double* scal = nullptr;
-  PcgState* pcg = nullptr;
+  PcgState* pcg = nullptr;   // [2], alternating by iteration parity
+  PcgState* pcg_flag = nullptr;
   OcState* oc = nullptr;
   int64_t* dbg_idx = nullptr;
   double* dbg_w = nullptr;
@@ -174,7 +175,10 @@ int prepare_coeffs(amt_problem* p) {
                      p->s32, g, p->cfg.density_index == AMT_INDEX_REF_COMPAT ? 1 : 0,
                      p->cfg.penalty, p->cfg.E_min, p->cfg.rho_min);
   hipLaunchKernelGGL(k_diag, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, p->diag, p->s64,
-                     p->Ke64, g, p->partials, p->tickets + T_DIAG, p->scal + S_MAXDIAG);
+                     p->Ke64, g, p->part(R_APPLY), p->stride_apply);
+  hipLaunchKernelGGL((k_finalize<1, true>), dim3(1), dim3(kBlock), 0, p->stream,
+                     p->part(R_APPLY), (int)nblk(p->n_n), p->stride_apply,
+                     p->scal + S_MAXDIAG);
   hipLaunchKernelGGL(k_dinv, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->diag,
                      p->fixcnt, p->scal + S_MAXDIAG, p->cfg.penalty_factor, p->scal + S_P,
                      p->dinv, p->dinv32, p->n_dof);
@@ -182,6 +186,14 @@ int prepare_coeffs(amt_problem* p) {
                      p->fixcnt, p->Fb, p->n_dof);
   p->coeffs_ready = true;
   return check_launch(p, "prepare_coeffs");
+}
+
+// out = a.b (device scalar), fixed-order reduction.
+void dot_dev(amt_problem* p, const double* a, const double* b, double* out) {
+  hipLaunchKernelGGL(k_dot, dim3(nred(p->n_dof)), dim3(kBlock), 0, p->stream, a, b, p->n_dof,
+                     p->part(R_MISC), p->stride_red);
+  hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kBlock), 0, p->stream, p->part(R_MISC),
+                     (int)nred(p->n_dof), p->stride_red, out);
 }
 
 void prof_begin(amt_problem* p, hipEvent_t* a) {
@@ -210,29 +222,43 @@ void prof_harvest(amt_problem* p) {
   p->ev_used = 0;
 }
 
-// Launch PCG pairs until the device reports done. T = vector type, X = solution pointer.
+// PCG from the current r (x already holds the start vector): init + start, then chunks of
+// (apply, update) pairs with one device->host poll of the check flag per chunk.
 template <typename T, typename TK, typename TX>
-int pcg_loop(amt_problem* p, TX* x, T* r, T* z, T** pp, T* q, const T* s, const TK* Ke,
-             const T* dinv, int t_apply, int t_update, PcgState* host_state) {
+int pcg_run(amt_problem* p, const double* b, TX* x, T* r, T* z, T** pp, T* q, const T* s,
+            const TK* Ke, const T* dinv, double tol, int maxit, PcgState* out) {
   const GridDims g = p->g;
+  const int nbu = (int)nred(p->n_dof), nba = (int)nblk(p->n_n);
+  hipLaunchKernelGGL(k_fill<T>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, pp[0], T(0),
+                     p->n_dof);
+  hipLaunchKernelGGL((k_pcg_init<T>), dim3(nbu), dim3(kBlock), 0, p->stream, b, r, z, dinv,
+                     p->n_dof, p->part(R_INIT), p->stride_red);
+  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, p->stream,
+                     (const double*)p->part(R_INIT), nbu, p->stride_red, b ? 1 : 0, tol, maxit,
+                     p->pcg);
+  AMT_CHECK(check_launch(p, "pcg init"));
+  HIP_TRY(p, hipMemcpyAsync(out, p->pcg, sizeof(PcgState), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(p, hipStreamSynchronize(p->stream));
   int t = 0;
-  for (;;) {
+  while (!out->done) {
     for (int c = 0; c < kChunk; ++c, ++t) {
       hipEvent_t ev;
       prof_begin(p, &ev);
-      hipLaunchKernelGGL((k_pcg_apply<T, TK>), dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream,
-                         z, pp[t & 1], pp[(t + 1) & 1], q, s, Ke, p->fixcnt, p->scal + S_P, g,
-                         p->pcg, p->partials, p->tickets + t_apply);
+      hipLaunchKernelGGL((k_pcg_apply<T, TK>), dim3(nba), dim3(kBlock), 0, p->stream, t, z,
+                         pp[t & 1], pp[(t + 1) & 1], q, s, Ke, p->fixcnt, p->scal + S_P, g,
+                         p->pcg, (const double*)p->part(R_UPD), nbu, p->part(R_APPLY),
+                         p->stride_red);
       prof_end(p, ev);
-      hipLaunchKernelGGL((k_pcg_update<T, TX>), dim3(nred(p->n_dof)), dim3(kBlock), 0,
-                         p->stream, x, r, z, pp[(t + 1) & 1], q, dinv, p->n_dof, p->pcg,
-                         p->partials, p->tickets + t_update);
+      hipLaunchKernelGGL((k_pcg_update<T, TX>), dim3(nbu), dim3(kBlock), 0, p->stream, t, x, r,
+                         z, pp[(t + 1) & 1], q, dinv, p->n_dof, p->pcg,
+                         (const double*)p->part(R_APPLY), nba, p->part(R_UPD), p->stride_red);
     }
+    hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kBlock), 0, p->stream, t, p->pcg,
+                       (const double*)p->part(R_UPD), nbu, p->stride_red, p->pcg_flag);
     AMT_CHECK(check_launch(p, "pcg chunk"));
-    HIP_TRY(p, hipMemcpyAsync(host_state, p->pcg, sizeof(PcgState), hipMemcpyDeviceToHost,
+    HIP_TRY(p, hipMemcpyAsync(out, p->pcg_flag, sizeof(PcgState), hipMemcpyDeviceToHost,
                               p->stream));
     HIP_TRY(p, hipStreamSynchronize(p->stream));
-    if (host_state->done) break;
   }
   prof_harvest(p);
   return AMT_OK;
@@ -241,8 +267,10 @@ int pcg_loop(amt_problem* p, TX* x, T* r, T* z, T** pp, T* q, const T* s, const 
 // FP64 residual r_out = Fb - K x and |r|^2 into scal[S_RR] (blocking read).
 int residual64(amt_problem* p, const double* x, double* rout, double* rr_host) {
   hipLaunchKernelGGL(k_residual, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, x, p->Fb,
-                     rout, p->s64, p->Ke64, p->fixcnt, p->scal + S_P, p->g, p->partials,
-                     p->tickets + T_RESID, p->scal + S_RR);
+                     rout, p->s64, p->Ke64, p->fixcnt, p->scal + S_P, p->g, p->part(R_APPLY),
+                     p->stride_apply);
+  hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kBlock), 0, p->stream, p->part(R_APPLY),
+                     (int)nblk(p->n_n), p->stride_apply, p->scal + S_RR);
   AMT_CHECK(check_launch(p, "k_residual"));
   if (rr_host) {
     HIP_TRY(p, hipMemcpyAsync(rr_host, p->scal + S_RR, sizeof(double), hipMemcpyDeviceToHost,
@@ -274,22 +302,13 @@ int solve_fp64(amt_problem* p, amt_solve_stats* st) {
     HIP_TRY(p, hipMemcpyAsync(p->r, p->Fb, sizeof(double) * p->n_dof, hipMemcpyDeviceToDevice,
                               p->stream));
   }
-  hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->p[0],
-                     0.0, p->n_dof);
-  hipLaunchKernelGGL((k_pcg_init<double>), dim3(nred(p->n_dof)), dim3(kBlock), 0, p->stream,
-                     p->Fb, p->r, p->z, p->dinv, p->n_dof, p->cfg.pcg_tol, p->cfg.pcg_maxit,
-                     p->pcg, p->partials, p->tickets + T_INIT);
-  AMT_CHECK(check_launch(p, "k_pcg_init"));
   PcgState hs;
-  HIP_TRY(p, hipMemcpyAsync(&hs, p->pcg, sizeof(PcgState), hipMemcpyDeviceToHost, p->stream));
-  HIP_TRY(p, hipStreamSynchronize(p->stream));
-  if (hs.rhs2 == 0.0) {  // Eigen: zero rhs -> x = 0
+  AMT_CHECK((pcg_run<double, double, double>(p, p->Fb, p->u, p->r, p->z, p->p, p->q, p->s64,
+                                             p->Ke64, p->dinv, p->cfg.pcg_tol,
+                                             p->cfg.pcg_maxit, &hs)));
+  if (hs.rhs2 == 0.0)  // Eigen: zero rhs -> x = 0
     hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream,
                        p->u, 0.0, p->n_dof);
-  } else if (!hs.done) {
-    AMT_CHECK((pcg_loop<double, double, double>(p, p->u, p->r, p->z, p->p, p->q, p->s64,
-                                                p->Ke64, p->dinv, T_APPLY, T_UPDATE, &hs)));
-  }
   double rr_true = 0.0;
   AMT_CHECK(residual64(p, p->u, p->q, &rr_true));
   st->iterations = hs.it;
@@ -308,8 +327,7 @@ int solve_mixed(amt_problem* p, amt_solve_stats* st) {
   if (!warm)
     hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream,
                        p->u, 0.0, p->n_dof);
-  hipLaunchKernelGGL(k_dot, dim3(nred(p->n_dof)), dim3(kBlock), 0, p->stream, p->Fb, p->Fb,
-                     p->n_dof, p->partials, p->tickets + T_DOT2, p->scal + S_BB);
+  dot_dev(p, p->Fb, p->Fb, p->scal + S_BB);
   double bb = 0.0, rr = 0.0;
   HIP_TRY(p, hipMemcpyAsync(&bb, p->scal + S_BB, sizeof(double), hipMemcpyDeviceToHost,
                             p->stream));
@@ -325,15 +343,10 @@ int solve_mixed(amt_problem* p, amt_solve_stats* st) {
                        p->r32, 1.0 / nr, p->n_dof);
     hipLaunchKernelGGL(k_fill<float>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->d32,
                        0.0f, p->n_dof);
-    hipLaunchKernelGGL(k_fill<float>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream,
-                       p->p32[0], 0.0f, p->n_dof);
     const int budget = std::min(p->cfg.inner_maxit, p->cfg.pcg_maxit - inner_total);
-    hipLaunchKernelGGL((k_pcg_init<float>), dim3(nred(p->n_dof)), dim3(kBlock), 0, p->stream,
-                       (const double*)nullptr, p->r32, p->z32, p->dinv32, p->n_dof,
-                       p->cfg.inner_tol, budget, p->pcg, p->partials, p->tickets + T_INIT32);
-    AMT_CHECK(check_launch(p, "k_pcg_init<float>"));
-    AMT_CHECK((pcg_loop<float, float, float>(p, p->d32, p->r32, p->z32, p->p32, p->q32, p->s32,
-                                             p->Ke32, p->dinv32, T_APPLY32, T_UPDATE32, &hs)));
+    AMT_CHECK((pcg_run<float, float, float>(p, nullptr, p->d32, p->r32, p->z32, p->p32,
+                                            p->q32, p->s32, p->Ke32, p->dinv32,
+                                            p->cfg.inner_tol, budget, &hs)));
     inner_total += std::max(hs.it, 1);
     hipLaunchKernelGGL(k_axpy_cast, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->u,
                        p->d32, nr, p->n_dof);
@@ -358,8 +371,9 @@ int do_solve(amt_problem* p, amt_solve_stats* st) {
 
 int build_design(amt_problem* p) {
   hipLaunchKernelGGL(k_design_mask, dim3(nred(p->n_e)), dim3(kBlock), 0, p->stream, p->rho,
-                     p->design, p->n_e, p->cfg.rho_min, p->partials, p->tickets + T_MASK,
-                     p->scal + S_NDESIGN);
+                     p->design, p->n_e, p->cfg.rho_min, p->part(R_MISC), p->stride_red);
+  hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kBlock), 0, p->stream, p->part(R_MISC),
+                     (int)nred(p->n_e), p->stride_red, p->scal + S_NDESIGN);
   AMT_CHECK(check_launch(p, "k_design_mask"));
   HIP_TRY(p, hipMemcpyAsync(&p->n_design, p->scal + S_NDESIGN, sizeof(double),
                             hipMemcpyDeviceToHost, p->stream));
@@ -379,14 +393,19 @@ int oc_update(amt_problem* p, OcState* hs) {
   init.n_design = p->n_design > 0 ? p->n_design : (double)p->n_e;  // :222-225 fallback
   HIP_TRY(p, hipMemcpyAsync(p->oc, &init, sizeof(OcState), hipMemcpyHostToDevice, p->stream));
   const unsigned nb = nred(p->n_e);
-  // ceil(100 / LEV) sweeps always suffice: each one advances LEV passes or finishes
-  for (int s = 0; s < (100 + kOcLevels - 1) / kOcLevels + 1; ++s)
+  // ceil(100 / LEV) + 1 sweeps always suffice: each one advances LEV passes or finishes
+  for (int sw = 0; sw < (100 + kOcLevels - 1) / kOcLevels + 1; ++sw) {
     hipLaunchKernelGGL((k_oc_sweep<kOcLevels>), dim3(nb), dim3(kBlock), 0, p->stream, p->rho,
                        p->dcf, p->design, p->n_e, p->cfg.move_limit, p->cfg.rho_min, p->oc,
-                       p->partials, p->tickets + T_OC_SWEEP);
+                       p->part(R_OC), p->stride_red);
+    hipLaunchKernelGGL((k_oc_walk<kOcLevels>), dim3(1), dim3(kBlock), 0, p->stream, p->oc,
+                       (const double*)p->part(R_OC), (int)nb, p->stride_red);
+  }
   hipLaunchKernelGGL(k_oc_final, dim3(nb), dim3(kBlock), 0, p->stream, p->rho, p->dcf,
-                     p->design, p->n_e, p->cfg.move_limit, p->cfg.rho_min, p->oc, p->partials,
-                     p->tickets + T_OC_FINAL0);
+                     p->design, p->n_e, p->cfg.move_limit, p->cfg.rho_min, p->oc,
+                     p->part(R_MISC), p->stride_red);
+  hipLaunchKernelGGL(k_oc_final_reduce, dim3(1), dim3(kBlock), 0, p->stream, p->oc,
+                     (const double*)p->part(R_MISC), (int)nb, p->stride_red);
   AMT_CHECK(check_launch(p, "oc"));
   p->coeffs_ready = false;
   if (hs) {
@@ -410,8 +429,7 @@ int run_iteration(amt_problem* p, amt_iteration_result* out) {
   AMT_CHECK(do_solve(p, &ss));
   const double t2 = now_s();
   // compliance with the unmodified forces (topology_optimizer.cpp:338)
-  hipLaunchKernelGGL(k_dot, dim3(nred(p->n_dof)), dim3(kBlock), 0, p->stream, p->u, p->F,
-                     p->n_dof, p->partials, p->tickets + T_DOT, p->scal + S_COMPL);
+  dot_dev(p, p->u, p->F, p->scal + S_COMPL);
   hipLaunchKernelGGL(k_sens, dim3(nblk(p->n_e)), dim3(kBlock), 0, p->stream, p->u, p->rho,
                      p->Ke64, p->g, p->cfg.penalty, p->dc);
   AMT_CHECK(check_launch(p, "sensitivities"));
@@ -556,13 +574,22 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
   }
   CREATE_TRY(dalloc(p, &p->dc, ne));
   CREATE_TRY(dalloc(p, &p->dcf, ne));
-  const int64_t npart = std::max<int64_t>({(int64_t)kOcCand * kMaxReduceBlocks,
-                                           4 * (int64_t)nblk(p->n_n), 4 * (int64_t)nblk(ne),
-                                           4 * (int64_t)kMaxReduceBlocks});
-  CREATE_TRY(dalloc(p, &p->partials, npart));
-  CREATE_TRY(dalloc(p, &p->tickets, T_COUNT));
+  p->stride_apply = (int)std::max<int64_t>(nblk(p->n_n), nblk(ne));
+  p->stride_red = std::max<int>(kMaxReduceBlocks, p->stride_apply);
+  {
+    // R_APPLY: 1 value per operator block; R_UPD: 2; R_INIT: 3; R_OC: kOcCand; R_MISC: 2
+    const int64_t sizes[R_COUNT] = {p->stride_apply, 2LL * p->stride_red, 3LL * p->stride_red,
+                                    (int64_t)kOcCand * p->stride_red, 2LL * p->stride_red};
+    int64_t off = 0;
+    for (int r = 0; r < R_COUNT; ++r) {
+      p->region_off[r] = off;
+      off += sizes[r];
+    }
+    CREATE_TRY(dalloc(p, &p->partials, off));
+  }
   CREATE_TRY(dalloc(p, &p->scal, S_COUNT));
-  CREATE_TRY(dalloc(p, &p->pcg, 1));
+  CREATE_TRY(dalloc(p, &p->pcg, 2));
+  CREATE_TRY(dalloc(p, &p->pcg_flag, 1));
   CREATE_TRY(dalloc(p, &p->oc, 1));
 
   std::vector<StencilEntry> st;
@@ -585,8 +612,7 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
     CREATE_TRY(copy_in(p, p->stencil, st.data(), st.size() * sizeof(StencilEntry)));
     CREATE_TRY(copy_in(p, p->stencil_w, sw.data(), sw.size() * sizeof(double)));
   }
-  if (hipMemsetAsync(p->tickets, 0, sizeof(unsigned) * T_COUNT, p->stream) != hipSuccess ||
-      hipMemsetAsync(p->fixcnt, 0, nd, p->stream) != hipSuccess ||
+  if (hipMemsetAsync(p->fixcnt, 0, nd, p->stream) != hipSuccess ||
       hipMemsetAsync(p->scal, 0, sizeof(double) * S_COUNT, p->stream) != hipSuccess)
     return bail(fail(p, AMT_EHIP, "hipMemset failed"));
   // initial density: uniform volume fraction (topology_optimizer.cpp:403-405)

@@ -9,7 +9,7 @@
 //                    (topology_optimizer.hpp:218-232)
 //
 // Reductions are deterministic: each block reduces in a fixed tree order and writes one
-// partial; the last block to finish (atomic ticket) sums the partials in a fixed order.
+// partial; the partials are summed in a fixed order by the next kernel in the stream.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -32,9 +32,10 @@ struct GridDims {
   int64_t n_elem;
 };
 
-// Device-resident solver state: the host never needs these values to keep iterating.
+// Device-resident PCG state. Two copies alternate by iteration parity: iteration t reads
+// S[t&1] and block 0 writes S[(t+1)&1], so no kernel reads a slot it is writing.
 struct PcgState {
-  double rz, alpha, beta, rr, rhs2, thresh, pq, pad0;
+  double rz, rr, rhs2, thresh;
   int it, done, converged, maxit;
 };
 
@@ -44,7 +45,14 @@ struct OcState {
 };
 
 // ---------------------------------------------------------------------------------------
-// Deterministic reductions
+// Deterministic, fence-free reductions
+//
+// A block reduces its values in a fixed tree order and stores one partial per value with a
+// plain store. The partials are reduced by the NEXT kernel in the stream (every block of it,
+// in the same fixed order, so all blocks obtain bitwise-identical scalars) or by a one-block
+// finalize kernel. The kernel boundary makes the partials visible. No block issues an
+// agent-scope fence: on gfx950 each such fence writes back the XCD's L2, which costs more
+// than the whole reduction when the kernel has just written tens of MB.
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -57,52 +65,14 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// Block-reduces NV values (sum, or max when IS_MAX) and writes partials[v*nblk + blockIdx.x].
-// Returns true in every thread of the block that finished last; that block must then call
-// finish_partials. All threads of the block must call this.
+// Block-reduce NV values; every thread receives the block totals in out.
 template <int NV, bool IS_MAX = false>
-__device__ bool block_partials(double (&v)[NV], double* partials, unsigned* ticket) {
+__device__ __forceinline__ void block_reduce(const double (&v)[NV], double (&out)[NV]) {
   __shared__ double red[NV][kWaves];
-  __shared__ bool last;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
-    double t = IS_MAX ? wave_max(v[q]) : wave_sum(v[q]);
-    if (lane == 0) red[q][w] = t;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      double t = red[q][0];
-#pragma unroll
-      for (int k = 1; k < kWaves; ++k) t = IS_MAX ? fmax(t, red[q][k]) : t + red[q][k];
-      partials[(size_t)q * gridDim.x + blockIdx.x] = t;
-    }
-    __threadfence();
-    const unsigned prev = atomicAdd(ticket, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  return last;
-}
-
-// Called by all threads of the last block: out[q] = fixed-order reduction of the partials.
-template <int NV, bool IS_MAX = false>
-__device__ void finish_partials(const double* partials, unsigned* ticket, double (&out)[NV]) {
-  __shared__ double red[NV][kWaves];
-  __threadfence();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int nb = gridDim.x;
-#pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    double t = IS_MAX ? 0.0 : 0.0;
-    for (int b = threadIdx.x; b < nb; b += kBlock) {
-      const double x = __hip_atomic_load(partials + (size_t)q * nb + b, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-      t = IS_MAX ? fmax(t, x) : t + x;
-    }
-    t = IS_MAX ? wave_max(t) : wave_sum(t);
+    const double t = IS_MAX ? wave_max(v[q]) : wave_sum(v[q]);
     if (lane == 0) red[q][w] = t;
   }
   __syncthreads();
@@ -113,7 +83,49 @@ __device__ void finish_partials(const double* partials, unsigned* ticket, double
     for (int k = 1; k < kWaves; ++k) t = IS_MAX ? fmax(t, red[q][k]) : t + red[q][k];
     out[q] = t;
   }
-  if (threadIdx.x == 0) atomicExch(ticket, 0u);
+  __syncthreads();
+}
+
+// partials[q*stride + blockIdx.x] = block total of v[q]. All threads of the block call it.
+template <int NV, bool IS_MAX = false>
+__device__ __forceinline__ void store_partials(const double (&v)[NV], double* partials,
+                                               int stride) {
+  double t[NV];
+  block_reduce<NV, IS_MAX>(v, t);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) partials[(size_t)q * stride + blockIdx.x] = t[q];
+  }
+}
+
+// out[q] = fixed-order reduction of partials[q*stride + 0..nb). All threads of the block
+// call it and all receive the result.
+template <int NV, bool IS_MAX = false>
+__device__ __forceinline__ void reduce_partials(const double* partials, int nb, int stride,
+                                                double (&out)[NV]) {
+  double v[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) {
+      const double x = partials[(size_t)q * stride + b];
+      t = IS_MAX ? fmax(t, x) : t + x;
+    }
+    v[q] = t;
+  }
+  block_reduce<NV, IS_MAX>(v, out);
+}
+
+template <int NV, bool IS_MAX = false>
+__global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ partials,
+                                                     int nb, int stride,
+                                                     double* __restrict__ out) {
+  double t[NV];
+  reduce_partials<NV, IS_MAX>(partials, nb, stride, t);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] = t[q];
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -268,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_apply(const T* __restrict__ x, T* __
   for (int c = 0; c < 3; ++c) y[3 * n + c] = yv[c];
 }
 
-// r = b - K x (with penalty), partial |r|^2 -> out[0] via the last block.
+// r = b - K x (with penalty); block partials of |r|^2.
 __global__ __launch_bounds__(kBlock) void k_residual(const double* __restrict__ x,
                                                      const double* __restrict__ b,
                                                      double* __restrict__ r,
@@ -276,8 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(const double* __restrict__ 
                                                      const double* __restrict__ Ke,
                                                      const uint8_t* __restrict__ fixcnt,
                                                      const double* __restrict__ P, GridDims g,
-                                                     double* partials, unsigned* ticket,
-                                                     double* out) {
+                                                     double* partials, int stride) {
   const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double acc[1] = {0.0};
   if (n < g.n_nodes) {
@@ -298,19 +309,14 @@ __global__ __launch_bounds__(kBlock) void k_residual(const double* __restrict__ 
       acc[0] = fma(rv, rv, acc[0]);
     }
   }
-  if (block_partials<1>(acc, partials, ticket)) {
-    double tot[1];
-    finish_partials<1>(partials, ticket, tot);
-    if (threadIdx.x == 0) out[0] = tot[0];
-  }
+  store_partials<1>(acc, partials, stride);
 }
 
-// Diagonal of K without penalty: d = sum_a s8[a] Ke0[3a+c,3a+c]; partial max|d|.
+// Diagonal of K without penalty: d = sum_a s8[a] Ke0[3a+c,3a+c]; block partials of max|d|.
 __global__ __launch_bounds__(kBlock) void k_diag(double* __restrict__ diag,
                                                  const double* __restrict__ s,
                                                  const double* __restrict__ Ke, GridDims g,
-                                                 double* partials, unsigned* ticket,
-                                                 double* maxdiag) {
+                                                 double* partials, int stride) {
   const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double m[1] = {0.0};
   if (n < g.n_nodes) {
@@ -327,11 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_diag(double* __restrict__ diag,
       m[0] = fmax(m[0], fabs(d));
     }
   }
-  if (block_partials<1, true>(m, partials, ticket)) {
-    double tot[1];
-    finish_partials<1, true>(partials, ticket, tot);
-    if (threadIdx.x == 0) maxdiag[0] = tot[0];
-  }
+  store_partials<1, true>(m, partials, stride);
 }
 
 // P = pf * max|K_ii| (linear_solver.cpp:46-47); Jacobi inverse of the penalised diagonal.
@@ -372,14 +374,13 @@ __global__ void k_fill(T* __restrict__ v, T val, int64_t n) {
 // PCG (Eigen conjugate_gradient loop with a Jacobi preconditioner)
 // ---------------------------------------------------------------------------------------
 
-// z = Dinv r; |b|^2, |r|^2, r.z -> state (Eigen: rhsNorm2, residualNorm2, absNew).
+// z = Dinv r; block partials of |b|^2, |r|^2, r.z (Eigen: rhsNorm2, residualNorm2, absNew).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(const double* __restrict__ b,
                                                      const T* __restrict__ r,
                                                      T* __restrict__ z,
                                                      const T* __restrict__ dinv, int64_t n,
-                                                     double tol, int maxit, PcgState* st,
-                                                     double* partials, unsigned* ticket) {
+                                                     double* partials, int stride) {
   double acc[3] = {0.0, 0.0, 0.0};
   AMT_GRID_STRIDE(i, n) {
     const T rv = r[i];
@@ -389,36 +390,73 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(const double* __restrict__ 
     acc[1] = fma((double)rv, (double)rv, acc[1]);
     acc[2] = fma((double)rv, (double)zv, acc[2]);
   }
-  if (block_partials<3>(acc, partials, ticket)) {
-    double tot[3];
-    finish_partials<3>(partials, ticket, tot);
-    if (threadIdx.x == 0) {
-      // without b (inner solve of a refinement step) the tolerance is relative to |r0|
-      const double bb = b ? tot[0] : tot[1];
-      st->rhs2 = bb;
-      st->thresh = fmax(tol * tol * bb, 2.2250738585072014e-308);
-      st->rr = tot[1];
-      st->rz = tot[2];
-      st->beta = 0.0;
-      st->alpha = 0.0;
-      st->it = 0;
-      st->maxit = maxit;
-      const int conv = (bb == 0.0) || (tot[1] < st->thresh);
-      st->done = conv || maxit <= 0;
-      st->converged = conv;
-    }
+  store_partials<3>(acc, partials, stride);
+}
+
+// One block: S[0] from the init partials. Without b (the inner solve of a refinement step)
+// the tolerance is relative to |r0|.
+__global__ __launch_bounds__(kBlock) void k_pcg_start(const double* __restrict__ partials,
+                                                      int nb, int stride, int has_b,
+                                                      double tol, int maxit, PcgState* S) {
+  double t[3];
+  reduce_partials<3>(partials, nb, stride, t);
+  if (threadIdx.x == 0) {
+    PcgState st;
+    st.rhs2 = has_b ? t[0] : t[1];
+    st.thresh = fmax(tol * tol * st.rhs2, 2.2250738585072014e-308);
+    st.rr = t[1];
+    st.rz = t[2];
+    st.it = 0;
+    st.maxit = maxit;
+    const int conv = (st.rhs2 == 0.0) || (t[1] < st.thresh);
+    st.converged = conv;
+    st.done = conv || maxit <= 0;
+    S[0] = st;
   }
 }
 
-// p = z + beta p_old (gathered on the fly), q = K p, partial p.q -> alpha = rz / pq.
+// Eigen loop bookkeeping at the top of iteration t (t > 0): the residual of update(t-1)
+// decides convergence (iterations stay t-1... counted as cur.it), else beta and it+1.
+// Returns the state iteration t continues with; beta is set when not done.
+__device__ __forceinline__ PcgState pcg_step_state(const PcgState& cur, int t,
+                                                   const double* upart, int nbu, int stride,
+                                                   double& beta) {
+  PcgState nx = cur;
+  beta = 0.0;
+  if (t == 0) return nx;
+  double v[2];
+  reduce_partials<2>(upart, nbu, stride, v);
+  nx.rr = v[0];
+  if (v[0] < cur.thresh) {
+    nx.done = 1;
+    nx.converged = 1;
+    return nx;
+  }
+  beta = v[1] / cur.rz;
+  nx.rz = v[1];
+  nx.it = cur.it + 1;
+  if (nx.it >= cur.maxit) nx.done = 1;
+  return nx;
+}
+
+// Iteration t, first half: p = z + beta p_old (gathered on the fly), q = K p, partials p.q.
 template <typename T, typename TK>
 __global__ __launch_bounds__(kBlock) void k_pcg_apply(
-    const T* __restrict__ z, const T* __restrict__ pold, T* __restrict__ pnew,
+    int t, const T* __restrict__ z, const T* __restrict__ pold, T* __restrict__ pnew,
     T* __restrict__ q, const T* __restrict__ s, const TK* __restrict__ Ke,
     const uint8_t* __restrict__ fixcnt, const double* __restrict__ P, GridDims g,
-    PcgState* st, double* partials, unsigned* ticket) {
-  if (st->done) return;
-  const T beta = (T)st->beta;
+    PcgState* S, const double* __restrict__ upart, int nbu, double* __restrict__ apart,
+    int stride) {
+  const PcgState cur = S[t & 1];
+  if (cur.done) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) S[(t + 1) & 1] = cur;
+    return;
+  }
+  double beta_d;
+  const PcgState nx = pcg_step_state(cur, t, upart, nbu, stride, beta_d);
+  if (blockIdx.x == 0 && threadIdx.x == 0) S[(t + 1) & 1] = nx;
+  if (nx.done) return;
+  const T beta = (T)beta_d;
   const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double acc[1] = {0.0};
   if (n < g.n_nodes) {
@@ -439,28 +477,21 @@ __global__ __launch_bounds__(kBlock) void k_pcg_apply(
       acc[0] = fma((double)nb.v[13][c], (double)yv[c], acc[0]);
     }
   }
-  if (block_partials<1>(acc, partials, ticket)) {
-    double tot[1];
-    finish_partials<1>(partials, ticket, tot);
-    if (threadIdx.x == 0) {
-      st->pq = tot[0];
-      st->alpha = st->rz / tot[0];
-    }
-  }
+  store_partials<1>(acc, apart, stride);
 }
 
-// x += alpha p; r -= alpha q; |r|^2 test; z = Dinv r; beta = r.z / rz_old.
+// Iteration t, second half: alpha = rz / p.q; x += alpha p; r -= alpha q; z = Dinv r;
+// partials |r|^2 and r.z (consumed by iteration t+1 or k_pcg_check).
 template <typename T, typename TX>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(TX* __restrict__ x, T* __restrict__ r,
-                                                       T* __restrict__ z,
-                                                       const T* __restrict__ p,
-                                                       const T* __restrict__ q,
-                                                       const T* __restrict__ dinv, int64_t n,
-                                                       PcgState* st, double* partials,
-                                                       unsigned* ticket) {
-  if (st->done) return;
-  const double alpha = st->alpha;
-  const T a = (T)alpha;
+__global__ __launch_bounds__(kBlock) void k_pcg_update(
+    int t, TX* __restrict__ x, T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p,
+    const T* __restrict__ q, const T* __restrict__ dinv, int64_t n, const PcgState* S,
+    const double* __restrict__ apart, int nba, double* __restrict__ upart, int stride) {
+  const PcgState st = S[(t + 1) & 1];
+  if (st.done) return;
+  double pq[1];
+  reduce_partials<1>(apart, nba, stride, pq);
+  const T a = (T)(st.rz / pq[0]);
   double acc[2] = {0.0, 0.0};
   AMT_GRID_STRIDE(i, n) {
     const T pv = p[i];
@@ -472,22 +503,21 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(TX* __restrict__ x, T* __
     acc[0] = fma((double)rv, (double)rv, acc[0]);
     acc[1] = fma((double)rv, (double)zv, acc[1]);
   }
-  if (block_partials<2>(acc, partials, ticket)) {
-    double tot[2];
-    finish_partials<2>(partials, ticket, tot);
-    if (threadIdx.x == 0) {
-      st->rr = tot[0];
-      if (tot[0] < st->thresh) {
-        st->done = 1;
-        st->converged = 1;
-      } else {
-        st->beta = tot[1] / st->rz;
-        st->rz = tot[1];
-        st->it += 1;
-        if (st->it >= st->maxit) st->done = 1;
-      }
-    }
+  store_partials<2>(acc, upart, stride);
+}
+
+// One block: the bookkeeping iteration t would do, written to *flag for the host to poll.
+__global__ __launch_bounds__(kBlock) void k_pcg_check(int t, const PcgState* S,
+                                                      const double* __restrict__ upart,
+                                                      int nbu, int stride, PcgState* flag) {
+  const PcgState cur = S[t & 1];
+  if (cur.done) {
+    if (threadIdx.x == 0) *flag = cur;
+    return;
   }
+  double beta;
+  const PcgState nx = pcg_step_state(cur, t, upart, nbu, stride, beta);
+  if (threadIdx.x == 0) *flag = nx;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -495,15 +525,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(TX* __restrict__ x, T* __
 // ---------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_dot(const double* __restrict__ a,
                                                 const double* __restrict__ b, int64_t n,
-                                                double* partials, unsigned* ticket,
-                                                double* out) {
+                                                double* partials, int stride) {
   double acc[1] = {0.0};
   AMT_GRID_STRIDE(i, n) acc[0] = fma(a[i], b[i], acc[0]);
-  if (block_partials<1>(acc, partials, ticket)) {
-    double tot[1];
-    finish_partials<1>(partials, ticket, tot);
-    if (threadIdx.x == 0) out[0] = tot[0];
-  }
+  store_partials<1>(acc, partials, stride);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -642,8 +667,8 @@ __global__ __launch_bounds__(kBlock) void k_oc_sweep(const double* __restrict__ 
                                                      const double* __restrict__ dcf,
                                                      const uint8_t* __restrict__ design,
                                                      int64_t n, double move, double rho_min,
-                                                     OcState* st, double* partials,
-                                                     unsigned* ticket) {
+                                                     const OcState* st, double* partials,
+                                                     int stride) {
   if (st->done) return;
   constexpr int NC = (1 << LEV) - 1;
   double lam[1 << LEV];
@@ -657,45 +682,53 @@ __global__ __launch_bounds__(kBlock) void k_oc_sweep(const double* __restrict__ 
 #pragma unroll
     for (int c = 0; c < NC; ++c) vol[c] += oc_candidate(r, d, lam[c + 1], move, rho_min);
   }
-  if (block_partials<NC>(vol, partials, ticket)) {
-    double tot[NC];
-    finish_partials<NC>(partials, ticket, tot);
-    if (threadIdx.x == 0) {
-      double l1 = st->l1, l2 = st->l2;
-      int nd = 1, passes = st->passes, done = 0;
-      for (int lev = 0; lev < LEV; ++lev) {
-        if (passes >= 100 || (l2 - l1) / (l1 + l2) < 1e-4) {
-          done = 1;
-          break;
-        }
-        const double lm = 0.5 * (l1 + l2);  // == lam[nd]
-        st->lmid = lm;
-        ++passes;
-        if (tot[nd - 1] / st->n_design > st->vf) {
-          l1 = lm;
-          nd = 2 * nd + 1;
-        } else {
-          l2 = lm;
-          nd = 2 * nd;
-        }
-      }
-      if (!done && (passes >= 100 || (l2 - l1) / (l1 + l2) < 1e-4)) done = 1;
-      st->l1 = l1;
-      st->l2 = l2;
-      st->passes = passes;
-      st->done = done;
+  store_partials<NC>(vol, partials, stride);
+}
+
+// One block: reduce the candidate volumes and walk the bisection tree with the reference's
+// break test; updates l1, l2, the last evaluated lambda and the pass count.
+template <int LEV>
+__global__ __launch_bounds__(kBlock) void k_oc_walk(OcState* st,
+                                                    const double* __restrict__ partials,
+                                                    int nb, int stride) {
+  if (st->done) return;
+  constexpr int NC = (1 << LEV) - 1;
+  double tot[NC];
+  reduce_partials<NC>(partials, nb, stride, tot);
+  if (threadIdx.x != 0) return;
+  double l1 = st->l1, l2 = st->l2;
+  int nd = 1, passes = st->passes, done = 0;
+  for (int lev = 0; lev < LEV; ++lev) {
+    if (passes >= 100 || (l2 - l1) / (l1 + l2) < 1e-4) {
+      done = 1;
+      break;
+    }
+    const double lm = 0.5 * (l1 + l2);  // == lam[nd] of the sweep
+    st->lmid = lm;
+    ++passes;
+    if (tot[nd - 1] / st->n_design > st->vf) {
+      l1 = lm;
+      nd = 2 * nd + 1;
+    } else {
+      l2 = lm;
+      nd = 2 * nd;
     }
   }
+  if (!done && (passes >= 100 || (l2 - l1) / (l1 + l2) < 1e-4)) done = 1;
+  st->l1 = l1;
+  st->l2 = l2;
+  st->passes = passes;
+  st->done = done;
 }
 
 // rho <- candidate at the last evaluated lambda (design) / unchanged (non-design);
-// max|drho| and the design-space volume sum (topology_optimizer.cpp:274-286, :364-381).
+// partials of max|drho| and the design-space volume (topology_optimizer.cpp:274-286, :364-381).
 __global__ __launch_bounds__(kBlock) void k_oc_final(double* __restrict__ rho,
                                                      const double* __restrict__ dcf,
                                                      const uint8_t* __restrict__ design,
                                                      int64_t n, double move, double rho_min,
-                                                     OcState* st, double* partials,
-                                                     unsigned* ticket) {
+                                                     const OcState* st, double* partials,
+                                                     int stride) {
   const double lm = st->lmid;
   const int any = st->passes > 0;
   double acc[1] = {0.0}, mx[1] = {0.0};
@@ -709,26 +742,27 @@ __global__ __launch_bounds__(kBlock) void k_oc_final(double* __restrict__ rho,
     mx[0] = fmax(mx[0], fabs(rn - r));
     rho[e] = rn;
   }
-  // two reductions share one ticket: first the max, then the sum
-  if (block_partials<1, true>(mx, partials, ticket)) {
-    double t[1];
-    finish_partials<1, true>(partials, ticket, t);
-    if (threadIdx.x == 0) st->change = t[0];
-  }
-  __syncthreads();
-  if (block_partials<1>(acc, partials + gridDim.x, ticket + 1)) {
-    double t[1];
-    finish_partials<1>(partials + gridDim.x, ticket + 1, t);
-    if (threadIdx.x == 0) st->vol_sum = t[0];
+  store_partials<1, true>(mx, partials, stride);
+  store_partials<1>(acc, partials + stride, stride);
+}
+
+__global__ __launch_bounds__(kBlock) void k_oc_final_reduce(OcState* st,
+                                                            const double* __restrict__ partials,
+                                                            int nb, int stride) {
+  double m[1], v[1];
+  reduce_partials<1, true>(partials, nb, stride, m);
+  reduce_partials<1>(partials + stride, nb, stride, v);
+  if (threadIdx.x == 0) {
+    st->change = m[0];
+    st->vol_sum = v[0];
   }
 }
 
-// Design mask from the initial density (topology_optimizer.cpp:419-428); count via partials.
+// Design mask from the initial density (topology_optimizer.cpp:419-428); partial counts.
 __global__ __launch_bounds__(kBlock) void k_design_mask(const double* __restrict__ rho,
                                                         uint8_t* __restrict__ design,
                                                         int64_t n, double rho_min,
-                                                        double* partials, unsigned* ticket,
-                                                        double* count) {
+                                                        double* partials, int stride) {
   double acc[1] = {0.0};
   AMT_GRID_STRIDE(e, n) {
     const double r = rho[e];
@@ -736,11 +770,7 @@ __global__ __launch_bounds__(kBlock) void k_design_mask(const double* __restrict
     design[e] = d;
     acc[0] += d;
   }
-  if (block_partials<1>(acc, partials, ticket)) {
-    double t[1];
-    finish_partials<1>(partials, ticket, t);
-    if (threadIdx.x == 0) count[0] = t[0];
-  }
+  store_partials<1>(acc, partials, stride);
 }
 
 }  // namespace amt

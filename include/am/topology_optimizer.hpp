@@ -132,6 +132,7 @@ class TopologyOptimizer {
   int nz() const { return nz_; }
   int64_t numElements() const { return n_elements_; }
   int64_t numDofs() const { return n_dofs_; }
+  Scalar elementSize() const { return element_size_; }
   const SIMPConfig& config() const { return config_; }
 
   /// Neighbour list of element slot e as the device filter walks it.
