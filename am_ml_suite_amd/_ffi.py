@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(PKG, "libamt.so")
 AMT_OK, AMT_EINVAL, AMT_ERANGE, AMT_EHIP, AMT_ENAN, AMT_ENOTSUP = range(6)
 AMT_INDEX_CONSISTENT, AMT_INDEX_REF_COMPAT = 0, 1
 AMT_PREC_FP64, AMT_PREC_MIXED_FP32 = 0, 1
-AMT_PRECOND_JACOBI, AMT_PRECOND_UNET = 0, 1
+AMT_PRECOND_JACOBI, AMT_PRECOND_UNET, AMT_PRECOND_MULTIGRID = 0, 1, 2
 
 # every symbol include/amt.h declares
 EXPORTS = [

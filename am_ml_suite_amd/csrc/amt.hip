@@ -20,6 +20,8 @@
 
 #include "../../include/amt.h"
 #include "kernels.hpp"
+#include "multigrid.hpp"
+#include <cstddef>
 
 namespace amt {
 void hex8_stiffness(double E, double nu, double h, double* Ke);
@@ -35,12 +37,15 @@ constexpr int kOcLevels = 5;
 constexpr int kOcCand = (1 << kOcLevels) - 1;
 constexpr int kMaxReduceBlocks = 1024;
 constexpr int kChunk = 32;  // PCG iterations launched between two host polls
+constexpr int kMgChunk = 8;  // multigrid flexible-CG iterations between two host polls
+constexpr float kMgOmega = 0.6f;  // damped-Jacobi weight of the V-cycle smoother
+constexpr int64_t kMgCoarseDofs = 300;  // coarsen until a level has at most this many DOFs
 
 // Partials regions (doubles) inside one buffer; see amt_problem::part()
 enum Region { R_APPLY = 0, R_UPD, R_INIT, R_OC, R_MISC, R_COUNT };
 
 // Device scalar slots
-enum Scal { S_MAXDIAG = 0, S_P, S_COMPL, S_NDESIGN, S_RR, S_BB, S_COUNT };
+enum Scal { S_MAXDIAG = 0, S_P, S_COMPL, S_NDESIGN, S_RR, S_BB, S_FLAG, S_COUNT };
 
 inline double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
@@ -81,6 +86,19 @@ struct amt_problem {
   int64_t* dbg_idx = nullptr;
   double* dbg_w = nullptr;
   int* dbg_cnt = nullptr;
+
+  // geometric multigrid (precond == AMT_PRECOND_MULTIGRID)
+  struct MgLevel {
+    MgGrid g;
+    float* A = nullptr;  // 27-point block stencil (levels >= 1)
+    float *wd = nullptr, *rhs = nullptr, *x = nullptr, *t = nullptr;
+  };
+  std::vector<MgLevel> mg;
+  double* mg_G = nullptr;     // 8 x 576 Galerkin child tables
+  float* mg_ainv = nullptr;   // coarsest dense inverse
+  int mg_nc = 0;              // coarsest DOFs
+  FcgState* fcg = nullptr;
+  std::vector<float> mg_host_stencil;
 
   bool loads_set = false, design_ready = false, coeffs_ready = false, have_u = false;
   double n_design = 0;
@@ -168,6 +186,13 @@ int copy_out(amt_problem* p, void* dst, const void* src, size_t bytes) {
   return AMT_OK;
 }
 
+int mg_setup(amt_problem* p);
+void dot_dev(amt_problem* p, const double* a, const double* b, double* out);
+int residual64(amt_problem* p, const double* x, double* rout, double* rr_host);
+void prof_begin(amt_problem* p, hipEvent_t* a);
+void prof_end(amt_problem* p, hipEvent_t a);
+void prof_harvest(amt_problem* p);
+
 // ---- operator coefficients for the current density ("assembly") ----
 int prepare_coeffs(amt_problem* p) {
   const GridDims g = p->g;
@@ -184,8 +209,260 @@ int prepare_coeffs(amt_problem* p) {
                      p->dinv, p->dinv32, p->n_dof);
   hipLaunchKernelGGL(k_bc_rhs, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->F,
                      p->fixcnt, p->Fb, p->n_dof);
+  AMT_CHECK(check_launch(p, "prepare_coeffs"));
+  if (!p->mg.empty()) AMT_CHECK(mg_setup(p));
   p->coeffs_ready = true;
-  return check_launch(p, "prepare_coeffs");
+  return AMT_OK;
+}
+
+// ---- geometric multigrid: hierarchy, per-iteration coarse operators, V-cycle ----
+
+// G_c = Q_c^T Ke0 Q_c for the 8 children c of a coarse element; Q_c interpolates the coarse
+// element's 8 nodal vectors trilinearly to child c's 8 nodes (local node order of
+// element_kernel.cpp:196-199).
+void mg_child_tables(const double* Ke, double* G) {
+  static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  for (int c = 0; c < 8; ++c) {
+    double w[8][8];  // w[b][a]: weight of coarse node a at child node b
+    for (int b = 0; b < 8; ++b)
+      for (int a = 0; a < 8; ++a) {
+        double v = 1.0;
+        for (int k = 0; k < 3; ++k) {
+          const double t = (off[c][k] + off[b][k]) / 2.0;
+          v *= off[a][k] ? t : 1.0 - t;
+        }
+        w[b][a] = v;
+      }
+    // G[3a+i][3a'+j] = sum_{b,b'} w[b][a] Ke[3b+i][3b'+j] w[b'][a']
+    for (int a = 0; a < 8; ++a)
+      for (int i = 0; i < 3; ++i)
+        for (int a2 = 0; a2 < 8; ++a2)
+          for (int j = 0; j < 3; ++j) {
+            double acc = 0.0;
+            for (int b = 0; b < 8; ++b) {
+              if (w[b][a] == 0.0) continue;
+              for (int b2 = 0; b2 < 8; ++b2)
+                acc += w[b][a] * Ke[(3 * b + i) * 24 + 3 * b2 + j] * w[b2][a2];
+            }
+            G[c * 576 + (3 * a + i) * 24 + 3 * a2 + j] = acc;
+          }
+  }
+}
+
+MgGrid mg_grid(int ex, int ey, int ez) {
+  MgGrid g;
+  g.ex = ex;
+  g.ey = ey;
+  g.ez = ez;
+  g.nnx = ex + 1;
+  g.nny = ey + 1;
+  g.nnz = ez + 1;
+  g.n_nodes = (int64_t)g.nnx * g.nny * g.nnz;
+  return g;
+}
+
+int mg_build_hierarchy(amt_problem* p, const double* Ke) {
+  amt_problem::MgLevel L0;
+  L0.g = mg_grid(p->g.nx, p->g.ny, p->g.nz);
+  AMT_CHECK(dalloc(p, &L0.wd, p->n_dof));
+  AMT_CHECK(dalloc(p, &L0.x, p->n_dof));
+  AMT_CHECK(dalloc(p, &L0.t, p->n_dof));
+  p->mg.push_back(L0);
+  for (;;) {
+    const MgGrid& f = p->mg.back().g;
+    if (3 * f.n_nodes <= kMgCoarseDofs && p->mg.size() > 1) break;
+    if (f.ex == 1 && f.ey == 1 && f.ez == 1 && p->mg.size() > 1) break;
+    amt_problem::MgLevel L;
+    L.g = mg_grid((f.ex + 1) / 2, (f.ey + 1) / 2, (f.ez + 1) / 2);
+    const int64_t nd = 3 * L.g.n_nodes;
+    AMT_CHECK(dalloc(p, &L.A, 243 * L.g.n_nodes));
+    AMT_CHECK(dalloc(p, &L.wd, nd));
+    AMT_CHECK(dalloc(p, &L.rhs, nd));
+    AMT_CHECK(dalloc(p, &L.x, nd));
+    AMT_CHECK(dalloc(p, &L.t, nd));
+    p->mg.push_back(L);
+  }
+  p->mg_nc = (int)(3 * p->mg.back().g.n_nodes);
+  if (p->mg_nc > 1024) return fail(p, AMT_EINVAL, "multigrid coarsest level too large");
+  AMT_CHECK(dalloc(p, &p->mg_ainv, (size_t)p->mg_nc * p->mg_nc));
+  std::vector<double> G(8 * 576);
+  mg_child_tables(Ke, G.data());
+  AMT_CHECK(dalloc(p, &p->mg_G, G.size()));
+  AMT_CHECK(copy_in(p, p->mg_G, G.data(), G.size() * sizeof(double)));
+  AMT_CHECK(dalloc(p, &p->fcg, 1));
+  return AMT_OK;
+}
+
+// Dense inverse of the coarsest Galerkin operator (host, FP64). Rows that no element reaches
+// are decoupled (unit diagonal); a pivot that rounding drives to <= 1e-13 of its diagonal is
+// decoupled too, so the result stays symmetric positive definite.
+void mg_dense_inverse(const float* st, const MgGrid& g, std::vector<float>& inv) {
+  const int n = (int)(3 * g.n_nodes);
+  std::vector<double> A((size_t)n * n, 0.0);
+  for (int64_t N = 0; N < g.n_nodes; ++N) {
+    const int I = (int)(N / ((int64_t)g.nny * g.nnz)), J = (int)((N / g.nnz) % g.nny),
+              K = (int)(N % g.nnz);
+    for (int d = 0; d < 27; ++d) {
+      const int MI = I + d / 9 - 1, MJ = J + (d / 3) % 3 - 1, MK = K + d % 3 - 1;
+      if (MI < 0 || MI >= g.nnx || MJ < 0 || MJ >= g.nny || MK < 0 || MK >= g.nnz) continue;
+      const int64_t M = ((int64_t)MI * g.nny + MJ) * g.nnz + MK;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          A[(size_t)(3 * N + i) * n + 3 * M + j] = st[(int64_t)(d * 9 + 3 * i + j) * g.n_nodes + N];
+    }
+  }
+  std::vector<double> diag(n);
+  for (int i = 0; i < n; ++i) {
+    diag[i] = A[(size_t)i * n + i];
+    if (!(diag[i] > 0.0)) {
+      for (int j = 0; j < n; ++j) A[(size_t)i * n + j] = A[(size_t)j * n + i] = 0.0;
+      A[(size_t)i * n + i] = diag[i] = 1.0;
+    }
+  }
+  // Cholesky A = L L^T (lower triangle in place)
+  for (int j = 0; j < n; ++j) {
+    double d = A[(size_t)j * n + j];
+    for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+    if (!(d > 1e-13 * diag[j])) {
+      for (int k = 0; k < j; ++k) A[(size_t)j * n + k] = 0.0;
+      for (int i = j + 1; i < n; ++i) A[(size_t)i * n + j] = 0.0;
+      A[(size_t)j * n + j] = 1.0 / std::sqrt(diag[j]);
+      continue;
+    }
+    const double l = std::sqrt(d);
+    A[(size_t)j * n + j] = l;
+    for (int i = j + 1; i < n; ++i) {
+      double v = A[(size_t)i * n + j];
+      for (int k = 0; k < j; ++k) v -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+      A[(size_t)i * n + j] = v / l;
+    }
+  }
+  // inverse = L^{-T} L^{-1}: solve for each unit vector
+  inv.assign((size_t)n * n, 0.f);
+  std::vector<double> y(n);
+  for (int c = 0; c < n; ++c) {
+    for (int i = 0; i < n; ++i) {
+      double v = (i == c) ? 1.0 : 0.0;
+      for (int k = 0; k < i; ++k) v -= A[(size_t)i * n + k] * y[k];
+      y[i] = v / A[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double v = y[i];
+      for (int k = i + 1; k < n; ++k) v -= A[(size_t)k * n + i] * y[k];
+      y[i] = v / A[(size_t)i * n + i];
+    }
+    for (int i = 0; i < n; ++i) inv[(size_t)i * n + c] = (float)y[i];
+  }
+}
+
+// Coarse operators for the current element scales and penalty (after k_dinv).
+int mg_setup(amt_problem* p) {
+  auto& mg = p->mg;
+  hipLaunchKernelGGL(k_mg_wd0, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->diag,
+                     p->fixcnt, p->scal + S_P, kMgOmega, mg[0].wd, p->n_dof);
+  hipLaunchKernelGGL(k_mg_galerkin1, dim3(nblk(mg[1].g.n_nodes), 27, 3), dim3(kBlock), 0,
+                     p->stream, p->s64, p->g, p->mg_G, p->fixcnt, p->scal + S_P, mg[1].g,
+                     mg[1].A);
+  for (size_t l = 1; l < mg.size(); ++l) {
+    if (l >= 2)
+      hipLaunchKernelGGL(k_mg_coarsen, dim3(nblk(mg[l].g.n_nodes), 27, 3), dim3(kBlock), 0,
+                         p->stream, mg[l - 1].A, mg[l - 1].g, mg[l].g, mg[l].A);
+    hipLaunchKernelGGL(k_mg_stencil_wdinv, dim3(nblk(3 * mg[l].g.n_nodes)), dim3(kBlock), 0,
+                       p->stream, mg[l].A, mg[l].g, kMgOmega, mg[l].wd);
+  }
+  AMT_CHECK(check_launch(p, "mg_setup"));
+  const auto& C = mg.back();
+  p->mg_host_stencil.resize(243 * C.g.n_nodes);
+  AMT_CHECK(copy_out(p, p->mg_host_stencil.data(), C.A, sizeof(float) * 243 * C.g.n_nodes));
+  std::vector<float> inv;
+  mg_dense_inverse(p->mg_host_stencil.data(), C.g, inv);
+  return copy_in(p, p->mg_ainv, inv.data(), inv.size() * sizeof(float));
+}
+
+// z = V-cycle(r) (FP32 inside); writes partials of z.r and z.q (q may be null) to R_INIT.
+void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q) {
+  auto& mg = p->mg;
+  const int* done = (const int*)((const char*)p->fcg + offsetof(FcgState, done));
+  const size_t L = mg.size();
+  hipLaunchKernelGGL(k_mg0_pre, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, r, p->g,
+                     p->s32, p->Ke32, p->fixcnt, p->scal + S_P, mg[0].wd, mg[0].x, mg[0].t,
+                     done);
+  hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[1].g.n_nodes)), dim3(kBlock), 0, p->stream,
+                     mg[0].t, mg[0].g, mg[1].rhs, mg[1].g, done);
+  for (size_t l = 1; l + 1 < L; ++l) {
+    hipLaunchKernelGGL(k_mg_pre, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
+                       mg[l].A, mg[l].g, mg[l].wd, mg[l].rhs, mg[l].x, mg[l].t, done);
+    hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[l + 1].g.n_nodes)), dim3(kBlock), 0,
+                       p->stream, mg[l].t, mg[l].g, mg[l + 1].rhs, mg[l + 1].g, done);
+  }
+  hipLaunchKernelGGL(k_mg_dense, dim3(1), dim3(kBlock), 0, p->stream, p->mg_ainv, p->mg_nc,
+                     mg[L - 1].rhs, mg[L - 1].x, done);
+  const float* res = mg[L - 1].x;  // correction of the level below the current one
+  for (size_t l = L - 2; l >= 1; --l) {
+    hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
+                       res, mg[l + 1].g, mg[l].x, mg[l].g, done);
+    hipLaunchKernelGGL(k_mg_post, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
+                       mg[l].A, mg[l].g, mg[l].wd, mg[l].rhs, mg[l].x, mg[l].t, done);
+    res = mg[l].t;
+  }
+  hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, res,
+                     mg[1].g, mg[0].x, mg[0].g, done);
+  hipLaunchKernelGGL(k_mg0_post, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, r, p->g,
+                     p->s32, p->Ke32, p->fixcnt, p->scal + S_P, mg[0].wd, mg[0].x, z, q,
+                     p->part(R_INIT), p->stride_red, done);
+}
+
+// Flexible CG (FP64) preconditioned by the V-cycle, from x = 0 or the previous displacement.
+int solve_mg(amt_problem* p, amt_solve_stats* st) {
+  const bool warm = p->cfg.warm_start && p->have_u;
+  if (!warm)
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream,
+                       p->u, 0.0, p->n_dof);
+  dot_dev(p, p->Fb, p->Fb, p->scal + S_BB);
+  AMT_CHECK(residual64(p, p->u, p->r, nullptr));
+  hipLaunchKernelGGL(k_fcg_start, dim3(1), dim3(64), 0, p->stream, p->fcg, p->scal + S_RR,
+                     p->scal + S_BB, p->cfg.pcg_tol, p->cfg.pcg_maxit);
+  hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->p[0],
+                     0.0, p->n_dof);
+  const int nba = (int)nblk(p->n_n), nbu = (int)nred(p->n_dof);
+  int* done_flag = (int*)(p->scal + S_FLAG);
+  mg_vcycle(p, p->r, p->z, nullptr);
+  hipLaunchKernelGGL(k_fcg_scalar, dim3(1), dim3(kBlock), 0, p->stream, p->fcg,
+                     (const double*)p->part(R_UPD), nbu, (const double*)p->part(R_APPLY), nba,
+                     (const double*)p->part(R_INIT), nba, p->stride_red, 1, done_flag);
+  AMT_CHECK(check_launch(p, "fcg init"));
+  FcgState hs;
+  AMT_CHECK(copy_out(p, &hs, p->fcg, sizeof(FcgState)));
+  int t = 0;
+  while (!hs.done) {
+    for (int c = 0; c < kMgChunk; ++c, ++t) {
+      hipEvent_t ev;
+      prof_begin(p, &ev);
+      hipLaunchKernelGGL(k_fcg_apply, dim3(nba), dim3(kBlock), 0, p->stream, p->z,
+                         p->p[t & 1], p->p[(t + 1) & 1], p->q, p->s64, p->Ke64, p->fixcnt,
+                         p->scal + S_P, p->g, p->fcg, p->part(R_APPLY));
+      prof_end(p, ev);
+      hipLaunchKernelGGL(k_fcg_update, dim3(nbu), dim3(kBlock), 0, p->stream, p->u, p->r,
+                         p->p[(t + 1) & 1], p->q, p->n_dof, p->fcg,
+                         (const double*)p->part(R_APPLY), nba, p->part(R_UPD));
+      mg_vcycle(p, p->r, p->z, p->q);
+      hipLaunchKernelGGL(k_fcg_scalar, dim3(1), dim3(kBlock), 0, p->stream, p->fcg,
+                         (const double*)p->part(R_UPD), nbu, (const double*)p->part(R_APPLY),
+                         nba, (const double*)p->part(R_INIT), nba, p->stride_red, 0, done_flag);
+    }
+    AMT_CHECK(check_launch(p, "fcg chunk"));
+    AMT_CHECK(copy_out(p, &hs, p->fcg, sizeof(FcgState)));
+  }
+  prof_harvest(p);
+  double rr_true = 0.0;
+  AMT_CHECK(residual64(p, p->u, p->q, &rr_true));
+  st->iterations = hs.it;
+  st->converged = hs.converged;
+  st->outer_iterations = 1;
+  st->rel_residual = hs.rhs2 > 0 ? std::sqrt(hs.rr / hs.rhs2) : 0.0;
+  st->true_rel_residual = hs.rhs2 > 0 ? std::sqrt(rr_true / hs.rhs2) : 0.0;
+  return AMT_OK;
 }
 
 // out = a.b (device scalar), fixed-order reduction.
@@ -364,7 +641,11 @@ int solve_mixed(amt_problem* p, amt_solve_stats* st) {
 int do_solve(amt_problem* p, amt_solve_stats* st) {
   if (!p->loads_set) return fail(p, AMT_EINVAL, "loads not set (amt_set_loads)");
   if (!p->coeffs_ready) AMT_CHECK(prepare_coeffs(p));
-  int rc = p->cfg.precision == AMT_PREC_MIXED_FP32 ? solve_mixed(p, st) : solve_fp64(p, st);
+  int rc;
+  if (p->cfg.precond == AMT_PRECOND_MULTIGRID)
+    rc = solve_mg(p, st);
+  else
+    rc = p->cfg.precision == AMT_PREC_MIXED_FP32 ? solve_mixed(p, st) : solve_fp64(p, st);
   if (rc == AMT_OK) p->have_u = true;
   return rc;
 }
@@ -511,8 +792,8 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
     return fail(nullptr, AMT_EINVAL, "invalid element size or material");
   if (!(cfg->filter_radius >= 0) || cfg->filter_radius > 64)
     return fail(nullptr, AMT_EINVAL, "filter radius must be in [0, 64]");
-  if (cfg->precond != AMT_PRECOND_JACOBI)
-    return fail(nullptr, AMT_ENOTSUP, "only the Jacobi preconditioner is built in this path");
+  if (cfg->precond != AMT_PRECOND_JACOBI && cfg->precond != AMT_PRECOND_MULTIGRID)
+    return fail(nullptr, AMT_ENOTSUP, "preconditioner not available in this build");
   const int64_t n_n = (int64_t)(cfg->nx + 1) * (cfg->ny + 1) * (cfg->nz + 1);
   if (3 * n_n > (int64_t)INT32_MAX) return fail(nullptr, AMT_EINVAL, "grid too large");
   int ndev = 0;
@@ -608,6 +889,7 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
   for (int i = 0; i < 576; ++i) Kf[i] = (float)Ke[i];
   CREATE_TRY(copy_in(p, p->Ke64, Ke, sizeof(Ke)));
   CREATE_TRY(copy_in(p, p->Ke32, Kf, sizeof(Kf)));
+  if (cfg->precond == AMT_PRECOND_MULTIGRID) CREATE_TRY(mg_build_hierarchy(p, Ke));
   if (!st.empty()) {
     CREATE_TRY(copy_in(p, p->stencil, st.data(), st.size() * sizeof(StencilEntry)));
     CREATE_TRY(copy_in(p, p->stencil_w, sw.data(), sw.size() * sizeof(double)));

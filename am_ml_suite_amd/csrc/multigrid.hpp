@@ -1,0 +1,542 @@
+// multigrid.hpp — geometric multigrid V-cycle (FP32) used as the preconditioner of a flexible
+// CG whose vectors, operator and inner products stay FP64.
+//
+// Hierarchy: level 0 is the fine voxel grid, applied matrix-free (s_e * Ke0). Level l+1 has
+// ceil(e/2) elements per axis of level l; the prolongation P is trilinear (1, 1/2 weights per
+// axis; a last odd fine node interpolates from a coarse node one layer past the fine grid).
+// Coarse operators are Galerkin, A_{l+1} = P^T A_l P, stored as 27-point block stencils in
+// SoA layout A[(d*9 + 3i + j) * n_nodes + node] (d = neighbour offset, i/j = components):
+//  - level 1 straight from the element densities: a coarse element E holds 8 fine children c,
+//    and P restricted to E is the fixed interpolation Q_c, so P^T A_0 P over E equals
+//    sum_c s_c G_c with G_c = Q_c^T Ke0 Q_c (eight 24x24 tables built on the host); the
+//    penalty rows of the fine operator add P^T diag(pen) P;
+//  - level l >= 2 by the generic stencil triple product.
+// The coarsest level (a few hundred DOFs) is solved with a dense inverse computed on the host.
+// Smoothing is one damped-Jacobi sweep before and after the coarse correction (pre-smoother
+// started from zero), so the V-cycle is a symmetric linear operator.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace amt {
+
+struct MgGrid {
+  int ex, ey, ez;  // elements per axis
+  int nnx, nny, nnz;
+  int64_t n_nodes;
+};
+
+__device__ __forceinline__ void mg_decode(int64_t n, const MgGrid& g, int& i, int& j, int& k) {
+  k = (int)(n % g.nnz);
+  const int64_t t = n / g.nnz;
+  j = (int)(t % g.nny);
+  i = (int)(t / g.nny);
+}
+__device__ __forceinline__ int64_t mg_node(const MgGrid& g, int i, int j, int k) {
+  return ((int64_t)i * g.nny + j) * g.nnz + k;
+}
+// 1D trilinear prolongation weight of fine node f from coarse node c
+__device__ __forceinline__ float pw1(int f, int c) {
+  const int d = f - 2 * c;
+  return d == 0 ? 1.0f : ((d == 1 || d == -1) ? 0.5f : 0.0f);
+}
+
+// ---------------------------------------------------------------------------------------
+// Coarse operator construction (once per OC iteration)
+// ---------------------------------------------------------------------------------------
+
+// Level 1, thread (coarse node N, row component i = blockIdx.z, offset D = blockIdx.y).
+// G: 8 x 576 doubles (child c, row 3a+i, col 3b+j). s: fine element scales, C order.
+__global__ __launch_bounds__(kBlock) void k_mg_galerkin1(const double* __restrict__ s,
+                                                         GridDims fg,
+                                                         const double* __restrict__ G,
+                                                         const uint8_t* __restrict__ fixcnt,
+                                                         const double* __restrict__ P,
+                                                         MgGrid cg, float* __restrict__ A) {
+  const int64_t N = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (N >= cg.n_nodes) return;
+  const int dI = (int)blockIdx.y / 9 - 1, dJ = ((int)blockIdx.y / 3) % 3 - 1,
+            dK = (int)blockIdx.y % 3 - 1;
+  const int i = blockIdx.z;
+  int I, J, K;
+  mg_decode(N, cg, I, J, K);
+  const int MI = I + dI, MJ = J + dJ, MK = K + dK;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (MI >= 0 && MI < cg.nnx && MJ >= 0 && MJ < cg.nny && MK >= 0 && MK < cg.nnz) {
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int bx = dI + kOX[a], by = dJ + kOY[a], bz = dK + kOZ[a];
+      if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;  // uniform
+      int bl = 0;  // local index of M in E (element_kernel.cpp:196-199 node order)
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (kOX[t] == bx && kOY[t] == by && kOZ[t] == bz) bl = t;
+      const int EI = I - kOX[a], EJ = J - kOY[a], EK = K - kOZ[a];
+      if (EI < 0 || EI >= cg.ex || EJ < 0 || EJ >= cg.ey || EK < 0 || EK >= cg.ez) continue;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int fi = 2 * EI + kOX[c], fj = 2 * EJ + kOY[c], fk = 2 * EK + kOZ[c];
+        if (fi >= fg.nx || fj >= fg.ny || fk >= fg.nz) continue;
+        const double se = s[((int64_t)fi * fg.ny + fj) * fg.nz + fk];
+        const double* Gr = G + c * 576 + (3 * a + i) * 24 + 3 * bl;
+        acc[0] = fma(se, Gr[0], acc[0]);
+        acc[1] = fma(se, Gr[1], acc[1]);
+        acc[2] = fma(se, Gr[2], acc[2]);
+      }
+    }
+    // penalty rows: sum_m w_N(m) w_M(m) pen(m,i) on the diagonal component
+    const double pen = *P;
+    for (int mx = 2 * I - 1; mx <= 2 * I + 1; ++mx) {
+      if (mx < 0 || mx > fg.nx || pw1(mx, MI) == 0.0f) continue;
+      for (int my = 2 * J - 1; my <= 2 * J + 1; ++my) {
+        if (my < 0 || my > fg.ny || pw1(my, MJ) == 0.0f) continue;
+        for (int mz = 2 * K - 1; mz <= 2 * K + 1; ++mz) {
+          if (mz < 0 || mz > fg.nz || pw1(mz, MK) == 0.0f) continue;
+          const int64_t m = ((int64_t)mx * fg.nny + my) * fg.nnz + mz;
+          const int cnt = fixcnt[3 * m + i];
+          if (!cnt) continue;
+          const double w = (double)pw1(mx, I) * pw1(my, J) * pw1(mz, K) * pw1(mx, MI) *
+                           pw1(my, MJ) * pw1(mz, MK);
+          acc[i] = fma(w, cnt * pen, acc[i]);
+        }
+      }
+    }
+  }
+  const int d = blockIdx.y;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) A[(int64_t)(d * 9 + 3 * i + j) * cg.n_nodes + N] = (float)acc[j];
+}
+
+// Level l+1 from level l: thread (coarse N, offset D = blockIdx.y, row i = blockIdx.z).
+__global__ __launch_bounds__(kBlock) void k_mg_coarsen(const float* __restrict__ Af, MgGrid fg,
+                                                       MgGrid cg, float* __restrict__ Ac) {
+  const int64_t N = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (N >= cg.n_nodes) return;
+  const int dI = (int)blockIdx.y / 9 - 1, dJ = ((int)blockIdx.y / 3) % 3 - 1,
+            dK = (int)blockIdx.y % 3 - 1;
+  const int i = blockIdx.z;
+  int I, J, K;
+  mg_decode(N, cg, I, J, K);
+  const int MI = I + dI, MJ = J + dJ, MK = K + dK;
+  float acc[3] = {0.f, 0.f, 0.f};
+  if (MI >= 0 && MI < cg.nnx && MJ >= 0 && MJ < cg.nny && MK >= 0 && MK < cg.nnz) {
+    for (int ax = -1; ax <= 1; ++ax) {
+      const int mx = 2 * I + ax;
+      if (mx < 0 || mx >= fg.nnx) continue;
+      for (int ay = -1; ay <= 1; ++ay) {
+        const int my = 2 * J + ay;
+        if (my < 0 || my >= fg.nny) continue;
+        for (int az = -1; az <= 1; ++az) {
+          const int mz = 2 * K + az;
+          if (mz < 0 || mz >= fg.nnz) continue;
+          const float wN = (ax ? 0.5f : 1.f) * (ay ? 0.5f : 1.f) * (az ? 0.5f : 1.f);
+          const int64_t m = mg_node(fg, mx, my, mz);
+          for (int ddx = -1; ddx <= 1; ++ddx) {
+            const float wx = pw1(mx + ddx, MI);
+            if (wx == 0.f || mx + ddx < 0 || mx + ddx >= fg.nnx) continue;
+            for (int ddy = -1; ddy <= 1; ++ddy) {
+              const float wy = pw1(my + ddy, MJ);
+              if (wy == 0.f || my + ddy < 0 || my + ddy >= fg.nny) continue;
+              for (int ddz = -1; ddz <= 1; ++ddz) {
+                const float wz = pw1(mz + ddz, MK);
+                if (wz == 0.f || mz + ddz < 0 || mz + ddz >= fg.nnz) continue;
+                const int d = (ddx + 1) * 9 + (ddy + 1) * 3 + (ddz + 1);
+                const float w = wN * wx * wy * wz;
+                const float* row = Af + (int64_t)(d * 9 + 3 * i) * fg.n_nodes + m;
+                acc[0] = fmaf(w, row[0], acc[0]);
+                acc[1] = fmaf(w, row[fg.n_nodes], acc[1]);
+                acc[2] = fmaf(w, row[2 * fg.n_nodes], acc[2]);
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  const int d = blockIdx.y;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) Ac[(int64_t)(d * 9 + 3 * i + j) * cg.n_nodes + N] = acc[j];
+}
+
+// Level-0 damped-Jacobi weights: omega / (K_ii + cnt_i P); 0 on rows no element touches.
+__global__ void k_mg_wd0(const double* __restrict__ diag, const uint8_t* __restrict__ fixcnt,
+                         const double* __restrict__ P, float omega, float* __restrict__ wd,
+                         int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double d = diag[i];
+  for (int t = 0; t < fixcnt[i]; ++t) d += *P;
+  wd[i] = d > 0.0 ? (float)(omega / d) : 0.f;
+}
+
+// Damped-Jacobi weights of a stencil level: wd = omega / A_ii (0 on empty rows).
+__global__ void k_mg_stencil_wdinv(const float* __restrict__ A, MgGrid g, float omega,
+                                   float* __restrict__ wd) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= 3 * g.n_nodes) return;
+  const int64_t n = t / 3;
+  const int i = (int)(t % 3);
+  const float d = A[(int64_t)(13 * 9 + 4 * i) * g.n_nodes + n];
+  wd[t] = d > 0.f ? omega / d : 0.f;
+}
+
+// ---------------------------------------------------------------------------------------
+// V-cycle kernels. Every kernel returns at once when the outer solve is done.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void stencil_row_apply(const float* __restrict__ A,
+                                                  const MgGrid& g, int64_t n,
+                                                  const float (&nb)[27][3], float (&y)[3]) {
+  y[0] = y[1] = y[2] = 0.f;
+#pragma unroll
+  for (int d = 0; d < 27; ++d)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        y[i] = fmaf(A[(int64_t)(d * 9 + 3 * i + j) * g.n_nodes + n], nb[d][j], y[i]);
+}
+
+// Loads f(q) for the 27 neighbours of node n (0 outside the grid); f maps a DOF to a value.
+template <typename F>
+__device__ __forceinline__ void mg_gather(const MgGrid& g, int64_t n, int i, int j, int k, F f,
+                                          float (&nb)[27][3]) {
+  const int64_t sj = g.nnz, si = (int64_t)g.nny * g.nnz;
+#pragma unroll
+  for (int di = -1; di <= 1; ++di)
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+      for (int dk = -1; dk <= 1; ++dk) {
+        const int m = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
+        const bool ok = (i + di) >= 0 && (i + di) < g.nnx && (j + dj) >= 0 &&
+                        (j + dj) < g.nny && (k + dk) >= 0 && (k + dk) < g.nnz;
+        const int64_t q = 3 * (n + di * si + dj * sj + dk);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nb[m][c] = ok ? f(q + c) : 0.f;
+      }
+}
+
+// Stencil level: x0 = wd*rhs (pre-smoothing from zero); t = rhs - A x0.
+__global__ __launch_bounds__(kBlock) void k_mg_pre(const float* __restrict__ A, MgGrid g,
+                                                   const float* __restrict__ wd,
+                                                   const float* __restrict__ rhs,
+                                                   float* __restrict__ x0,
+                                                   float* __restrict__ t, const int* done) {
+  if (*done) return;
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  int i, j, k;
+  mg_decode(n, g, i, j, k);
+  float nb[27][3];
+  mg_gather(g, n, i, j, k, [&](int64_t q) { return wd[q] * rhs[q]; }, nb);
+  float y[3];
+  stencil_row_apply(A, g, n, nb, y);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    x0[3 * n + c] = nb[13][c];
+    t[3 * n + c] = rhs[3 * n + c] - y[c];
+  }
+}
+
+// Stencil level: out = x + wd*(rhs - A x) (post-smoothing).
+__global__ __launch_bounds__(kBlock) void k_mg_post(const float* __restrict__ A, MgGrid g,
+                                                    const float* __restrict__ wd,
+                                                    const float* __restrict__ rhs,
+                                                    const float* __restrict__ x,
+                                                    float* __restrict__ out, const int* done) {
+  if (*done) return;
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  int i, j, k;
+  mg_decode(n, g, i, j, k);
+  float nb[27][3];
+  mg_gather(g, n, i, j, k, [&](int64_t q) { return x[q]; }, nb);
+  float y[3];
+  stencil_row_apply(A, g, n, nb, y);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int64_t q = 3 * n + c;
+    out[q] = nb[13][c] + wd[q] * (rhs[q] - y[c]);
+  }
+}
+
+// Restriction r_c = P^T r_f (coarse node per thread).
+__global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict__ rf,
+                                                        MgGrid fg, float* __restrict__ rc,
+                                                        MgGrid cg, const int* done) {
+  if (*done) return;
+  const int64_t N = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (N >= cg.n_nodes) return;
+  int I, J, K;
+  mg_decode(N, cg, I, J, K);
+  float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ax = -1; ax <= 1; ++ax)
+#pragma unroll
+    for (int ay = -1; ay <= 1; ++ay)
+#pragma unroll
+      for (int az = -1; az <= 1; ++az) {
+        const int mx = 2 * I + ax, my = 2 * J + ay, mz = 2 * K + az;
+        if (mx < 0 || mx >= fg.nnx || my < 0 || my >= fg.nny || mz < 0 || mz >= fg.nnz) continue;
+        const float w = (ax ? 0.5f : 1.f) * (ay ? 0.5f : 1.f) * (az ? 0.5f : 1.f);
+        const int64_t m = mg_node(fg, mx, my, mz);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] = fmaf(w, rf[3 * m + c], acc[c]);
+      }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) rc[3 * N + c] = acc[c];
+}
+
+// Prolongation x_f += P x_c (fine node per thread).
+__global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__ xc, MgGrid cg,
+                                                       float* __restrict__ xf, MgGrid fg,
+                                                       const int* done) {
+  if (*done) return;
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= fg.n_nodes) return;
+  int i, j, k;
+  mg_decode(n, fg, i, j, k);
+  const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+  const int i1 = (i & 1) ? i0 + 1 : i0, j1 = (j & 1) ? j0 + 1 : j0, k1 = (k & 1) ? k0 + 1 : k0;
+  const float wi = (i & 1) ? 0.5f : 1.f, wj = (j & 1) ? 0.5f : 1.f, wk = (k & 1) ? 0.5f : 1.f;
+  float acc[3] = {0.f, 0.f, 0.f};
+  const int ci[2] = {i0, i1}, cj[2] = {j0, j1}, ck[2] = {k0, k1};
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        if ((a && !(i & 1)) || (b && !(j & 1)) || (c && !(k & 1))) continue;
+        const int64_t N = mg_node(cg, ci[a], cj[b], ck[c]);
+        const float w = wi * wj * wk;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[q] = fmaf(w, xc[3 * N + q], acc[q]);
+      }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) xf[3 * n + q] += acc[q];
+}
+
+// Coarsest level: x = Ainv r (dense, n <= 1024), one block.
+__global__ __launch_bounds__(kBlock) void k_mg_dense(const float* __restrict__ Ainv, int n,
+                                                     const float* __restrict__ r,
+                                                     float* __restrict__ x, const int* done) {
+  if (*done) return;
+  for (int row = threadIdx.x; row < n; row += kBlock) {
+    float acc = 0.f;
+    for (int c = 0; c < n; ++c) acc = fmaf(Ainv[(int64_t)row * n + c], r[c], acc);
+    x[row] = acc;
+  }
+}
+
+// Level 0 (matrix-free, FP32 s and Ke0): x0 = wd*r, t = r - K x0; r is the FP64 CG residual.
+__global__ __launch_bounds__(kBlock) void k_mg0_pre(const double* __restrict__ r, GridDims g,
+                                                    const float* __restrict__ s,
+                                                    const float* __restrict__ Ke,
+                                                    const uint8_t* __restrict__ fixcnt,
+                                                    const double* __restrict__ P,
+                                                    const float* __restrict__ wd,
+                                                    float* __restrict__ x0,
+                                                    float* __restrict__ t, const int* done) {
+  if (*done) return;
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  int i, j, k;
+  node_decode(n, g, i, j, k);
+  Nbr<float> nb;
+  {
+    const int64_t sj = g.nnz, si = (int64_t)g.nny * g.nnz;
+#pragma unroll
+    for (int di = -1; di <= 1; ++di)
+#pragma unroll
+      for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+        for (int dk = -1; dk <= 1; ++dk) {
+          const int m = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
+          const bool ok = (i + di) >= 0 && (i + di) <= g.nx && (j + dj) >= 0 &&
+                          (j + dj) <= g.ny && (k + dk) >= 0 && (k + dk) <= g.nz;
+          const int64_t q = 3 * (n + di * si + dj * sj + dk);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) nb.v[m][c] = ok ? wd[q + c] * (float)r[q + c] : 0.f;
+        }
+  }
+  float s8[8];
+  load_element_scales(s, g, i, j, k, s8);
+  float y[3];
+  apply_node(nb, s8, Ke, y);
+  const float pen = (float)(*P);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int64_t q = 3 * n + c;
+    y[c] = fmaf((float)fixcnt[q] * pen, nb.v[13][c], y[c]);
+    x0[q] = nb.v[13][c];
+    t[q] = (float)r[q] - y[c];
+  }
+}
+
+// Level 0 post-smoothing into the FP64 preconditioned residual z = x + wd*(r - K x), with
+// block partials of z.r and z.q (the flexible-CG coefficients; q may be null).
+__global__ __launch_bounds__(kBlock) void k_mg0_post(const double* __restrict__ r, GridDims g,
+                                                     const float* __restrict__ s,
+                                                     const float* __restrict__ Ke,
+                                                     const uint8_t* __restrict__ fixcnt,
+                                                     const double* __restrict__ P,
+                                                     const float* __restrict__ wd,
+                                                     const float* __restrict__ x,
+                                                     double* __restrict__ z,
+                                                     const double* __restrict__ q,
+                                                     double* partials, int stride,
+                                                     const int* done) {
+  if (*done) return;
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (n < g.n_nodes) {
+    int i, j, k;
+    node_decode(n, g, i, j, k);
+    Nbr<float> nb;
+    gather_nbr(x, g, n, i, j, k, nb);
+    float s8[8];
+    load_element_scales(s, g, i, j, k, s8);
+    float y[3];
+    apply_node(nb, s8, Ke, y);
+    const float pen = (float)(*P);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t qq = 3 * n + c;
+      y[c] = fmaf((float)fixcnt[qq] * pen, nb.v[13][c], y[c]);
+      const double rv = r[qq];
+      // wd == 0 marks a DOF no element touches (zero row): its correction stays 0
+      const double zv = wd[qq] == 0.f
+                            ? 0.0
+                            : (double)nb.v[13][c] + (double)wd[qq] * (rv - (double)y[c]);
+      z[qq] = zv;
+      acc[0] = fma(zv, rv, acc[0]);
+      if (q) acc[1] = fma(zv, q[qq], acc[1]);
+    }
+  }
+  store_partials<2>(acc, partials, stride);
+}
+
+// ---------------------------------------------------------------------------------------
+// Flexible CG around the V-cycle (FP64 vectors, operator and reductions)
+// ---------------------------------------------------------------------------------------
+struct FcgState {
+  double rz, rr, rhs2, thresh, beta, pq;
+  int it, done, converged, maxit;
+};
+
+// rr0 = |r0|^2 and bb = |b|^2 from device scalars.
+__global__ void k_fcg_start(FcgState* st, const double* rr, const double* bb, double tol,
+                            int maxit) {
+  if (threadIdx.x || blockIdx.x) return;
+  FcgState s;
+  s.rhs2 = *bb;
+  s.thresh = fmax(tol * tol * s.rhs2, 2.2250738585072014e-308);
+  s.rr = *rr;
+  s.rz = 0.0;
+  s.beta = 0.0;
+  s.pq = 0.0;
+  s.it = 0;
+  s.maxit = maxit;
+  const int conv = (s.rhs2 == 0.0) || (s.rr < s.thresh);
+  s.converged = conv;
+  s.done = conv || maxit <= 0;
+  *st = s;
+}
+
+// p = z + beta p_old; q = K p (FP64); partials p.q.
+__global__ __launch_bounds__(kBlock) void k_fcg_apply(
+    const double* __restrict__ z, const double* __restrict__ pold, double* __restrict__ pnew,
+    double* __restrict__ q, const double* __restrict__ s, const double* __restrict__ Ke,
+    const uint8_t* __restrict__ fixcnt, const double* __restrict__ P, GridDims g,
+    const FcgState* st, double* partials) {
+  if (st->done) return;
+  const double beta = st->beta;
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double acc[1] = {0.0};
+  if (n < g.n_nodes) {
+    int i, j, k;
+    node_decode(n, g, i, j, k);
+    Nbr<double> nb;
+    gather_dir(z, pold, beta, g, n, i, j, k, nb);
+    double s8[8];
+    load_element_scales(s, g, i, j, k, s8);
+    double yv[3];
+    apply_node(nb, s8, Ke, yv);
+    const double pen = *P;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      yv[c] = fma((double)fixcnt[3 * n + c] * pen, nb.v[13][c], yv[c]);
+      pnew[3 * n + c] = nb.v[13][c];
+      q[3 * n + c] = yv[c];
+      acc[0] = fma(nb.v[13][c], yv[c], acc[0]);
+    }
+  }
+  store_partials<1>(acc, partials, gridDim.x);
+}
+
+// alpha = rz / p.q; x += alpha p; r -= alpha q; partials |r|^2.
+__global__ __launch_bounds__(kBlock) void k_fcg_update(double* __restrict__ x,
+                                                       double* __restrict__ r,
+                                                       const double* __restrict__ p,
+                                                       const double* __restrict__ q, int64_t n,
+                                                       const FcgState* st,
+                                                       const double* __restrict__ apart,
+                                                       int nba, double* __restrict__ upart) {
+  if (st->done) return;
+  double pq[1];
+  reduce_partials<1>(apart, nba, nba, pq);
+  const double a = st->rz / pq[0];
+  double acc[1] = {0.0};
+  AMT_GRID_STRIDE(i, n) {
+    x[i] = fma(a, p[i], x[i]);
+    const double rv = fma(-a, q[i], r[i]);
+    r[i] = rv;
+    acc[0] = fma(rv, rv, acc[0]);
+  }
+  store_partials<1>(acc, upart, gridDim.x);
+}
+
+// One block after the V-cycle: convergence test on |r|^2, then the flexible (Polak-Ribiere /
+// Notay) beta = -(z.q)/(p.q) and rz = z.r. first=1: only rz from the first V-cycle.
+__global__ __launch_bounds__(kBlock) void k_fcg_scalar(FcgState* st,
+                                                       const double* __restrict__ upart,
+                                                       int nbu, const double* __restrict__ apart,
+                                                       int nba, const double* __restrict__ zpart,
+                                                       int nbz, int zstride, int first,
+                                                       int* done_flag) {
+  FcgState s = *st;
+  if (s.done) {
+    if (threadIdx.x == 0) *done_flag = 1;
+    return;
+  }
+  double zr[2];
+  reduce_partials<2>(zpart, nbz, zstride, zr);
+  if (first) {
+    if (threadIdx.x == 0) {
+      s.rz = zr[0];
+      s.beta = 0.0;
+      *st = s;
+    }
+    return;
+  }
+  double rr[1], pq[1];
+  reduce_partials<1>(upart, nbu, nbu, rr);
+  reduce_partials<1>(apart, nba, nba, pq);
+  if (threadIdx.x != 0) return;
+  s.rr = rr[0];
+  s.pq = pq[0];
+  if (rr[0] < s.thresh) {
+    s.done = 1;
+    s.converged = 1;
+  } else {
+    s.beta = -zr[1] / pq[0];
+    s.rz = zr[0];
+    s.it += 1;
+    if (s.it >= s.maxit) s.done = 1;
+  }
+  *st = s;
+  *done_flag = s.done;
+}
+
+}  // namespace amt

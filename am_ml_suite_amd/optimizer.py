@@ -61,6 +61,7 @@ class SIMPConfig:
     penalty_factor: float = 1e8
     density_index: str = "reference"  # or "consistent"
     precision: str = "fp64"           # or "mixed_fp32"
+    preconditioner: str = "jacobi"    # or "multigrid" (FP32 V-cycle inside FP64 flexible CG)
     warm_start: bool = False
     inner_tolerance: float = 1e-4
     inner_max_iterations: int = 20000
@@ -127,7 +128,8 @@ def make_amt_config(nx, ny, nz, element_size, material, config):
                        "consistent": _ffi.AMT_INDEX_CONSISTENT}[config.density_index]
     c.precision = {"fp64": _ffi.AMT_PREC_FP64,
                    "mixed_fp32": _ffi.AMT_PREC_MIXED_FP32}[config.precision]
-    c.precond = _ffi.AMT_PRECOND_JACOBI
+    c.precond = {"jacobi": _ffi.AMT_PRECOND_JACOBI, "unet": _ffi.AMT_PRECOND_UNET,
+                 "multigrid": _ffi.AMT_PRECOND_MULTIGRID}[config.preconditioner]
     c.warm_start = int(bool(config.warm_start))
     c.inner_tol = config.inner_tolerance
     c.inner_maxit = int(config.inner_max_iterations)

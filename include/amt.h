@@ -37,7 +37,10 @@ extern "C" {
 
 enum { AMT_INDEX_CONSISTENT = 0, AMT_INDEX_REF_COMPAT = 1 };
 enum { AMT_PREC_FP64 = 0, AMT_PREC_MIXED_FP32 = 1 };
-enum { AMT_PRECOND_JACOBI = 0, AMT_PRECOND_UNET = 1 };
+/* Preconditioners. JACOBI: the reference's solve with Jacobi in place of IC (FP64, or FP32
+ * inner solves in FP64 refinement with AMT_PREC_MIXED_FP32). MULTIGRID: flexible CG with FP64
+ * vectors, operator and reductions around an FP32 Galerkin geometric-multigrid V-cycle. */
+enum { AMT_PRECOND_JACOBI = 0, AMT_PRECOND_UNET = 1, AMT_PRECOND_MULTIGRID = 2 };
 
 typedef struct amt_problem amt_problem;
 
