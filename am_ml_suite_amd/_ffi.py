@@ -36,7 +36,7 @@ class AmtConfig(C.Structure):
         ("pcg_tol", C.c_double), ("pcg_maxit", C.c_int), ("penalty_factor", C.c_double),
         ("density_index", C.c_int), ("precision", C.c_int), ("precond", C.c_int),
         ("warm_start", C.c_int), ("inner_tol", C.c_double), ("inner_maxit", C.c_int),
-        ("verbose", C.c_int),
+        ("verbose", C.c_int), ("use_graphs", C.c_int),
     ]
 
 

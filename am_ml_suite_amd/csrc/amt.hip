@@ -33,11 +33,12 @@ namespace {
 
 thread_local std::string g_create_error;
 
-constexpr int kOcLevels = 5;
+constexpr int kOcLevels = 6;  // bisection levels evaluated per OC sweep launch
+constexpr int kOcBlocks = 256;  // OC sweep grid (partials per candidate)
 constexpr int kOcCand = (1 << kOcLevels) - 1;
 constexpr int kMaxReduceBlocks = 1024;
 constexpr int kChunk = 32;  // PCG iterations launched between two host polls
-constexpr int kMgChunk = 8;  // multigrid flexible-CG iterations between two host polls
+constexpr int kMgChunk = 4;  // flexible-CG iterations per captured graph (even)
 constexpr float kMgOmega = 0.6f;  // damped-Jacobi weight of the V-cycle smoother
 constexpr int64_t kMgCoarseDofs = 300;  // coarsen until a level has at most this many DOFs
 
@@ -57,6 +58,8 @@ inline double now_s() {
 struct amt_problem {
   amt_config cfg;
   GridDims g;
+  TileGrid tg;
+  unsigned nb_tile = 0;  // blocks of the LDS-tiled operator kernels
   int64_t n_e = 0, n_n = 0, n_dof = 0;
   hipStream_t stream = nullptr;
   std::string err;
@@ -99,6 +102,12 @@ struct amt_problem {
   int mg_nc = 0;              // coarsest DOFs
   FcgState* fcg = nullptr;
   std::vector<float> mg_host_stencil;
+  // one captured chunk of kMgChunk flexible-CG iterations, replayed per chunk
+  hipGraph_t fcg_graph = nullptr;
+  hipGraphExec_t fcg_exec = nullptr;
+  bool fcg_graph_failed = false;
+  int mg_last_iters = 0;  // iterations of the previous solve: sizes the first launch batch
+
 
   bool loads_set = false, design_ready = false, coeffs_ready = false, have_u = false;
   double n_design = 0;
@@ -385,32 +394,82 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q) {
   auto& mg = p->mg;
   const int* done = (const int*)((const char*)p->fcg + offsetof(FcgState, done));
   const size_t L = mg.size();
-  hipLaunchKernelGGL(k_mg0_pre, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, r, p->g,
+  hipLaunchKernelGGL(k_mg0_pre, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g, p->tg,
                      p->s32, p->Ke32, p->fixcnt, p->scal + S_P, mg[0].wd, mg[0].x, mg[0].t,
                      done);
   hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[1].g.n_nodes)), dim3(kBlock), 0, p->stream,
                      mg[0].t, mg[0].g, mg[1].rhs, mg[1].g, done);
   for (size_t l = 1; l + 1 < L; ++l) {
-    hipLaunchKernelGGL(k_mg_pre, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
+    hipLaunchKernelGGL(k_mg_pre, dim3(nblk(mg[l].g.n_nodes), 3), dim3(kBlock), 0, p->stream,
                        mg[l].A, mg[l].g, mg[l].wd, mg[l].rhs, mg[l].x, mg[l].t, done);
     hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[l + 1].g.n_nodes)), dim3(kBlock), 0,
                        p->stream, mg[l].t, mg[l].g, mg[l + 1].rhs, mg[l + 1].g, done);
   }
-  hipLaunchKernelGGL(k_mg_dense, dim3(1), dim3(kBlock), 0, p->stream, p->mg_ainv, p->mg_nc,
-                     mg[L - 1].rhs, mg[L - 1].x, done);
+  hipLaunchKernelGGL(k_mg_dense, dim3((p->mg_nc + kWaves - 1) / kWaves), dim3(kBlock), 0,
+                     p->stream, p->mg_ainv, p->mg_nc, mg[L - 1].rhs, mg[L - 1].x, done);
   const float* res = mg[L - 1].x;  // correction of the level below the current one
   for (size_t l = L - 2; l >= 1; --l) {
     hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
                        res, mg[l + 1].g, mg[l].x, mg[l].g, done);
-    hipLaunchKernelGGL(k_mg_post, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
+    hipLaunchKernelGGL(k_mg_post, dim3(nblk(mg[l].g.n_nodes), 3), dim3(kBlock), 0, p->stream,
                        mg[l].A, mg[l].g, mg[l].wd, mg[l].rhs, mg[l].x, mg[l].t, done);
     res = mg[l].t;
   }
   hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, res,
                      mg[1].g, mg[0].x, mg[0].g, done);
-  hipLaunchKernelGGL(k_mg0_post, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, r, p->g,
+  hipLaunchKernelGGL(k_mg0_post, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g, p->tg,
                      p->s32, p->Ke32, p->fixcnt, p->scal + S_P, mg[0].wd, mg[0].x, z, q,
                      p->part(R_INIT), p->stride_red, done);
+}
+
+// kMgChunk flexible-CG iterations (apply, update, V-cycle, scalar). kMgChunk is even, so
+// every chunk starts on the same p-buffer parity and one captured graph serves all chunks.
+void fcg_enqueue_chunk(amt_problem* p, bool eager_prof) {
+  const int nba = (int)p->nb_tile, nbu = (int)nred(p->n_dof);
+  int* done_flag = (int*)(p->scal + S_FLAG);
+  for (int t = 0; t < kMgChunk; ++t) {
+    hipEvent_t ev = nullptr;
+    if (eager_prof && p->prof_on && p->ev_used + 2 <= (int)p->ev_pool.size()) {
+      ev = p->ev_pool[p->ev_used];
+      (void)hipEventRecord(ev, p->stream);
+    }
+    hipLaunchKernelGGL(k_fcg_apply, dim3(nba), dim3(kBlock), 0, p->stream, p->z, p->p[t & 1],
+                       p->p[(t + 1) & 1], p->q, p->s64, p->Ke64, p->fixcnt, p->scal + S_P,
+                       p->g, p->tg, p->fcg, p->part(R_APPLY));
+    if (ev) prof_end(p, ev);
+    hipLaunchKernelGGL(k_fcg_update, dim3(nbu), dim3(kBlock), 0, p->stream, p->u, p->r,
+                       p->p[(t + 1) & 1], p->q, p->n_dof, p->fcg,
+                       (const double*)p->part(R_APPLY), nba, p->part(R_UPD));
+    mg_vcycle(p, p->r, p->z, p->q);
+    hipLaunchKernelGGL(k_fcg_scalar, dim3(1), dim3(kBlock), 0, p->stream, p->fcg,
+                       (const double*)p->part(R_UPD), nbu, (const double*)p->part(R_APPLY),
+                       nba, (const double*)p->part(R_INIT), nba, p->stride_red, 0, done_flag);
+  }
+}
+
+// Captures the chunk once per handle. Returns false if no graph could be built, in which
+// case the eager launches are used.
+bool fcg_graph_ready(amt_problem* p) {
+  if (p->fcg_exec) return true;
+  if (p->fcg_graph_failed) return false;
+  p->fcg_graph_failed = true;
+  if (hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  fcg_enqueue_chunk(p, false);
+  hipGraph_t g = nullptr;
+  const hipError_t e1 = hipStreamEndCapture(p->stream, &g);
+  if (e1 == hipSuccess && g &&
+      hipGraphInstantiate(&p->fcg_exec, g, nullptr, nullptr, 0) == hipSuccess) {
+    p->fcg_graph = g;
+    p->fcg_graph_failed = false;
+    return true;
+  }
+  (void)hipGetLastError();
+  if (g) (void)hipGraphDestroy(g);
+  p->fcg_exec = nullptr;
+  return false;
 }
 
 // Flexible CG (FP64) preconditioned by the V-cycle, from x = 0 or the previous displacement.
@@ -425,35 +484,37 @@ int solve_mg(amt_problem* p, amt_solve_stats* st) {
                      p->scal + S_BB, p->cfg.pcg_tol, p->cfg.pcg_maxit);
   hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->p[0],
                      0.0, p->n_dof);
-  const int nba = (int)nblk(p->n_n), nbu = (int)nred(p->n_dof);
+  const int nba = (int)p->nb_tile, nbu = (int)nred(p->n_dof);
   int* done_flag = (int*)(p->scal + S_FLAG);
-  mg_vcycle(p, p->r, p->z, nullptr);
+  mg_vcycle(p, p->r, p->z, nullptr);  // z0 = M r0
   hipLaunchKernelGGL(k_fcg_scalar, dim3(1), dim3(kBlock), 0, p->stream, p->fcg,
                      (const double*)p->part(R_UPD), nbu, (const double*)p->part(R_APPLY), nba,
                      (const double*)p->part(R_INIT), nba, p->stride_red, 1, done_flag);
   AMT_CHECK(check_launch(p, "fcg init"));
   FcgState hs;
   AMT_CHECK(copy_out(p, &hs, p->fcg, sizeof(FcgState)));
-  int t = 0;
+  const bool graphs = p->cfg.use_graphs && fcg_graph_ready(p);
+  // With profiling on, the first chunk of each solve is launched eagerly with an event pair
+  // around every operator launch (a live sample inside the timed run); the rest replay the
+  // captured graph.
+  // The first batch runs as many chunks as the previous solve needed (minus one), so a
+  // warm-started solve usually finishes after one or two host polls; kernels launched past
+  // convergence return at once.
+  bool first = true;
   while (!hs.done) {
-    for (int c = 0; c < kMgChunk; ++c, ++t) {
-      hipEvent_t ev;
-      prof_begin(p, &ev);
-      hipLaunchKernelGGL(k_fcg_apply, dim3(nba), dim3(kBlock), 0, p->stream, p->z,
-                         p->p[t & 1], p->p[(t + 1) & 1], p->q, p->s64, p->Ke64, p->fixcnt,
-                         p->scal + S_P, p->g, p->fcg, p->part(R_APPLY));
-      prof_end(p, ev);
-      hipLaunchKernelGGL(k_fcg_update, dim3(nbu), dim3(kBlock), 0, p->stream, p->u, p->r,
-                         p->p[(t + 1) & 1], p->q, p->n_dof, p->fcg,
-                         (const double*)p->part(R_APPLY), nba, p->part(R_UPD));
-      mg_vcycle(p, p->r, p->z, p->q);
-      hipLaunchKernelGGL(k_fcg_scalar, dim3(1), dim3(kBlock), 0, p->stream, p->fcg,
-                         (const double*)p->part(R_UPD), nbu, (const double*)p->part(R_APPLY),
-                         nba, (const double*)p->part(R_INIT), nba, p->stride_red, 0, done_flag);
+    const int chunks = first ? std::max(1, p->mg_last_iters / kMgChunk - 1) : 1;
+    for (int c = 0; c < chunks; ++c) {
+      if (graphs && !(p->prof_on && first && c == 0)) {
+        HIP_TRY(p, hipGraphLaunch(p->fcg_exec, p->stream));
+      } else {
+        fcg_enqueue_chunk(p, true);
+      }
     }
+    first = false;
     AMT_CHECK(check_launch(p, "fcg chunk"));
     AMT_CHECK(copy_out(p, &hs, p->fcg, sizeof(FcgState)));
   }
+  p->mg_last_iters = hs.it;
   prof_harvest(p);
   double rr_true = 0.0;
   AMT_CHECK(residual64(p, p->u, p->q, &rr_true));
@@ -505,7 +566,7 @@ template <typename T, typename TK, typename TX>
 int pcg_run(amt_problem* p, const double* b, TX* x, T* r, T* z, T** pp, T* q, const T* s,
             const TK* Ke, const T* dinv, double tol, int maxit, PcgState* out) {
   const GridDims g = p->g;
-  const int nbu = (int)nred(p->n_dof), nba = (int)nblk(p->n_n);
+  const int nbu = (int)nred(p->n_dof), nba = (int)p->nb_tile;
   hipLaunchKernelGGL(k_fill<T>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, pp[0], T(0),
                      p->n_dof);
   hipLaunchKernelGGL((k_pcg_init<T>), dim3(nbu), dim3(kBlock), 0, p->stream, b, r, z, dinv,
@@ -523,7 +584,7 @@ int pcg_run(amt_problem* p, const double* b, TX* x, T* r, T* z, T** pp, T* q, co
       prof_begin(p, &ev);
       hipLaunchKernelGGL((k_pcg_apply<T, TK>), dim3(nba), dim3(kBlock), 0, p->stream, t, z,
                          pp[t & 1], pp[(t + 1) & 1], q, s, Ke, p->fixcnt, p->scal + S_P, g,
-                         p->pcg, (const double*)p->part(R_UPD), nbu, p->part(R_APPLY),
+                         p->tg, p->pcg, (const double*)p->part(R_UPD), nbu, p->part(R_APPLY),
                          p->stride_red);
       prof_end(p, ev);
       hipLaunchKernelGGL((k_pcg_update<T, TX>), dim3(nbu), dim3(kBlock), 0, p->stream, t, x, r,
@@ -543,11 +604,10 @@ int pcg_run(amt_problem* p, const double* b, TX* x, T* r, T* z, T** pp, T* q, co
 
 // FP64 residual r_out = Fb - K x and |r|^2 into scal[S_RR] (blocking read).
 int residual64(amt_problem* p, const double* x, double* rout, double* rr_host) {
-  hipLaunchKernelGGL(k_residual, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, x, p->Fb,
-                     rout, p->s64, p->Ke64, p->fixcnt, p->scal + S_P, p->g, p->part(R_APPLY),
-                     p->stride_apply);
+  hipLaunchKernelGGL(k_residual, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, x, p->Fb, rout,
+                     p->s64, p->Ke64, p->fixcnt, p->scal + S_P, p->g, p->tg, p->part(R_APPLY));
   hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kBlock), 0, p->stream, p->part(R_APPLY),
-                     (int)nblk(p->n_n), p->stride_apply, p->scal + S_RR);
+                     (int)p->nb_tile, (int)p->nb_tile, p->scal + S_RR);
   AMT_CHECK(check_launch(p, "k_residual"));
   if (rr_host) {
     HIP_TRY(p, hipMemcpyAsync(rr_host, p->scal + S_RR, sizeof(double), hipMemcpyDeviceToHost,
@@ -673,7 +733,7 @@ int oc_update(amt_problem* p, OcState* hs) {
   init.vf = p->cfg.volume_fraction;
   init.n_design = p->n_design > 0 ? p->n_design : (double)p->n_e;  // :222-225 fallback
   HIP_TRY(p, hipMemcpyAsync(p->oc, &init, sizeof(OcState), hipMemcpyHostToDevice, p->stream));
-  const unsigned nb = nred(p->n_e);
+  const unsigned nb = std::min<unsigned>(nred(p->n_e), kOcBlocks);
   // ceil(100 / LEV) + 1 sweeps always suffice: each one advances LEV passes or finishes
   for (int sw = 0; sw < (100 + kOcLevels - 1) / kOcLevels + 1; ++sw) {
     hipLaunchKernelGGL((k_oc_sweep<kOcLevels>), dim3(nb), dim3(kBlock), 0, p->stream, p->rho,
@@ -769,6 +829,7 @@ void amt_config_default(amt_config* c) {
   c->inner_tol = 1e-4;
   c->inner_maxit = 20000;
   c->verbose = 0;
+  c->use_graphs = 1;
 }
 
 int amt_element_stiffness(double E, double nu, double h, double* Ke) {
@@ -855,7 +916,10 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
   }
   CREATE_TRY(dalloc(p, &p->dc, ne));
   CREATE_TRY(dalloc(p, &p->dcf, ne));
-  p->stride_apply = (int)std::max<int64_t>(nblk(p->n_n), nblk(ne));
+  p->tg = make_tile_grid(p->g);
+  p->nb_tile = 8u * (unsigned)p->tg.per_xcd;
+  p->stride_apply = (int)std::max<int64_t>({(int64_t)nblk(p->n_n), (int64_t)nblk(ne),
+                                            (int64_t)p->nb_tile});
   p->stride_red = std::max<int>(kMaxReduceBlocks, p->stride_apply);
   {
     // R_APPLY: 1 value per operator block; R_UPD: 2; R_INIT: 3; R_OC: kOcCand; R_MISC: 2
@@ -911,6 +975,9 @@ int amt_destroy(amt_problem* p) {
   if (!p) return AMT_OK;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   for (auto e : p->ev_pool) (void)hipEventDestroy(e);
+  if (p->fcg_exec) (void)hipGraphExecDestroy(p->fcg_exec);
+  if (p->fcg_graph) (void)hipGraphDestroy(p->fcg_graph);
+
   for (void* a : p->allocs) (void)hipFree(a);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
@@ -1016,9 +1083,9 @@ int amt_apply_operator(amt_problem* p, const double* x, double* y, int with_bc) 
   if (with_bc && !p->loads_set) return fail(p, AMT_EINVAL, "loads not set (amt_set_loads)");
   if (!p->coeffs_ready) AMT_CHECK(prepare_coeffs(p));
   AMT_CHECK(copy_in(p, p->z, x, sizeof(double) * p->n_dof));
-  hipLaunchKernelGGL((k_apply<double, double>), dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream,
+  hipLaunchKernelGGL((k_apply<double, double>), dim3(p->nb_tile), dim3(kBlock), 0, p->stream,
                      p->z, p->q, p->s64, p->Ke64, p->fixcnt,
-                     with_bc ? (const double*)(p->scal + S_P) : nullptr, p->g);
+                     with_bc ? (const double*)(p->scal + S_P) : nullptr, p->g, p->tg);
   AMT_CHECK(check_launch(p, "k_apply"));
   return copy_out(p, y, p->q, sizeof(double) * p->n_dof);
 }

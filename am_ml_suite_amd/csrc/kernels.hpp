@@ -210,49 +210,9 @@ __device__ __forceinline__ void apply_node(const Nbr<T>& nb, const T (&s8)[8],
   }
 }
 
-template <typename T>
-__device__ __forceinline__ void gather_nbr(const T* __restrict__ x, const GridDims& g,
-                                           int64_t n, int i, int j, int k, Nbr<T>& nb) {
-  const int64_t sj = g.nnz, si = (int64_t)g.nny * g.nnz;
-#pragma unroll
-  for (int di = -1; di <= 1; ++di)
-#pragma unroll
-    for (int dj = -1; dj <= 1; ++dj)
-#pragma unroll
-      for (int dk = -1; dk <= 1; ++dk) {
-        const int m = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
-        const bool ok = (i + di) >= 0 && (i + di) <= g.nx && (j + dj) >= 0 &&
-                        (j + dj) <= g.ny && (k + dk) >= 0 && (k + dk) <= g.nz;
-        const int64_t q = 3 * (n + di * si + dj * sj + dk);
-        nb.v[m][0] = ok ? x[q + 0] : T(0);
-        nb.v[m][1] = ok ? x[q + 1] : T(0);
-        nb.v[m][2] = ok ? x[q + 2] : T(0);
-      }
-}
-
-// Same gather, but the value read is p = z + beta*p_old (the CG direction update done on
-// the fly, so the next operator launch needs no separate p-update kernel). Every thread
-// evaluates the identical fma for a given DOF, so the owner writes exactly what its
-// neighbours used.
-template <typename T>
-__device__ __forceinline__ void gather_dir(const T* __restrict__ z, const T* __restrict__ pold,
-                                           T beta, const GridDims& g, int64_t n, int i, int j,
-                                           int k, Nbr<T>& nb) {
-  const int64_t sj = g.nnz, si = (int64_t)g.nny * g.nnz;
-#pragma unroll
-  for (int di = -1; di <= 1; ++di)
-#pragma unroll
-    for (int dj = -1; dj <= 1; ++dj)
-#pragma unroll
-      for (int dk = -1; dk <= 1; ++dk) {
-        const int m = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
-        const bool ok = (i + di) >= 0 && (i + di) <= g.nx && (j + dj) >= 0 &&
-                        (j + dj) <= g.ny && (k + dk) >= 0 && (k + dk) <= g.nz;
-        const int64_t q = 3 * (n + di * si + dj * sj + dk);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) nb.v[m][c] = ok ? fma(beta, pold[q + c], z[q + c]) : T(0);
-      }
-}
+}  // namespace amt
+#include "tile_op.hpp"
+namespace amt {
 
 // y = K x (+ penalty rows when P != null): fixcnt[dof] * P * x[dof]
 template <typename T, typename TK>
@@ -260,15 +220,21 @@ __global__ __launch_bounds__(kBlock) void k_apply(const T* __restrict__ x, T* __
                                                   const T* __restrict__ s,
                                                   const TK* __restrict__ Ke,
                                                   const uint8_t* __restrict__ fixcnt,
-                                                  const double* __restrict__ P, GridDims g) {
-  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (n >= g.n_nodes) return;
-  int i, j, k;
-  node_decode(n, g, i, j, k);
+                                                  const double* __restrict__ P, GridDims g,
+                                                  TileGrid tg) {
+  __shared__ TileSmem<T> sm;
+  int i0, j0, k0;
+  if (!tile_origin(tg, i0, j0, k0)) return;  // whole block: no barrier follows for it
+  tile_load(sm, g, i0, j0, k0, [&](int64_t q) { return x[q]; }, s);
+  __syncthreads();
+  int li, lj, lk;
+  tile_thread(li, lj, lk);
+  const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+  if (i > g.nx || j > g.ny || k > g.nz) return;
+  const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
   Nbr<T> nb;
-  gather_nbr(x, g, n, i, j, k, nb);
   T s8[8];
-  load_element_scales(s, g, i, j, k, s8);
+  tile_fetch(sm, li, lj, lk, nb, s8);
   T yv[3];
   apply_node(nb, s8, Ke, yv);
   if (P) {
@@ -280,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_apply(const T* __restrict__ x, T* __
   for (int c = 0; c < 3; ++c) y[3 * n + c] = yv[c];
 }
 
-// r = b - K x (with penalty); block partials of |r|^2.
+// r = b - K x (with penalty); block partials of |r|^2 (one per block of the tile grid).
 __global__ __launch_bounds__(kBlock) void k_residual(const double* __restrict__ x,
                                                      const double* __restrict__ b,
                                                      double* __restrict__ r,
@@ -288,16 +254,21 @@ __global__ __launch_bounds__(kBlock) void k_residual(const double* __restrict__ 
                                                      const double* __restrict__ Ke,
                                                      const uint8_t* __restrict__ fixcnt,
                                                      const double* __restrict__ P, GridDims g,
-                                                     double* partials, int stride) {
-  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+                                                     TileGrid tg, double* partials) {
+  __shared__ TileSmem<double> sm;
+  int i0, j0, k0;
+  const bool has = tile_origin(tg, i0, j0, k0);
+  if (has) tile_load(sm, g, i0, j0, k0, [&](int64_t q) { return x[q]; }, s);
+  __syncthreads();
+  int li, lj, lk;
+  tile_thread(li, lj, lk);
+  const int i = i0 + li, j = j0 + lj, k = k0 + lk;
   double acc[1] = {0.0};
-  if (n < g.n_nodes) {
-    int i, j, k;
-    node_decode(n, g, i, j, k);
+  if (has && i <= g.nx && j <= g.ny && k <= g.nz) {
+    const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
     Nbr<double> nb;
-    gather_nbr(x, g, n, i, j, k, nb);
     double s8[8];
-    load_element_scales(s, g, i, j, k, s8);
+    tile_fetch(sm, li, lj, lk, nb, s8);
     double yv[3];
     apply_node(nb, s8, Ke, yv);
     const double pen = *P;
@@ -309,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(const double* __restrict__ 
       acc[0] = fma(rv, rv, acc[0]);
     }
   }
-  store_partials<1>(acc, partials, stride);
+  store_partials<1>(acc, partials, gridDim.x);
 }
 
 // Diagonal of K without penalty: d = sum_a s8[a] Ke0[3a+c,3a+c]; block partials of max|d|.
@@ -439,12 +410,13 @@ __device__ __forceinline__ PcgState pcg_step_state(const PcgState& cur, int t,
   return nx;
 }
 
-// Iteration t, first half: p = z + beta p_old (gathered on the fly), q = K p, partials p.q.
+// Iteration t, first half: p = z + beta p_old (formed once per node in the LDS tile),
+// q = K p, partials p.q (one per block of the tile grid).
 template <typename T, typename TK>
 __global__ __launch_bounds__(kBlock) void k_pcg_apply(
     int t, const T* __restrict__ z, const T* __restrict__ pold, T* __restrict__ pnew,
     T* __restrict__ q, const T* __restrict__ s, const TK* __restrict__ Ke,
-    const uint8_t* __restrict__ fixcnt, const double* __restrict__ P, GridDims g,
+    const uint8_t* __restrict__ fixcnt, const double* __restrict__ P, GridDims g, TileGrid tg,
     PcgState* S, const double* __restrict__ upart, int nbu, double* __restrict__ apart,
     int stride) {
   const PcgState cur = S[t & 1];
@@ -457,15 +429,21 @@ __global__ __launch_bounds__(kBlock) void k_pcg_apply(
   if (blockIdx.x == 0 && threadIdx.x == 0) S[(t + 1) & 1] = nx;
   if (nx.done) return;
   const T beta = (T)beta_d;
-  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  __shared__ TileSmem<T> sm;
+  int i0, j0, k0;
+  const bool has = tile_origin(tg, i0, j0, k0);
+  if (has)
+    tile_load(sm, g, i0, j0, k0, [&](int64_t qq) { return fma(beta, pold[qq], z[qq]); }, s);
+  __syncthreads();
+  int li, lj, lk;
+  tile_thread(li, lj, lk);
+  const int i = i0 + li, j = j0 + lj, k = k0 + lk;
   double acc[1] = {0.0};
-  if (n < g.n_nodes) {
-    int i, j, k;
-    node_decode(n, g, i, j, k);
+  if (has && i <= g.nx && j <= g.ny && k <= g.nz) {
+    const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
     Nbr<T> nb;
-    gather_dir(z, pold, beta, g, n, i, j, k, nb);
     T s8[8];
-    load_element_scales(s, g, i, j, k, s8);
+    tile_fetch(sm, li, lj, lk, nb, s8);
     T yv[3];
     apply_node(nb, s8, Ke, yv);
     const T pen = (T)(*P);
@@ -643,25 +621,25 @@ __device__ __forceinline__ double oc_candidate(double r, double dcf, double lmid
   return std_max(rho_min, std_max(r - move, std_min(1.0, std_min(r + move, c))));
 }
 
-template <int LEV>
-__device__ __forceinline__ void oc_tree(double l1, double l2, double (&lam)[1 << LEV]) {
-  // heap order: node 1 = [l1,l2]; node 2j = left half (l2 := mid), 2j+1 = right (l1 := mid)
-  double lo[1 << LEV], hi[1 << LEV];
-  lo[1] = l1;
-  hi[1] = l2;
-#pragma unroll
-  for (int nd = 1; nd < (1 << LEV); ++nd) {
-    lam[nd] = 0.5 * (lo[nd] + hi[nd]);
-    if (2 * nd < (1 << LEV)) {
-      lo[2 * nd] = lo[nd];
-      hi[2 * nd] = lam[nd];
-      lo[2 * nd + 1] = lam[nd];
-      hi[2 * nd + 1] = hi[nd];
-    }
+// Midpoint of bisection-tree node nd (heap order: node 1 = [l1,l2], child 2j = left half where
+// l2 := mid, 2j+1 = right half where l1 := mid), with the reference's (l1+l2)/2 arithmetic
+// along the path, so it equals the lambda a sequential bisection would evaluate there.
+__device__ __forceinline__ double oc_tree_mid(double l1, double l2, int nd) {
+  int depth = 31 - __clz(nd);
+  for (int b = depth - 1; b >= 0; --b) {
+    const double m = 0.5 * (l1 + l2);
+    if ((nd >> b) & 1)
+      l1 = m;
+    else
+      l2 = m;
   }
-  lam[0] = 0.0;
+  return 0.5 * (l1 + l2);
 }
 
+// Candidate volumes for the 2^LEV - 1 midpoints of the next LEV bisection levels. For the
+// volume sums the candidate uses rho * sqrt(-dc) / sqrt(lambda + 1e-20) (one sqrt per element,
+// one multiply per candidate); k_oc_final recomputes the density at the chosen lambda with
+// the reference's own expression.
 template <int LEV>
 __global__ __launch_bounds__(kBlock) void k_oc_sweep(const double* __restrict__ rho,
                                                      const double* __restrict__ dcf,
@@ -671,16 +649,21 @@ __global__ __launch_bounds__(kBlock) void k_oc_sweep(const double* __restrict__ 
                                                      int stride) {
   if (st->done) return;
   constexpr int NC = (1 << LEV) - 1;
-  double lam[1 << LEV];
-  oc_tree<LEV>(st->l1, st->l2, lam);
+  __shared__ double gfac[NC];
+  if (threadIdx.x < NC)
+    gfac[threadIdx.x] = 1.0 / sqrt(oc_tree_mid(st->l1, st->l2, threadIdx.x + 1) + 1e-20);
+  __syncthreads();
   double vol[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) vol[c] = 0.0;
   AMT_GRID_STRIDE(e, n) {
     if (!design[e]) continue;
-    const double r = rho[e], d = dcf[e];
+    const double r = rho[e];
+    const double rs = r * sqrt(-dcf[e]);
+    const double lo = r - move, hi = r + move;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) vol[c] += oc_candidate(r, d, lam[c + 1], move, rho_min);
+    for (int c = 0; c < NC; ++c)
+      vol[c] += std_max(rho_min, std_max(lo, std_min(1.0, std_min(hi, rs * gfac[c]))));
   }
   store_partials<NC>(vol, partials, stride);
 }
@@ -693,9 +676,12 @@ __global__ __launch_bounds__(kBlock) void k_oc_walk(OcState* st,
                                                     int nb, int stride) {
   if (st->done) return;
   constexpr int NC = (1 << LEV) - 1;
+  __shared__ double tots[NC];
   double tot[NC];
   reduce_partials<NC>(partials, nb, stride, tot);
   if (threadIdx.x != 0) return;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) tots[c] = tot[c];
   double l1 = st->l1, l2 = st->l2;
   int nd = 1, passes = st->passes, done = 0;
   for (int lev = 0; lev < LEV; ++lev) {
@@ -703,10 +689,10 @@ __global__ __launch_bounds__(kBlock) void k_oc_walk(OcState* st,
       done = 1;
       break;
     }
-    const double lm = 0.5 * (l1 + l2);  // == lam[nd] of the sweep
+    const double lm = 0.5 * (l1 + l2);  // == oc_tree_mid(st->l1, st->l2, nd)
     st->lmid = lm;
     ++passes;
-    if (tot[nd - 1] / st->n_design > st->vf) {
+    if (tots[nd - 1] / st->n_design > st->vf) {
       l1 = lm;
       nd = 2 * nd + 1;
     } else {

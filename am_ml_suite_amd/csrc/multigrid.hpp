@@ -216,7 +216,20 @@ __device__ __forceinline__ void mg_gather(const MgGrid& g, int64_t n, int i, int
       }
 }
 
-// Stencil level: x0 = wd*rhs (pre-smoothing from zero); t = rhs - A x0.
+// Row i of the stencil product at node n: sum_d sum_j A[d][i][j] nb[d][j].
+__device__ __forceinline__ float stencil_row(const float* __restrict__ A, const MgGrid& g,
+                                             int64_t n, int i, const float (&nb)[27][3]) {
+  float y = 0.f;
+#pragma unroll
+  for (int d = 0; d < 27; ++d)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      y = fmaf(A[(int64_t)(d * 9 + 3 * i + j) * g.n_nodes + n], nb[d][j], y);
+  return y;
+}
+
+// Stencil level, thread (node, row i = blockIdx.y): x0 = wd*rhs (pre-smoothing from zero);
+// t = rhs - A x0. Three threads per node keep the small coarse grids busy.
 __global__ __launch_bounds__(kBlock) void k_mg_pre(const float* __restrict__ A, MgGrid g,
                                                    const float* __restrict__ wd,
                                                    const float* __restrict__ rhs,
@@ -225,20 +238,18 @@ __global__ __launch_bounds__(kBlock) void k_mg_pre(const float* __restrict__ A, 
   if (*done) return;
   const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (n >= g.n_nodes) return;
+  const int r = blockIdx.y;
   int i, j, k;
   mg_decode(n, g, i, j, k);
   float nb[27][3];
   mg_gather(g, n, i, j, k, [&](int64_t q) { return wd[q] * rhs[q]; }, nb);
-  float y[3];
-  stencil_row_apply(A, g, n, nb, y);
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    x0[3 * n + c] = nb[13][c];
-    t[3 * n + c] = rhs[3 * n + c] - y[c];
-  }
+  const float y = stencil_row(A, g, n, r, nb);
+  const int64_t q = 3 * n + r;
+  x0[q] = wd[q] * rhs[q];  // == nb[13][r]; read directly to keep nb in registers
+  t[q] = rhs[q] - y;
 }
 
-// Stencil level: out = x + wd*(rhs - A x) (post-smoothing).
+// Stencil level, thread (node, row): out = x + wd*(rhs - A x) (post-smoothing).
 __global__ __launch_bounds__(kBlock) void k_mg_post(const float* __restrict__ A, MgGrid g,
                                                     const float* __restrict__ wd,
                                                     const float* __restrict__ rhs,
@@ -247,17 +258,14 @@ __global__ __launch_bounds__(kBlock) void k_mg_post(const float* __restrict__ A,
   if (*done) return;
   const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (n >= g.n_nodes) return;
+  const int r = blockIdx.y;
   int i, j, k;
   mg_decode(n, g, i, j, k);
   float nb[27][3];
   mg_gather(g, n, i, j, k, [&](int64_t q) { return x[q]; }, nb);
-  float y[3];
-  stencil_row_apply(A, g, n, nb, y);
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const int64_t q = 3 * n + c;
-    out[q] = nb[13][c] + wd[q] * (rhs[q] - y[c]);
-  }
+  const float y = stencil_row(A, g, n, r, nb);
+  const int64_t q = 3 * n + r;
+  out[q] = x[q] + wd[q] * (rhs[q] - y);
 }
 
 // Restriction r_c = P^T r_f (coarse node per thread).
@@ -317,21 +325,26 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__
   for (int q = 0; q < 3; ++q) xf[3 * n + q] += acc[q];
 }
 
-// Coarsest level: x = Ainv r (dense, n <= 1024), one block.
+// Coarsest level: x = Ainv r (dense, n <= 1024); one wave per row, lanes split the columns,
+// four rows per block so the rows' loads are in flight together.
 __global__ __launch_bounds__(kBlock) void k_mg_dense(const float* __restrict__ Ainv, int n,
                                                      const float* __restrict__ r,
                                                      float* __restrict__ x, const int* done) {
   if (*done) return;
-  for (int row = threadIdx.x; row < n; row += kBlock) {
-    float acc = 0.f;
-    for (int c = 0; c < n; ++c) acc = fmaf(Ainv[(int64_t)row * n + c], r[c], acc);
-    x[row] = acc;
-  }
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (row >= n) return;
+  float acc = 0.f;
+  for (int c = lane; c < n; c += 64) acc = fmaf(Ainv[(int64_t)row * n + c], r[c], acc);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0) x[row] = acc;
 }
 
-// Level 0 (matrix-free, FP32 s and Ke0): x0 = wd*r, t = r - K x0; r is the FP64 CG residual.
+// Level 0 (matrix-free, FP32 s and Ke0, LDS tile): x0 = wd*r, t = r - K x0; r is the FP64
+// CG residual.
 __global__ __launch_bounds__(kBlock) void k_mg0_pre(const double* __restrict__ r, GridDims g,
-                                                    const float* __restrict__ s,
+                                                    TileGrid tg, const float* __restrict__ s,
                                                     const float* __restrict__ Ke,
                                                     const uint8_t* __restrict__ fixcnt,
                                                     const double* __restrict__ P,
@@ -339,29 +352,19 @@ __global__ __launch_bounds__(kBlock) void k_mg0_pre(const double* __restrict__ r
                                                     float* __restrict__ x0,
                                                     float* __restrict__ t, const int* done) {
   if (*done) return;
-  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (n >= g.n_nodes) return;
-  int i, j, k;
-  node_decode(n, g, i, j, k);
+  __shared__ TileSmem<float> sm;
+  int i0, j0, k0;
+  if (!tile_origin(tg, i0, j0, k0)) return;
+  tile_load(sm, g, i0, j0, k0, [&](int64_t q) { return wd[q] * (float)r[q]; }, s);
+  __syncthreads();
+  int li, lj, lk;
+  tile_thread(li, lj, lk);
+  const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+  if (i > g.nx || j > g.ny || k > g.nz) return;
+  const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
   Nbr<float> nb;
-  {
-    const int64_t sj = g.nnz, si = (int64_t)g.nny * g.nnz;
-#pragma unroll
-    for (int di = -1; di <= 1; ++di)
-#pragma unroll
-      for (int dj = -1; dj <= 1; ++dj)
-#pragma unroll
-        for (int dk = -1; dk <= 1; ++dk) {
-          const int m = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
-          const bool ok = (i + di) >= 0 && (i + di) <= g.nx && (j + dj) >= 0 &&
-                          (j + dj) <= g.ny && (k + dk) >= 0 && (k + dk) <= g.nz;
-          const int64_t q = 3 * (n + di * si + dj * sj + dk);
-#pragma unroll
-          for (int c = 0; c < 3; ++c) nb.v[m][c] = ok ? wd[q + c] * (float)r[q + c] : 0.f;
-        }
-  }
   float s8[8];
-  load_element_scales(s, g, i, j, k, s8);
+  tile_fetch(sm, li, lj, lk, nb, s8);
   float y[3];
   apply_node(nb, s8, Ke, y);
   const float pen = (float)(*P);
@@ -377,7 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_mg0_pre(const double* __restrict__ r
 // Level 0 post-smoothing into the FP64 preconditioned residual z = x + wd*(r - K x), with
 // block partials of z.r and z.q (the flexible-CG coefficients; q may be null).
 __global__ __launch_bounds__(kBlock) void k_mg0_post(const double* __restrict__ r, GridDims g,
-                                                     const float* __restrict__ s,
+                                                     TileGrid tg, const float* __restrict__ s,
                                                      const float* __restrict__ Ke,
                                                      const uint8_t* __restrict__ fixcnt,
                                                      const double* __restrict__ P,
@@ -388,15 +391,20 @@ __global__ __launch_bounds__(kBlock) void k_mg0_post(const double* __restrict__ 
                                                      double* partials, int stride,
                                                      const int* done) {
   if (*done) return;
-  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  __shared__ TileSmem<float> sm;
+  int i0, j0, k0;
+  const bool has = tile_origin(tg, i0, j0, k0);
+  if (has) tile_load(sm, g, i0, j0, k0, [&](int64_t qq) { return x[qq]; }, s);
+  __syncthreads();
+  int li, lj, lk;
+  tile_thread(li, lj, lk);
+  const int i = i0 + li, j = j0 + lj, k = k0 + lk;
   double acc[2] = {0.0, 0.0};
-  if (n < g.n_nodes) {
-    int i, j, k;
-    node_decode(n, g, i, j, k);
+  if (has && i <= g.nx && j <= g.ny && k <= g.nz) {
+    const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
     Nbr<float> nb;
-    gather_nbr(x, g, n, i, j, k, nb);
     float s8[8];
-    load_element_scales(s, g, i, j, k, s8);
+    tile_fetch(sm, li, lj, lk, nb, s8);
     float y[3];
     apply_node(nb, s8, Ke, y);
     const float pen = (float)(*P);
@@ -444,23 +452,29 @@ __global__ void k_fcg_start(FcgState* st, const double* rr, const double* bb, do
   *st = s;
 }
 
-// p = z + beta p_old; q = K p (FP64); partials p.q.
+// p = z + beta p_old (formed once per node in the LDS tile); q = K p (FP64); partials p.q.
 __global__ __launch_bounds__(kBlock) void k_fcg_apply(
     const double* __restrict__ z, const double* __restrict__ pold, double* __restrict__ pnew,
     double* __restrict__ q, const double* __restrict__ s, const double* __restrict__ Ke,
-    const uint8_t* __restrict__ fixcnt, const double* __restrict__ P, GridDims g,
+    const uint8_t* __restrict__ fixcnt, const double* __restrict__ P, GridDims g, TileGrid tg,
     const FcgState* st, double* partials) {
   if (st->done) return;
   const double beta = st->beta;
-  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  __shared__ TileSmem<double> sm;
+  int i0, j0, k0;
+  const bool has = tile_origin(tg, i0, j0, k0);
+  if (has)
+    tile_load(sm, g, i0, j0, k0, [&](int64_t qq) { return fma(beta, pold[qq], z[qq]); }, s);
+  __syncthreads();
+  int li, lj, lk;
+  tile_thread(li, lj, lk);
+  const int i = i0 + li, j = j0 + lj, k = k0 + lk;
   double acc[1] = {0.0};
-  if (n < g.n_nodes) {
-    int i, j, k;
-    node_decode(n, g, i, j, k);
+  if (has && i <= g.nx && j <= g.ny && k <= g.nz) {
+    const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
     Nbr<double> nb;
-    gather_dir(z, pold, beta, g, n, i, j, k, nb);
     double s8[8];
-    load_element_scales(s, g, i, j, k, s8);
+    tile_fetch(sm, li, lj, lk, nb, s8);
     double yv[3];
     apply_node(nb, s8, Ke, yv);
     const double pen = *P;

@@ -1,0 +1,116 @@
+// tile_op.hpp — LDS-tiled matrix-free hex8 operator for the fine voxel grid.
+//
+// A block of 256 threads owns a TI x TJ x TK = 4 x 8 x 8 tile of nodes (one node per thread,
+// k fastest as in the DOF numbering). It loads the tile plus a one-node halo (6 x 10 x 10
+// nodes, 3 components each) into LDS with contiguous, coalesced loads along k, applying the
+// caller's transform (e.g. p = z + beta*p_old) exactly once per node, and the 5 x 9 x 9
+// element scales around the tile. Every thread then reads its 27 neighbours from LDS at
+// compile-time offsets and applies Ke0 (uniform scalar loads). Compared with each thread
+// gathering its 27 neighbours from global memory, this removes the per-neighbour 64-bit
+// address arithmetic and bounds checks and the up-to-27-fold recomputation of transformed
+// values.
+//
+// Block -> tile mapping is XCD-aware: the hardware deals consecutive workgroups round-robin
+// to the 8 XCDs, so workgroup b is given tile (b % 8) * per_xcd + b / 8, which hands every
+// XCD a contiguous slab of tiles whose halos then hit in that XCD's L2.
+// Included from kernels.hpp after the operator primitives (GridDims, Nbr, apply_node).
+#pragma once
+
+namespace amt {
+
+constexpr int kTI = 4, kTJ = 8, kTK = 8;
+constexpr int kHI = kTI + 2, kHJ = kTJ + 2, kHK = kTK + 2;
+constexpr int kHalo = kHI * kHJ * kHK;               // 600 nodes
+constexpr int kEI = kTI + 1, kEJ = kTJ + 1, kEK = kTK + 1;
+constexpr int kETile = kEI * kEJ * kEK;              // 405 elements
+static_assert(kTI * kTJ * kTK == kBlock, "one node per thread");
+
+struct TileGrid {
+  int ti, tj, tk;   // tiles per axis
+  int n_tiles;
+  int per_xcd;      // ceil(n_tiles / 8)
+};
+
+inline TileGrid make_tile_grid(const GridDims& g) {
+  TileGrid t;
+  t.ti = (g.nx + 1 + kTI - 1) / kTI;
+  t.tj = (g.ny + 1 + kTJ - 1) / kTJ;
+  t.tk = (g.nz + 1 + kTK - 1) / kTK;
+  t.n_tiles = t.ti * t.tj * t.tk;
+  t.per_xcd = (t.n_tiles + 7) / 8;
+  return t;
+}
+// launch with 8 * per_xcd blocks; blocks mapped past n_tiles exit after the barriers
+
+template <typename T>
+struct TileSmem {
+  T x[kHalo * 3];
+  T s[kETile];
+};
+
+// Tile origin (node coordinates) of this block; returns false if the block has no tile.
+__device__ __forceinline__ bool tile_origin(const TileGrid& tg, int& i0, int& j0, int& k0) {
+  const int b = blockIdx.x;
+  const int t = (b & 7) * tg.per_xcd + (b >> 3);
+  if (t >= tg.n_tiles) return false;
+  const int tk = t % tg.tk, tj = (t / tg.tk) % tg.tj, ti = t / (tg.tk * tg.tj);
+  i0 = ti * kTI;
+  j0 = tj * kTJ;
+  k0 = tk * kTK;
+  return true;
+}
+
+// Cooperative load: halo node values f(dof) (0 outside the grid) and element scales.
+template <typename T, typename F>
+__device__ __forceinline__ void tile_load(TileSmem<T>& sm, const GridDims& g, int i0, int j0,
+                                          int k0, F f, const T* __restrict__ s) {
+  for (int q = threadIdx.x; q < kHalo * 3; q += kBlock) {
+    const int c = q % 3;
+    const int node = q / 3;
+    const int hk = node % kHK, hj = (node / kHK) % kHJ, hi = node / (kHK * kHJ);
+    const int i = i0 + hi - 1, j = j0 + hj - 1, k = k0 + hk - 1;
+    T v = T(0);
+    if (i >= 0 && i <= g.nx && j >= 0 && j <= g.ny && k >= 0 && k <= g.nz)
+      v = f(3 * (((int64_t)i * g.nny + j) * g.nnz + k) + c);
+    sm.x[q] = v;
+  }
+  for (int q = threadIdx.x; q < kETile; q += kBlock) {
+    const int ek = q % kEK, ej = (q / kEK) % kEJ, ei = q / (kEK * kEJ);
+    const int i = i0 + ei - 1, j = j0 + ej - 1, k = k0 + ek - 1;
+    T v = T(0);
+    if (i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz)
+      v = s[((int64_t)i * g.ny + j) * g.nz + k];
+    sm.s[q] = v;
+  }
+}
+
+// This thread's node inside the tile.
+__device__ __forceinline__ void tile_thread(int& li, int& lj, int& lk) {
+  lk = threadIdx.x % kTK;
+  lj = (threadIdx.x / kTK) % kTJ;
+  li = threadIdx.x / (kTK * kTJ);
+}
+
+// Neighbourhood and element scales of the thread's node from LDS.
+template <typename T>
+__device__ __forceinline__ void tile_fetch(const TileSmem<T>& sm, int li, int lj, int lk,
+                                           Nbr<T>& nb, T (&s8)[8]) {
+  const int base = ((li + 1) * kHJ + (lj + 1)) * kHK + (lk + 1);
+#pragma unroll
+  for (int di = -1; di <= 1; ++di)
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+      for (int dk = -1; dk <= 1; ++dk) {
+        const int m = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
+        const int o = 3 * (base + (di * kHJ + dj) * kHK + dk);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nb.v[m][c] = sm.x[o + c];
+      }
+  // element (i - ox[a], ...) sits at tile element index (li+1-ox, lj+1-oy, lk+1-oz)
+  const int eb = ((li + 1) * kEJ + (lj + 1)) * kEK + (lk + 1);
+#pragma unroll
+  for (int a = 0; a < 8; ++a) s8[a] = sm.s[eb - (kOX[a] * kEJ + kOY[a]) * kEK - kOZ[a]];
+}
+
+}  // namespace amt

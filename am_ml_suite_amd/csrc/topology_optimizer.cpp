@@ -82,9 +82,11 @@ TopologyOptimizer::TopologyOptimizer(int nx, int ny, int nz, Scalar element_size
   c.penalty_factor = config.penalty_factor;
   c.density_index = (int)config.density_index;
   c.precision = (int)config.precision;
+  c.precond = (int)config.preconditioner;
   c.warm_start = config.warm_start ? 1 : 0;
   c.inner_tol = config.inner_tolerance;
   c.verbose = 0;
+  c.use_graphs = 1;
   const int rc = amt_create(&c, &h_);
   if (rc != AMT_OK) {
     const std::string msg = amt_last_error(nullptr);

@@ -65,6 +65,7 @@ class SIMPConfig:
     warm_start: bool = False
     inner_tolerance: float = 1e-4
     inner_max_iterations: int = 20000
+    use_graphs: bool = True
 
 
 @dataclass
@@ -134,6 +135,7 @@ def make_amt_config(nx, ny, nz, element_size, material, config):
     c.inner_tol = config.inner_tolerance
     c.inner_maxit = int(config.inner_max_iterations)
     c.verbose = 0
+    c.use_graphs = int(bool(config.use_graphs))
     return c
 
 
@@ -216,6 +218,14 @@ class TopologyOptimizer:
         return _convert(r)
 
     runIteration = run_iteration
+
+    def step(self):
+        """One OC iteration on the device-resident density (no host copies of the field)."""
+        if not self._mask_built:
+            self.begin_run()
+        r = _ffi.AmtIterationResult()
+        self._check(self._L.amt_run_iteration(self._h, C.byref(r)))
+        return _convert(r)
 
     def run(self, initial_density, forces, fixed_dofs,
             callback: Optional[Callable] = None) -> OptimizationResult:

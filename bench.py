@@ -46,7 +46,7 @@ def run_path(problem, warmup, steps, precision, warm_start, profile, precond="ja
     opt.begin_run()
     res = []
     for it in range(warmup):
-        r = opt.run_iteration(rho, p.F, p.fixed)
+        r = opt.step()
         res.append(r)
         log("  [%s] warmup %d: c=%.9e pcg=%d outer=%d solve=%.3fs" % (
             precision, it, r.compliance, r.solver_iterations, r.outer_iterations, r.solve_time_s))
@@ -55,7 +55,7 @@ def run_path(problem, warmup, steps, precision, warm_start, profile, precond="ja
     opt.sync()
     t0 = time.perf_counter()
     for it in range(steps):
-        r = opt.run_iteration(rho, p.F, p.fixed)
+        r = opt.step()
         res.append(r)
         log("  [%s] step %d: c=%.9e pcg=%d outer=%d res=%.2e solve=%.3fs" % (
             precision, it, r.compliance, r.solver_iterations, r.outer_iterations,

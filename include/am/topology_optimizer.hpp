@@ -50,6 +50,9 @@ using Vector = std::vector<double>;
 
 enum class DensityIndex { Consistent = 0, Reference = 1 };
 enum class Precision { FP64 = 0, MixedFP32 = 1 };
+/// Jacobi: the reference's CG with Jacobi in place of IncompleteCholesky. Multigrid: FP64
+/// flexible CG around an FP32 Galerkin geometric-multigrid V-cycle.
+enum class Preconditioner { Jacobi = 0, Multigrid = 2 };
 
 /// topology_optimizer.hpp:49-59, plus the solver settings of topology_optimizer.cpp:35-40
 struct SIMPConfig {
@@ -69,6 +72,7 @@ struct SIMPConfig {
   // device options
   DensityIndex density_index = DensityIndex::Reference;
   Precision precision = Precision::FP64;
+  Preconditioner preconditioner = Preconditioner::Jacobi;
   bool warm_start = false;
   Scalar inner_tolerance = 1e-4;
 };

@@ -64,6 +64,7 @@ typedef struct amt_config {
   double inner_tol;      /* mixed precision: relative reduction per FP32 inner solve */
   int inner_maxit;
   int verbose;
+  int use_graphs;        /* 1: replay the multigrid CG iteration chunk as a HIP graph */
 } amt_config;
 
 /* IterationResult (topology_optimizer.hpp:65-74) plus solver detail. */

@@ -331,3 +331,21 @@ def test_multigrid_run_matches_oracle(shape):
     assert np.abs(res.final_density - rho_ref).max() <= 1e-4
     assert all(r.solver_converged and r.true_rel_residual <= 1.5e-8
                for r in res.iteration_results)
+
+
+def test_graph_replays_match_eager():
+    """The multigrid CG chunk replayed as a HIP graph gives bitwise the same iterates as the
+    eager launches, over several OC iterations (many replays, eager work in between)."""
+    p = A.bracket_like_problem(40, 20, 20)
+    runs = []
+    for graphs in (False, True, True):
+        cfg = make_config(p, preconditioner="multigrid", warm_start=True, max_iterations=4,
+                          use_graphs=graphs)
+        opt = A.TopologyOptimizer(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3), cfg)
+        res = opt.run(p.rho0, p.F, p.fixed)
+        runs.append((res.compliance_history, [r.solver_iterations for r in res.iteration_results],
+                     res.final_density.copy(), opt.get_displacement()))
+        opt.close()
+    for other in runs[1:]:
+        assert other[0] == runs[0][0] and other[1] == runs[0][1]
+        assert np.array_equal(other[2], runs[0][2]) and np.array_equal(other[3], runs[0][3])
