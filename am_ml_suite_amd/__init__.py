@@ -6,7 +6,7 @@ Python mirror of the reference optimizer API over that ABI (ctypes).
 """
 from .optimizer import (  # noqa: F401
     AmtError, CPPTopologyOptimizer, IterationResult, MaterialProperties, OptimizationResult,
-    SIMPConfig, TopologyOptimizer, device_name, element_stiffness, get_max_threads,
+    SIMPConfig, TopologyOptimizer, UNet, device_name, element_stiffness, get_max_threads,
     is_available)
 from .problems import (  # noqa: F401
     LoadCase, VoxelDomain, bracket_domain, bracket_load_case, cantilever_problem,

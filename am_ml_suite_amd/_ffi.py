@@ -22,6 +22,9 @@ EXPORTS = [
     "amt_run_iteration", "amt_run", "amt_apply_operator", "amt_get_diagonal", "amt_solve",
     "amt_sensitivities", "amt_filter", "amt_filter_neighbors", "amt_oc_update", "amt_sync",
     "amt_device_info", "amt_profile_enable", "amt_profile_read", "amt_element_stiffness",
+    "amt_unet_create", "amt_unet_create_for", "amt_unet_destroy", "amt_unet_num_params",
+    "amt_unet_get_params", "amt_unet_set_params", "amt_unet_forward", "amt_unet_backward",
+    "amt_unet_activation", "amt_unet_get_gain", "amt_unet_set_gain", "amt_attach_unet", "amt_unet_collect", "amt_unet_train",
 ]
 
 
@@ -46,7 +49,7 @@ class AmtIterationResult(C.Structure):
         ("max_density_change", C.c_double),
         ("solver_iterations", C.c_int), ("solver_converged", C.c_int),
         ("bisect_passes", C.c_int), ("outer_iterations", C.c_int),
-        ("true_rel_residual", C.c_double),
+        ("solver_fallback", C.c_int), ("true_rel_residual", C.c_double),
         ("assembly_time_s", C.c_double), ("solve_time_s", C.c_double),
         ("sensitivity_time_s", C.c_double), ("update_time_s", C.c_double),
     ]
@@ -55,7 +58,7 @@ class AmtIterationResult(C.Structure):
 class AmtSolveStats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int), ("converged", C.c_int), ("outer_iterations", C.c_int),
-        ("rel_residual", C.c_double), ("true_rel_residual", C.c_double),
+        ("fallback", C.c_int), ("rel_residual", C.c_double), ("true_rel_residual", C.c_double),
         ("solve_time_s", C.c_double),
     ]
 
@@ -107,8 +110,24 @@ def load():
     L.amt_profile_enable.argtypes = [P, C.c_int]
     L.amt_profile_read.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     L.amt_element_stiffness.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
+    _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.amt_unet_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(P)]
+    L.amt_unet_create_for.argtypes = [P, C.c_uint64, C.POINTER(P)]
+    L.amt_unet_destroy.argtypes = [P]
+    L.amt_unet_num_params.argtypes = [P]
+    L.amt_unet_get_params.argtypes = [P, _fp, C.c_int64]
+    L.amt_unet_set_params.argtypes = [P, _fp, C.c_int64]
+    L.amt_unet_forward.argtypes = [P, _fp, _fp]
+    L.amt_unet_backward.argtypes = [P, _fp, _fp]
+    L.amt_unet_activation.argtypes = [P, C.c_int, _fp, C.c_int64]
+    L.amt_attach_unet.argtypes = [P, P]
+    L.amt_unet_collect.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
+    L.amt_unet_train.argtypes = [P, P, C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+    L.amt_unet_get_gain.argtypes = [P, C.POINTER(C.c_double)]
+    L.amt_unet_set_gain.argtypes = [P, C.c_double]
     for name in EXPORTS:
         if name not in ("amt_config_default", "amt_last_error"):
             getattr(L, name).restype = C.c_int
+    L.amt_unet_num_params.restype = C.c_int64
     _lib = L
     return L

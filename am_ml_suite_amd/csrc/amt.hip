@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -21,6 +22,8 @@
 #include "../../include/amt.h"
 #include "kernels.hpp"
 #include "multigrid.hpp"
+#include "unet_glue.hpp"
+#include "unet_net.hpp"
 #include <cstddef>
 
 namespace amt {
@@ -39,6 +42,8 @@ constexpr int kOcCand = (1 << kOcLevels) - 1;
 constexpr int kMaxReduceBlocks = 1024;
 constexpr int kChunk = 32;  // PCG iterations launched between two host polls
 constexpr int kMgChunk = 4;  // flexible-CG iterations per captured graph (even)
+constexpr int kNnPatience = 16;  // learned FCG: chunks allowed without a 10% |r| drop
+constexpr double kNnGainMomentum = 0.05;  // running means behind the correction's weight
 constexpr float kMgOmega = 0.6f;  // damped-Jacobi weight of the V-cycle smoother
 constexpr int64_t kMgCoarseDofs = 300;  // coarsen until a level has at most this many DOFs
 
@@ -46,7 +51,8 @@ constexpr int64_t kMgCoarseDofs = 300;  // coarsen until a level has at most thi
 enum Region { R_APPLY = 0, R_UPD, R_INIT, R_OC, R_MISC, R_COUNT };
 
 // Device scalar slots
-enum Scal { S_MAXDIAG = 0, S_P, S_COMPL, S_NDESIGN, S_RR, S_BB, S_FLAG, S_COUNT };
+enum Scal { S_MAXDIAG = 0, S_P, S_COMPL, S_NDESIGN, S_RR, S_BB, S_FLAG, S_NNZZ, S_NNRDR,
+            S_NNUAU, S_NNUR, S_NND0, S_NNTH = S_NND0 + 5, S_COUNT };
 
 inline double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
@@ -108,6 +114,12 @@ struct amt_problem {
   bool fcg_graph_failed = false;
   int mg_last_iters = 0;  // iterations of the previous solve: sizes the first launch batch
 
+
+  // learned preconditioner (precond == AMT_PRECOND_UNET, or a learned warm start)
+  amt_unet* net = nullptr;  // attached, not owned
+  float* nn_feat = nullptr;
+  float *nn_U = nullptr, *nn_AU = nullptr, *nn_J = nullptr, *nn_KJ = nullptr;
+  std::vector<float*> bank;  // Jacobi-PCG residual snapshots (training inputs)
 
   bool loads_set = false, design_ready = false, coeffs_ready = false, have_u = false;
   double n_design = 0;
@@ -220,6 +232,9 @@ int prepare_coeffs(amt_problem* p) {
                      p->fixcnt, p->Fb, p->n_dof);
   AMT_CHECK(check_launch(p, "prepare_coeffs"));
   if (!p->mg.empty()) AMT_CHECK(mg_setup(p));
+  if (p->nn_feat)
+    hipLaunchKernelGGL(k_nn_feature, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, p->s64,
+                       p->g, p->nn_feat);
   p->coeffs_ready = true;
   return AMT_OK;
 }
@@ -299,7 +314,6 @@ int mg_build_hierarchy(amt_problem* p, const double* Ke) {
   mg_child_tables(Ke, G.data());
   AMT_CHECK(dalloc(p, &p->mg_G, G.size()));
   AMT_CHECK(copy_in(p, p->mg_G, G.data(), G.size() * sizeof(double)));
-  AMT_CHECK(dalloc(p, &p->fcg, 1));
   return AMT_OK;
 }
 
@@ -564,7 +578,8 @@ void prof_harvest(amt_problem* p) {
 // (apply, update) pairs with one device->host poll of the check flag per chunk.
 template <typename T, typename TK, typename TX>
 int pcg_run(amt_problem* p, const double* b, TX* x, T* r, T* z, T** pp, T* q, const T* s,
-            const TK* Ke, const T* dinv, double tol, int maxit, PcgState* out) {
+            const TK* Ke, const T* dinv, double tol, int maxit, PcgState* out,
+            const std::function<bool()>* on_chunk = nullptr) {
   const GridDims g = p->g;
   const int nbu = (int)nred(p->n_dof), nba = (int)p->nb_tile;
   hipLaunchKernelGGL(k_fill<T>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, pp[0], T(0),
@@ -597,6 +612,7 @@ int pcg_run(amt_problem* p, const double* b, TX* x, T* r, T* z, T** pp, T* q, co
     HIP_TRY(p, hipMemcpyAsync(out, p->pcg_flag, sizeof(PcgState), hipMemcpyDeviceToHost,
                               p->stream));
     HIP_TRY(p, hipStreamSynchronize(p->stream));
+    if (on_chunk && !out->done && !(*on_chunk)()) break;
   }
   prof_harvest(p);
   return AMT_OK;
@@ -651,6 +667,7 @@ int solve_fp64(amt_problem* p, amt_solve_stats* st) {
   st->iterations = hs.it;
   st->converged = hs.converged;
   st->outer_iterations = 1;
+  st->fallback = 0;
   st->rel_residual = hs.rhs2 > 0 ? std::sqrt(hs.rr / hs.rhs2) : 0.0;
   st->true_rel_residual = hs.rhs2 > 0 ? std::sqrt(rr_true / hs.rhs2) : 0.0;
   return AMT_OK;
@@ -698,12 +715,199 @@ int solve_mixed(amt_problem* p, amt_solve_stats* st) {
   return AMT_OK;
 }
 
+
+// ---- learned preconditioner: a BF16 U-Net correction on top of Jacobi ----
+
+void nn_dims(const amt_problem* p, int& px, int& py, int& pz) {
+  px = (p->g.nx + 1 + 7) / 8 * 8;
+  py = (p->g.ny + 1 + 7) / 8 * 8;
+  pz = (p->g.nz + 1 + 7) / 8 * 8;
+}
+
+int nn_check(amt_problem* p, amt_unet* net) {
+  if (!net) return fail(p, AMT_EINVAL, "no U-Net attached");
+  int px, py, pz;
+  nn_dims(p, px, py, pz);
+  if (net->d[0].px != px || net->d[0].py != py || net->d[0].pz != pz)
+    return fail(p, AMT_EINVAL, "U-Net grid does not match the problem (use amt_unet_create_for)");
+  return AMT_OK;
+}
+
+int nn_alloc(amt_problem* p) {
+  if (p->nn_feat) return AMT_OK;
+  AMT_CHECK(dalloc(p, &p->nn_feat, p->n_n));
+  AMT_CHECK(dalloc(p, &p->nn_U, p->n_dof));
+  AMT_CHECK(dalloc(p, &p->nn_AU, p->n_dof));
+  AMT_CHECK(dalloc(p, &p->nn_J, p->n_dof));
+  AMT_CHECK(dalloc(p, &p->nn_KJ, p->n_dof));
+  return AMT_OK;
+}
+
+// Network forward on residual r: X0 from the density feature and D^-1 r / rms.
+template <typename T>
+void nn_forward(amt_problem* p, amt_unet* net, const T* r) {
+  net->stream = p->stream;
+  const unsigned nbz = nred(p->n_dof);
+  hipLaunchKernelGGL(k_nn_zz<T>, dim3(nbz), dim3(kBlock), 0, p->stream, r, p->dinv, p->n_dof,
+                     p->part(R_MISC));
+  hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kBlock), 0, p->stream, p->part(R_MISC),
+                     (int)nbz, (int)nbz, p->scal + S_NNZZ);
+  const VoxDims& d = net->d[0];
+  hipLaunchKernelGGL(k_nn_pack<T>, dim3(nblk(d.nv * 16)), dim3(kBlock), 0, p->stream, r, p->dinv,
+                     p->nn_feat, p->g, d, p->scal + S_NNZZ, net->act[amt_unet::X0]);
+  amt_unet_impl::forward(net);
+}
+
+// z = M(r) = D^-1 r + gain * rms * net(feat, D^-1 r / rms); partials of z.r, z.q into R_INIT
+// (nred(n_dof) of them).
+template <typename T>
+void nn_apply(amt_problem* p, amt_unet* net, const T* r, double* z64, float* z32,
+              const double* q) {
+  nn_forward<T>(p, net, r);
+  hipLaunchKernelGGL(k_nn_unpack<T>, dim3(nred(p->n_dof)), dim3(kBlock), 0, p->stream,
+                     net->act[amt_unet::OUT], net->d[0], p->g, r, p->dinv, p->diag, p->fixcnt,
+                     p->scal + S_NNZZ, net->gain, z64, z32, q, p->part(R_INIT));
+}
+
+// Flexible CG (FP64) with the learned preconditioner. Stagnation (kNnPatience chunks without
+// a 10% drop of |r| below its last mark), a non-finite residual or a non-positive curvature
+// hands the solve to plain FP64 Jacobi-PCG from the current iterate; st->fallback reports it.
+int solve_unet(amt_problem* p, amt_solve_stats* st) {
+  amt_unet* net = p->net;
+  AMT_CHECK(nn_check(p, net));
+  const bool warm = p->cfg.warm_start && p->have_u;
+  dot_dev(p, p->Fb, p->Fb, p->scal + S_BB);
+  if (!warm) {
+    // learned warm start: u0 = gamma M(F_bc), gamma the energy-optimal multiple
+    nn_apply<double>(p, net, p->Fb, p->z, nullptr, nullptr);
+    hipLaunchKernelGGL((k_apply<double, double>), dim3(p->nb_tile), dim3(kBlock), 0, p->stream,
+                       p->z, p->q, p->s64, p->Ke64, p->fixcnt, (const double*)(p->scal + S_P),
+                       p->g, p->tg);
+    dot_dev(p, p->z, p->Fb, p->scal + S_NNUAU);
+    dot_dev(p, p->z, p->q, p->scal + S_NNUR);
+    hipLaunchKernelGGL(k_nn_scale, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->u, p->z,
+                       p->scal + S_NNUAU, p->n_dof);
+  }
+  AMT_CHECK(residual64(p, p->u, p->r, nullptr));
+  hipLaunchKernelGGL(k_fcg_start, dim3(1), dim3(64), 0, p->stream, p->fcg, p->scal + S_RR,
+                     p->scal + S_BB, p->cfg.pcg_tol, p->cfg.pcg_maxit);
+  hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->p[0],
+                     0.0, p->n_dof);
+  // k_nn_unpack runs nred(n_dof) blocks: its partials fit the R_INIT region
+  const int nba = (int)p->nb_tile, nbu = (int)nred(p->n_dof), nbz = nbu;
+  int* done_flag = (int*)(p->scal + S_FLAG);
+  nn_apply<double>(p, net, p->r, p->z, nullptr, nullptr);
+  hipLaunchKernelGGL(k_fcg_scalar, dim3(1), dim3(kBlock), 0, p->stream, p->fcg,
+                     (const double*)p->part(R_UPD), nbu, (const double*)p->part(R_APPLY), nba,
+                     (const double*)p->part(R_INIT), nbz, nbz, 1, done_flag);
+  AMT_CHECK(check_launch(p, "unet fcg init"));
+  FcgState hs;
+  AMT_CHECK(copy_out(p, &hs, p->fcg, sizeof(FcgState)));
+  // Stagnation test: |r| must drop 10% below its last mark within kNnPatience chunks.
+  double rr_mark = hs.rr;
+  int t = 0, fell_back = 0, chunk = 0, mark_chunk = 0;
+  while (!hs.done) {
+    for (int c = 0; c < kMgChunk; ++c, ++t) {
+      hipLaunchKernelGGL(k_fcg_apply, dim3(nba), dim3(kBlock), 0, p->stream, p->z, p->p[t & 1],
+                         p->p[(t + 1) & 1], p->q, p->s64, p->Ke64, p->fixcnt, p->scal + S_P,
+                         p->g, p->tg, p->fcg, p->part(R_APPLY));
+      hipLaunchKernelGGL(k_fcg_update, dim3(nbu), dim3(kBlock), 0, p->stream, p->u, p->r,
+                         p->p[(t + 1) & 1], p->q, p->n_dof, p->fcg,
+                         (const double*)p->part(R_APPLY), nba, p->part(R_UPD));
+      nn_apply<double>(p, net, p->r, p->z, nullptr, p->q);
+      hipLaunchKernelGGL(k_fcg_scalar, dim3(1), dim3(kBlock), 0, p->stream, p->fcg,
+                         (const double*)p->part(R_UPD), nbu, (const double*)p->part(R_APPLY),
+                         nba, (const double*)p->part(R_INIT), nbz, nbz, 0, done_flag);
+    }
+    AMT_CHECK(check_launch(p, "unet fcg chunk"));
+    AMT_CHECK(copy_out(p, &hs, p->fcg, sizeof(FcgState)));
+    if (hs.done) break;
+    ++chunk;
+    if (!std::isfinite(hs.rr) || !(hs.pq > 0.0)) {
+      fell_back = 1;
+      break;
+    }
+    if (hs.rr < 0.81 * rr_mark) {
+      rr_mark = hs.rr;
+      mark_chunk = chunk;
+    } else if (chunk - mark_chunk >= kNnPatience) {
+      fell_back = 1;
+      break;
+    }
+  }
+  if (fell_back) {
+    const int used = hs.it;
+    const int saved_warm = p->cfg.warm_start, saved_maxit = p->cfg.pcg_maxit;
+    p->cfg.warm_start = 1;
+    p->have_u = true;
+    p->cfg.pcg_maxit = std::max(1, saved_maxit - used);
+    const int rc = solve_fp64(p, st);
+    p->cfg.warm_start = saved_warm;
+    p->cfg.pcg_maxit = saved_maxit;
+    AMT_CHECK(rc);
+    st->iterations += used;
+    st->outer_iterations = 2;
+    st->fallback = 1;
+    return AMT_OK;
+  }
+  double rr_true = 0.0;
+  AMT_CHECK(residual64(p, p->u, p->q, &rr_true));
+  st->iterations = hs.it;
+  st->converged = hs.converged;
+  st->outer_iterations = 1;
+  st->fallback = 0;
+  st->rel_residual = hs.rhs2 > 0 ? std::sqrt(hs.rr / hs.rhs2) : 0.0;
+  st->true_rel_residual = hs.rhs2 > 0 ? std::sqrt(rr_true / hs.rhs2) : 0.0;
+  return AMT_OK;
+}
+
+// One training step on residual sample r (FP32): energy of U = J + theta N (J = D^-1 r, N the
+// network's correction, theta from the running means), two FP32 operator applications;
+// terms (optional) gets {U.KU, U.r, r.D^-1 r, J.KJ, J.r}. lr <= 0 only evaluates.
+void nn_train_step(amt_problem* p, amt_unet* net, const float* r, float lr, double* terms) {
+  const unsigned nbz = nred(p->n_dof);
+  hipLaunchKernelGGL(k_nn_rdr, dim3(nbz), dim3(kBlock), 0, p->stream, r, p->dinv, p->n_dof,
+                     p->part(R_UPD));
+  hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kBlock), 0, p->stream, p->part(R_UPD),
+                     (int)nbz, (int)nbz, p->scal + S_NNRDR);
+  nn_forward<float>(p, net, r);
+  const VoxDims& d = net->d[0];
+  hipLaunchKernelGGL(k_nn_fields, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream,
+                     net->act[amt_unet::OUT], d, p->g, r, p->dinv, p->diag, p->fixcnt,
+                     p->scal + S_NNZZ, p->nn_J, p->nn_U);
+  const double* P = p->scal + S_P;
+  hipLaunchKernelGGL((k_apply<float, float>), dim3(p->nb_tile), dim3(kBlock), 0, p->stream,
+                     p->nn_J, p->nn_KJ, p->s32, p->Ke32, p->fixcnt, P, p->g, p->tg);
+  hipLaunchKernelGGL((k_apply<float, float>), dim3(p->nb_tile), dim3(kBlock), 0, p->stream,
+                     p->nn_U, p->nn_AU, p->s32, p->Ke32, p->fixcnt, P, p->g, p->tg);
+  // 5 partials per block: R_OC (idle during training) holds kOcCand * stride_red doubles;
+  // the running means' ratio is the weight of N in this step's U and its gradient
+  hipLaunchKernelGGL(k_nn_dots, dim3(nbz), dim3(kBlock), 0, p->stream, r, p->nn_J, p->nn_KJ,
+                     p->nn_U, p->nn_AU, p->n_dof, p->part(R_OC), (int)nbz);
+  hipLaunchKernelGGL((k_finalize<5>), dim3(1), dim3(kBlock), 0, p->stream, p->part(R_OC),
+                     (int)nbz, (int)nbz, p->scal + S_NND0);
+  const bool eval_only = !(lr > 0.f);
+  hipLaunchKernelGGL(k_nn_theta, dim3(1), dim3(64), 0, p->stream, p->scal + S_NND0,
+                     p->scal + S_NNRDR, net->gain, eval_only ? 0.0 : kNnGainMomentum,
+                     p->scal + S_NNTH, terms);
+  if (eval_only) return;
+  hipLaunchKernelGGL(k_nn_loss_grad, dim3(nblk(d.nv * 16)), dim3(kBlock), 0, p->stream,
+                     p->nn_KJ, p->nn_AU, r, p->diag, p->fixcnt, p->g, d, p->scal + S_NNZZ,
+                     p->scal + S_NNTH, p->scal + S_NNRDR, net->grd[amt_unet::OUT]);
+  hipLaunchKernelGGL(k_fill<float>, dim3(nblk(net->n_params)), dim3(kBlock), 0, p->stream,
+                     net->grad, 0.f, net->n_params);
+  amt_unet_impl::backward(net);
+  amt_unet_impl::adam_step(net, lr);
+}
+
 int do_solve(amt_problem* p, amt_solve_stats* st) {
   if (!p->loads_set) return fail(p, AMT_EINVAL, "loads not set (amt_set_loads)");
   if (!p->coeffs_ready) AMT_CHECK(prepare_coeffs(p));
   int rc;
   if (p->cfg.precond == AMT_PRECOND_MULTIGRID)
     rc = solve_mg(p, st);
+  else if (p->cfg.precond == AMT_PRECOND_UNET)
+    rc = solve_unet(p, st);
   else
     rc = p->cfg.precision == AMT_PREC_MIXED_FP32 ? solve_mixed(p, st) : solve_fp64(p, st);
   if (rc == AMT_OK) p->have_u = true;
@@ -790,6 +994,7 @@ int run_iteration(amt_problem* p, amt_iteration_result* out) {
   R.solver_iterations = ss.iterations;
   R.solver_converged = ss.converged;
   R.outer_iterations = ss.outer_iterations;
+  R.solver_fallback = ss.fallback;
   R.true_rel_residual = ss.true_rel_residual;
   R.assembly_time_s = t1 - t0;
   R.solve_time_s = t2 - t1;
@@ -853,8 +1058,9 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
     return fail(nullptr, AMT_EINVAL, "invalid element size or material");
   if (!(cfg->filter_radius >= 0) || cfg->filter_radius > 64)
     return fail(nullptr, AMT_EINVAL, "filter radius must be in [0, 64]");
-  if (cfg->precond != AMT_PRECOND_JACOBI && cfg->precond != AMT_PRECOND_MULTIGRID)
-    return fail(nullptr, AMT_ENOTSUP, "preconditioner not available in this build");
+  if (cfg->precond != AMT_PRECOND_JACOBI && cfg->precond != AMT_PRECOND_MULTIGRID &&
+      cfg->precond != AMT_PRECOND_UNET)
+    return fail(nullptr, AMT_EINVAL, "unknown preconditioner");
   const int64_t n_n = (int64_t)(cfg->nx + 1) * (cfg->ny + 1) * (cfg->nz + 1);
   if (3 * n_n > (int64_t)INT32_MAX) return fail(nullptr, AMT_EINVAL, "grid too large");
   int ndev = 0;
@@ -934,6 +1140,7 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
   }
   CREATE_TRY(dalloc(p, &p->scal, S_COUNT));
   CREATE_TRY(dalloc(p, &p->pcg, 2));
+  CREATE_TRY(dalloc(p, &p->fcg, 1));  // flexible CG state (multigrid and learned preconditioners)
   CREATE_TRY(dalloc(p, &p->pcg_flag, 1));
   CREATE_TRY(dalloc(p, &p->oc, 1));
 
@@ -1172,6 +1379,204 @@ int amt_oc_update(amt_problem* p, const double* dcf, double* max_change, int* pa
   AMT_CHECK(oc_update(p, &hs));
   if (max_change) *max_change = hs.change;
   if (passes) *passes = hs.passes;
+  return AMT_OK;
+}
+
+int amt_unet_create(int px, int py, int pz, uint64_t seed, amt_unet** out) {
+  if (!out) return fail(nullptr, AMT_EINVAL, "null argument");
+  *out = nullptr;
+  const int rc = amt_unet_impl::net_create(px, py, pz, seed, out);
+  if (rc != AMT_OK)
+    return fail(nullptr, rc, rc == AMT_EINVAL ? "U-Net grid must be positive multiples of 8"
+                                              : "U-Net allocation failed");
+  return AMT_OK;
+}
+
+int amt_unet_create_for(amt_problem* p, uint64_t seed, amt_unet** out) {
+  if (!p || !out) return AMT_EINVAL;
+  int px, py, pz;
+  nn_dims(p, px, py, pz);
+  return amt_unet_create(px, py, pz, seed, out);
+}
+
+int amt_unet_destroy(amt_unet* net) {
+  amt_unet_impl::net_destroy(net);
+  return AMT_OK;
+}
+
+int64_t amt_unet_num_params(const amt_unet* net) { return net ? net->n_params : 0; }
+
+int amt_unet_get_params(amt_unet* net, float* params, int64_t n) {
+  if (!net || !params || n != net->n_params) return fail(nullptr, AMT_EINVAL, "bad buffer");
+  if (hipStreamSynchronize(net->stream) != hipSuccess ||
+      hipMemcpy(params, net->master, n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(nullptr, AMT_EHIP, "copy failed");
+  return AMT_OK;
+}
+
+int amt_unet_set_params(amt_unet* net, const float* params, int64_t n) {
+  if (!net || !params || n != net->n_params) return fail(nullptr, AMT_EINVAL, "bad buffer");
+  if (hipStreamSynchronize(net->stream) != hipSuccess ||
+      hipMemcpy(net->master, params, n * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(nullptr, AMT_EHIP, "copy failed");
+  amt_unet_impl::refresh_bf16(net);
+  net->adam_t = 0;
+  if (hipMemsetAsync(net->m, 0, n * 4, net->stream) != hipSuccess ||
+      hipMemsetAsync(net->v, 0, n * 4, net->stream) != hipSuccess ||
+      hipStreamSynchronize(net->stream) != hipSuccess)
+    return fail(nullptr, AMT_EHIP, "reset failed");
+  return AMT_OK;
+}
+
+static int net_io(amt_unet* net, int act, const float* in, float* out) {
+  const int64_t n = net->d[net->act_l[act]].nv * net->act_c[act];
+  float* tmp = nullptr;
+  if (hipMalloc(&tmp, n * 4) != hipSuccess) return fail(nullptr, AMT_EHIP, "hipMalloc failed");
+  int rc = AMT_OK;
+  if (in) {
+    if (hipMemcpyAsync(tmp, in, n * 4, hipMemcpyHostToDevice, net->stream) != hipSuccess)
+      rc = AMT_EHIP;
+    hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(n)), dim3(kBlock), 0, net->stream, tmp,
+                       act == amt_unet::OUT ? net->grd[act] : net->act[act], n);
+  } else {
+    hipLaunchKernelGGL(k_bf16_to_f32, dim3(nblk(n)), dim3(kBlock), 0, net->stream,
+                       net->act[act], tmp, n);
+    if (hipMemcpyAsync(out, tmp, n * 4, hipMemcpyDeviceToHost, net->stream) != hipSuccess)
+      rc = AMT_EHIP;
+  }
+  if (hipStreamSynchronize(net->stream) != hipSuccess) rc = AMT_EHIP;
+  (void)hipFree(tmp);
+  return rc == AMT_OK ? AMT_OK : fail(nullptr, rc, "U-Net transfer failed");
+}
+
+int amt_unet_forward(amt_unet* net, const float* x, float* y) {
+  if (!net || !x || !y) return fail(nullptr, AMT_EINVAL, "null argument");
+  net->stream = net->own_stream;
+  AMT_CHECK(net_io(net, amt_unet::X0, x, nullptr));
+  amt_unet_impl::forward(net);
+  return net_io(net, amt_unet::OUT, nullptr, y);
+}
+
+int amt_unet_activation(amt_unet* net, int tensor, float* out, int64_t n) {
+  if (!net || !out) return fail(nullptr, AMT_EINVAL, "null argument");
+  if (tensor < 0 || tensor >= amt_unet::NT_) return fail(nullptr, AMT_ERANGE, "tensor out of range");
+  if (n != net->d[net->act_l[tensor]].nv * net->act_c[tensor])
+    return fail(nullptr, AMT_EINVAL, "size does not match the tensor");
+  net->stream = net->own_stream;
+  return net_io(net, tensor, nullptr, out);
+}
+
+int amt_unet_backward(amt_unet* net, const float* dy, float* dparams) {
+  if (!net || !dy || !dparams) return fail(nullptr, AMT_EINVAL, "null argument");
+  net->stream = net->own_stream;
+  AMT_CHECK(net_io(net, amt_unet::OUT, dy, nullptr));
+  hipLaunchKernelGGL(k_fill<float>, dim3(nblk(net->n_params)), dim3(kBlock), 0, net->stream,
+                     net->grad, 0.f, net->n_params);
+  amt_unet_impl::backward(net);
+  if (hipStreamSynchronize(net->stream) != hipSuccess ||
+      hipMemcpy(dparams, net->grad, net->n_params * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(nullptr, AMT_EHIP, "backward failed");
+  return AMT_OK;
+}
+
+int amt_attach_unet(amt_problem* p, amt_unet* net) {
+  if (!p) return AMT_EINVAL;
+  if (net) AMT_CHECK(nn_check(p, net));
+  p->net = net;
+  if (net) {
+    AMT_CHECK(nn_alloc(p));
+    net->stream = p->stream;
+    p->coeffs_ready = false;
+  }
+  return AMT_OK;
+}
+
+int amt_unet_collect(amt_problem* p, int max_samples, int* n_total) {
+  if (!p || max_samples < 0) return AMT_EINVAL;
+  if (!p->loads_set) return fail(p, AMT_EINVAL, "loads not set (amt_set_loads)");
+  if (!p->coeffs_ready) AMT_CHECK(prepare_coeffs(p));
+  int taken = 0;
+  auto snap = [&](const double* r) -> int {
+    float* b = nullptr;
+    AMT_CHECK(dalloc(p, &b, p->n_dof));
+    hipLaunchKernelGGL(k_d2f, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, r, b, p->n_dof);
+    p->bank.push_back(b);
+    ++taken;
+    return AMT_OK;
+  };
+  if (max_samples > 0) {
+    // plain FP64 Jacobi-PCG from u = 0; the residual after every 32 passes is a sample
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->u,
+                       0.0, p->n_dof);
+    HIP_TRY(p, hipMemcpyAsync(p->r, p->Fb, sizeof(double) * p->n_dof, hipMemcpyDeviceToDevice,
+                              p->stream));
+    AMT_CHECK(snap(p->r));
+    int rc_snap = AMT_OK;
+    std::function<bool()> hook = [&]() {
+      if (taken >= max_samples) return false;
+      rc_snap = snap(p->r);
+      return rc_snap == AMT_OK && taken < max_samples;
+    };
+    PcgState hs;
+    AMT_CHECK((pcg_run<double, double, double>(p, p->Fb, p->u, p->r, p->z, p->p, p->q, p->s64,
+                                               p->Ke64, p->dinv, p->cfg.pcg_tol,
+                                               p->cfg.pcg_maxit, &hs, &hook)));
+    AMT_CHECK(rc_snap);
+    p->have_u = false;
+  }
+  HIP_TRY(p, hipStreamSynchronize(p->stream));
+  if (n_total) *n_total = (int)p->bank.size();
+  return AMT_OK;
+}
+
+int amt_unet_train(amt_problem* p, amt_unet* net, int steps, double lr, double* losses,
+                   double* jacobi_losses) {
+  if (!p || !net || steps < 0) return AMT_EINVAL;
+  AMT_CHECK(nn_check(p, net));
+  if (p->bank.empty()) return fail(p, AMT_EINVAL, "no training samples (amt_unet_collect)");
+  AMT_CHECK(nn_alloc(p));
+  amt_unet* saved = p->net;
+  p->net = net;
+  if (!p->coeffs_ready) AMT_CHECK(prepare_coeffs(p));
+  hipLaunchKernelGGL(k_nn_feature, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, p->s64, p->g,
+                     p->nn_feat);
+  const bool want = losses || jacobi_losses;
+  double* lt = nullptr;
+  if (want) AMT_CHECK(dalloc(p, &lt, 5 * (size_t)std::max(steps, 1)));
+  for (int s = 0; s < steps; ++s)
+    nn_train_step(p, net, p->bank[s % p->bank.size()], (float)lr, lt ? lt + 5 * s : nullptr);
+  AMT_CHECK(check_launch(p, "unet train"));
+  if (want) {
+    std::vector<double> h(5 * (size_t)steps);
+    AMT_CHECK(copy_out(p, h.data(), lt, sizeof(double) * h.size()));
+    for (int s = 0; s < steps; ++s) {
+      const double* t = &h[5 * (size_t)s];
+      if (losses) losses[s] = (t[0] - 2.0 * t[1]) / t[2];
+      if (jacobi_losses) jacobi_losses[s] = (t[3] - 2.0 * t[4]) / t[2];
+    }
+  }
+  HIP_TRY(p, hipStreamSynchronize(p->stream));
+  p->net = saved;
+  if (saved) saved->stream = p->stream;
+  return AMT_OK;
+}
+
+int amt_unet_get_gain(amt_unet* net, double* gain) {
+  if (!net || !gain) return fail(nullptr, AMT_EINVAL, "null argument");
+  if (hipStreamSynchronize(net->stream) != hipSuccess ||
+      hipMemcpy(gain, net->gain, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(nullptr, AMT_EHIP, "gain read failed");
+  return AMT_OK;
+}
+
+int amt_unet_set_gain(amt_unet* net, double gain) {
+  if (!net) return fail(nullptr, AMT_EINVAL, "null argument");
+  if (!std::isfinite(gain)) return fail(nullptr, AMT_EINVAL, "gain must be finite");
+  // restart the running means at this weight
+  const double g3[3] = {gain, gain, 1.0};
+  if (hipStreamSynchronize(net->stream) != hipSuccess ||
+      hipMemcpy(net->gain, g3, sizeof(g3), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(nullptr, AMT_EHIP, "gain write failed");
   return AMT_OK;
 }
 

@@ -1,0 +1,308 @@
+// unet_net.hpp — host side of the BF16 3D U-Net: parameters, activations, forward, backward,
+// Adam. Included by amt.hip (one translation unit for all device code).
+//
+// Architecture (4 levels, base 16 channels, 3x3x3 convolutions, ReLU):
+//   enc0 16->16->16 | pool | enc1 16->32->32 | pool | enc2 32->64->64 | pool |
+//   bottleneck 64->128->128 | up | dec2 (128+64)->64->64 | up | dec1 (64+32)->32->32 | up |
+//   dec0 (32+16)->16->16 | head 1x1x1 16->16 (linear; channels 0..2 are the correction)
+// Input: 16 channels (0: nodal density feature, 1..3: normalised Jacobi-scaled residual,
+// rest zero) on the node grid padded to multiples of 8.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "unet.hpp"
+
+struct amt_unet {
+  struct Layer {
+    int taps, c1, c2, cout, relu;
+    int64_t w_off, b_off, n_w;
+  };
+  amt::VoxDims d[4];
+  std::vector<Layer> L;
+  int64_t n_params = 0;
+  float *master = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
+  // {weight of the correction in the preconditioner, running numerator, running denominator}
+  double* gain = nullptr;
+  uint16_t *wb = nullptr, *wflip = nullptr;
+  int64_t max_w = 0;
+  // activations (BF16, channels-last) and their gradients
+  enum { X0, A0a, S0, P1, A1a, S1, P2, A2a, S2, P3, B3a, B3, U2, C2a, C2, U1, C1a, C1, U0, C0a,
+         C0, OUT, NT_ };
+  uint16_t* act[NT_] = {};
+  uint16_t* grd[NT_] = {};
+  int act_c[NT_] = {}, act_l[NT_] = {};
+  int adam_t = 0;
+  hipStream_t stream = nullptr;      // stream in use (the attached problem's, or own_stream)
+  hipStream_t own_stream = nullptr;
+  std::vector<void*> allocs;
+  std::string err;
+};
+
+namespace amt_unet_impl {
+
+using amt::VoxDims;
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+inline uint64_t splitmix(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline double normal(uint64_t& s) {
+  const double u1 = ((splitmix(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  const double u2 = ((splitmix(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+template <typename T>
+int nalloc(amt_unet* n, T** p, size_t count) {
+  void* v = nullptr;
+  if (hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+    n->err = "hipMalloc failed (U-Net buffers)";
+    return AMT_EHIP;
+  }
+  n->allocs.push_back(v);
+  *p = (T*)v;
+  return AMT_OK;
+}
+
+inline void net_destroy(amt_unet* n) {
+  if (!n) return;
+  if (n->stream) (void)hipStreamSynchronize(n->stream);
+  if (n->own_stream) (void)hipStreamSynchronize(n->own_stream);
+  for (void* a : n->allocs) (void)hipFree(a);
+  if (n->own_stream) (void)hipStreamDestroy(n->own_stream);
+  delete n;
+}
+
+// px, py, pz: padded grid (multiples of 8).
+inline int net_create(int px, int py, int pz, uint64_t seed, amt_unet** out) {
+  if (px <= 0 || py <= 0 || pz <= 0 || px % 8 || py % 8 || pz % 8) return AMT_EINVAL;
+  amt_unet* n = new amt_unet();
+  for (int l = 0; l < 4; ++l) {
+    n->d[l].px = px >> l;
+    n->d[l].py = py >> l;
+    n->d[l].pz = pz >> l;
+    n->d[l].nv = (int64_t)n->d[l].px * n->d[l].py * n->d[l].pz;
+  }
+  struct Spec {
+    int taps, c1, c2, cout, relu;
+  };
+  const Spec specs[15] = {{27, 16, 0, 16, 1},   {27, 16, 0, 16, 1},  {27, 16, 0, 32, 1},
+                          {27, 32, 0, 32, 1},   {27, 32, 0, 64, 1},  {27, 64, 0, 64, 1},
+                          {27, 64, 0, 128, 1},  {27, 128, 0, 128, 1}, {27, 128, 64, 64, 1},
+                          {27, 64, 0, 64, 1},   {27, 64, 32, 32, 1}, {27, 32, 0, 32, 1},
+                          {27, 32, 16, 16, 1},  {27, 16, 0, 16, 1},  {1, 16, 0, 16, 0}};
+  int64_t off = 0;
+  for (const auto& s : specs) {
+    amt_unet::Layer L;
+    L.taps = s.taps;
+    L.c1 = s.c1;
+    L.c2 = s.c2;
+    L.cout = s.cout;
+    L.relu = s.relu;
+    L.n_w = (int64_t)s.taps * s.cout * (s.c1 + s.c2);
+    L.w_off = off;
+    off += L.n_w;
+    L.b_off = off;
+    off += s.cout;
+    n->max_w = std::max(n->max_w, L.n_w);
+    n->L.push_back(L);
+  }
+  n->n_params = off;
+  const int chans[amt_unet::NT_] = {16, 16, 16, 16, 32, 32, 32, 64, 64, 64, 128,
+                                    128, 128, 64, 64, 64, 32, 32, 32, 16, 16, 16};
+  const int lev[amt_unet::NT_] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 2, 2, 2, 1, 1, 1, 0, 0, 0, 0};
+  auto bail = [&](int rc) {
+    net_destroy(n);
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&n->own_stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(AMT_EHIP);
+  n->stream = n->own_stream;
+  for (int t = 0; t < amt_unet::NT_; ++t) {
+    n->act_c[t] = chans[t];
+    n->act_l[t] = lev[t];
+    const size_t cnt = (size_t)n->d[lev[t]].nv * chans[t];
+    if (nalloc(n, &n->act[t], cnt) || nalloc(n, &n->grd[t], cnt)) return bail(AMT_EHIP);
+    if (hipMemset(n->act[t], 0, cnt * 2) != hipSuccess) return bail(AMT_EHIP);
+  }
+  if (nalloc(n, &n->master, n->n_params) || nalloc(n, &n->grad, n->n_params) ||
+      nalloc(n, &n->m, n->n_params) || nalloc(n, &n->v, n->n_params) ||
+      nalloc(n, &n->wb, n->n_params) || nalloc(n, &n->wflip, n->max_w) ||
+      nalloc(n, &n->gain, 3) || hipMemset(n->gain, 0, 3 * sizeof(double)) != hipSuccess)
+    return bail(AMT_EHIP);
+  // He initialisation (ReLU layers), unit-variance head; the gain starts at 0, so an untrained
+  // network leaves the preconditioner plain Jacobi
+  std::vector<float> h(n->n_params, 0.f);
+  uint64_t s = seed ? seed : 0x1234567ull;
+  for (size_t li = 0; li < n->L.size(); ++li) {
+    const auto& L = n->L[li];
+    const bool head = li + 1 == n->L.size();
+    const double sd = std::sqrt((head ? 1.0 : 2.0) / (L.taps * (L.c1 + L.c2)));
+    for (int64_t q = 0; q < L.n_w; ++q) h[L.w_off + q] = (float)(sd * normal(s));
+  }
+  if (hipMemcpy(n->master, h.data(), h.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(n->m, 0, h.size() * 4) != hipSuccess ||
+      hipMemset(n->v, 0, h.size() * 4) != hipSuccess)
+    return bail(AMT_EHIP);
+  hipLaunchKernelGGL(amt::k_f32_to_bf16, dim3(ceil_div(n->n_params, 256)), dim3(256), 0,
+                     n->stream, n->master, n->wb, n->n_params);
+  if (hipStreamSynchronize(n->stream) != hipSuccess) return bail(AMT_EHIP);
+  *out = n;
+  return AMT_OK;
+}
+
+inline void refresh_bf16(amt_unet* n) {
+  hipLaunchKernelGGL(amt::k_f32_to_bf16, dim3(ceil_div(n->n_params, 256)), dim3(256), 0,
+                     n->stream, n->master, n->wb, n->n_params);
+}
+
+// y = conv(x1 ++ x2) with the layer's weights; NT = output channels / 16.
+inline void conv_launch(hipStream_t st, int taps, int nt, const uint16_t* x1, int c1,
+                        const uint16_t* x2, int c2, const uint16_t* W, const float* bias,
+                        uint16_t* y1, uint16_t* y2, int ysplit, int relu, int accum,
+                        const VoxDims& d) {
+  const dim3 grid((unsigned)ceil_div(d.nv, 64)), blk(256);
+#define AMT_CONV_CASE(T, N)                                                                 \
+  if (taps == T && nt == N) {                                                               \
+    hipLaunchKernelGGL((amt::k_conv_mfma<T, N>), grid, blk, 0, st, x1, c1, x2, c2, W, bias, \
+                       y1, y2, ysplit, relu, accum, d);                                     \
+    return;                                                                                 \
+  }
+  AMT_CONV_CASE(27, 1) AMT_CONV_CASE(27, 2) AMT_CONV_CASE(27, 3) AMT_CONV_CASE(27, 4)
+  AMT_CONV_CASE(27, 6) AMT_CONV_CASE(27, 8) AMT_CONV_CASE(27, 12) AMT_CONV_CASE(1, 1)
+#undef AMT_CONV_CASE
+}
+
+inline void forward_layer(amt_unet* n, int li, int in1, int in2, int out) {
+  const auto& L = n->L[li];
+  const VoxDims& d = n->d[n->act_l[out]];
+  conv_launch(n->stream, L.taps, L.cout / 16, n->act[in1], L.c1, in2 >= 0 ? n->act[in2] : nullptr,
+              L.c2, n->wb + L.w_off, n->master + L.b_off, n->act[out], nullptr, L.cout, L.relu,
+              0, d);
+}
+
+inline void pool(amt_unet* n, int in, int out) {
+  const VoxDims& dc = n->d[n->act_l[out]];
+  const int C = n->act_c[in];
+  hipLaunchKernelGGL(amt::k_pool, dim3(ceil_div(dc.nv * C, 256)), dim3(256), 0, n->stream,
+                     n->act[in], n->d[n->act_l[in]], C, n->act[out], dc);
+}
+inline void upsample(amt_unet* n, int in, int out) {
+  const VoxDims& d = n->d[n->act_l[out]];
+  const int C = n->act_c[in];
+  hipLaunchKernelGGL(amt::k_upsample, dim3(ceil_div(d.nv * C, 256)), dim3(256), 0, n->stream,
+                     n->act[in], n->d[n->act_l[in]], C, n->act[out], d);
+}
+
+// Forward pass on act[X0] -> act[OUT] (stream-ordered, no sync).
+inline void forward(amt_unet* n) {
+  using U = amt_unet;
+  forward_layer(n, 0, U::X0, -1, U::A0a);
+  forward_layer(n, 1, U::A0a, -1, U::S0);
+  pool(n, U::S0, U::P1);
+  forward_layer(n, 2, U::P1, -1, U::A1a);
+  forward_layer(n, 3, U::A1a, -1, U::S1);
+  pool(n, U::S1, U::P2);
+  forward_layer(n, 4, U::P2, -1, U::A2a);
+  forward_layer(n, 5, U::A2a, -1, U::S2);
+  pool(n, U::S2, U::P3);
+  forward_layer(n, 6, U::P3, -1, U::B3a);
+  forward_layer(n, 7, U::B3a, -1, U::B3);
+  upsample(n, U::B3, U::U2);
+  forward_layer(n, 8, U::U2, U::S2, U::C2a);
+  forward_layer(n, 9, U::C2a, -1, U::C2);
+  upsample(n, U::C2, U::U1);
+  forward_layer(n, 10, U::U1, U::S1, U::C1a);
+  forward_layer(n, 11, U::C1a, -1, U::C1);
+  upsample(n, U::C1, U::U0);
+  forward_layer(n, 12, U::U0, U::S0, U::C0a);
+  forward_layer(n, 13, U::C0a, -1, U::C0);
+  forward_layer(n, 14, U::C0, -1, U::OUT);
+}
+
+// Backward of one conv layer. dy = grad of its output (post-ReLU; the ReLU mask is applied
+// here). Accumulates dW/db into n->grad; writes the input gradients (dx1, dx2 split) unless
+// dx1 < 0; accum_dx adds to existing input gradients.
+inline void backward_layer(amt_unet* n, int li, int in1, int in2, int out, int dx1, int dx2,
+                           int accum_dx) {
+  const auto& L = n->L[li];
+  const VoxDims& d = n->d[n->act_l[out]];
+  const int64_t cnt = d.nv * L.cout;
+  if (L.relu)
+    hipLaunchKernelGGL(amt::k_relu_bwd, dim3(ceil_div(cnt, 256)), dim3(256), 0, n->stream,
+                       n->grd[out], n->act[out], cnt);
+  const int64_t range = std::max<int64_t>(2048, ceil_div(d.nv, 32));
+  const dim3 wg((unsigned)ceil_div(d.nv, range), L.taps);
+  if (L.taps == 27)
+    hipLaunchKernelGGL(amt::k_conv_wgrad<27>, wg, dim3(256), 0, n->stream, n->act[in1], L.c1,
+                       in2 >= 0 ? n->act[in2] : nullptr, L.c2, n->grd[out], L.cout,
+                       n->grad + L.w_off, n->grad + L.b_off, d, range);
+  else
+    hipLaunchKernelGGL(amt::k_conv_wgrad<1>, wg, dim3(256), 0, n->stream, n->act[in1], L.c1,
+                       in2 >= 0 ? n->act[in2] : nullptr, L.c2, n->grd[out], L.cout,
+                       n->grad + L.w_off, n->grad + L.b_off, d, range);
+  if (dx1 < 0) return;
+  const int cin = L.c1 + L.c2;
+  hipLaunchKernelGGL(amt::k_w_flip, dim3(ceil_div(L.n_w, 256)), dim3(256), 0, n->stream,
+                     n->master + L.w_off, L.taps, L.cout, cin, n->wflip);
+  conv_launch(n->stream, L.taps, cin / 16, n->grd[out], L.cout, nullptr, 0, n->wflip, nullptr,
+              n->grd[dx1], dx2 >= 0 ? n->grd[dx2] : nullptr, L.c1, 0, accum_dx, d);
+}
+
+inline void upsample_bwd(amt_unet* n, int out, int in) {
+  const VoxDims& dc = n->d[n->act_l[in]];
+  const int C = n->act_c[in];
+  hipLaunchKernelGGL(amt::k_upsample_bwd, dim3(ceil_div(dc.nv * C, 256)), dim3(256), 0,
+                     n->stream, n->grd[out], n->d[n->act_l[out]], C, n->grd[in], dc);
+}
+inline void pool_bwd(amt_unet* n, int out, int in, int accum) {
+  const VoxDims& d = n->d[n->act_l[in]];
+  const int C = n->act_c[in];
+  hipLaunchKernelGGL(amt::k_pool_bwd, dim3(ceil_div(d.nv * C, 256)), dim3(256), 0, n->stream,
+                     n->grd[out], n->d[n->act_l[out]], C, n->grd[in], d, accum);
+}
+
+// Backward from grd[OUT] (after forward). Parameter gradients are accumulated into n->grad
+// (zero it first for a fresh gradient).
+inline void backward(amt_unet* n) {
+  using U = amt_unet;
+  backward_layer(n, 14, U::C0, -1, U::OUT, U::C0, -1, 0);
+  backward_layer(n, 13, U::C0a, -1, U::C0, U::C0a, -1, 0);
+  backward_layer(n, 12, U::U0, U::S0, U::C0a, U::U0, U::S0, 0);
+  upsample_bwd(n, U::U0, U::C1);
+  backward_layer(n, 11, U::C1a, -1, U::C1, U::C1a, -1, 0);
+  backward_layer(n, 10, U::U1, U::S1, U::C1a, U::U1, U::S1, 0);
+  upsample_bwd(n, U::U1, U::C2);
+  backward_layer(n, 9, U::C2a, -1, U::C2, U::C2a, -1, 0);
+  backward_layer(n, 8, U::U2, U::S2, U::C2a, U::U2, U::S2, 0);
+  upsample_bwd(n, U::U2, U::B3);
+  backward_layer(n, 7, U::B3a, -1, U::B3, U::B3a, -1, 0);
+  backward_layer(n, 6, U::P3, -1, U::B3a, U::P3, -1, 0);
+  pool_bwd(n, U::P3, U::S2, 1);
+  backward_layer(n, 5, U::A2a, -1, U::S2, U::A2a, -1, 0);
+  backward_layer(n, 4, U::P2, -1, U::A2a, U::P2, -1, 0);
+  pool_bwd(n, U::P2, U::S1, 1);
+  backward_layer(n, 3, U::A1a, -1, U::S1, U::A1a, -1, 0);
+  backward_layer(n, 2, U::P1, -1, U::A1a, U::P1, -1, 0);
+  pool_bwd(n, U::P1, U::S0, 1);
+  backward_layer(n, 1, U::A0a, -1, U::S0, U::A0a, -1, 0);
+  backward_layer(n, 0, U::X0, -1, U::A0a, -1, -1, 0);
+}
+
+inline void adam_step(amt_unet* n, float lr) {
+  ++n->adam_t;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)n->adam_t), c2 = 1.f - std::pow(b2, (float)n->adam_t);
+  hipLaunchKernelGGL(amt::k_adam, dim3(ceil_div(n->n_params, 256)), dim3(256), 0, n->stream,
+                     n->master, n->grad, n->m, n->v, n->n_params, lr, b1, b2, 1e-8f, c1, c2);
+  refresh_bf16(n);
+}
+
+}  // namespace amt_unet_impl

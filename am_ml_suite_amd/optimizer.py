@@ -81,6 +81,7 @@ class IterationResult:
     update_time_s: float = 0.0
     solver_converged: bool = False
     outer_iterations: int = 0
+    solver_fallback: bool = False
     true_rel_residual: float = 0.0
     bisect_passes: int = 0
 
@@ -104,7 +105,8 @@ def _convert(r):
         assembly_time_s=r.assembly_time_s, solve_time_s=r.solve_time_s,
         sensitivity_time_s=r.sensitivity_time_s, update_time_s=r.update_time_s,
         solver_converged=bool(r.solver_converged), outer_iterations=r.outer_iterations,
-        true_rel_residual=r.true_rel_residual, bisect_passes=r.bisect_passes)
+        solver_fallback=bool(r.solver_fallback), true_rel_residual=r.true_rel_residual,
+        bisect_passes=r.bisect_passes)
 
 
 def make_amt_config(nx, ny, nz, element_size, material, config):
@@ -280,7 +282,8 @@ class TopologyOptimizer:
         st = _ffi.AmtSolveStats()
         self._check(self._L.amt_solve(self._h, u, C.byref(st)))
         return u, dict(iterations=st.iterations, converged=bool(st.converged),
-                       outer_iterations=st.outer_iterations, rel_residual=st.rel_residual,
+                       outer_iterations=st.outer_iterations, fallback=bool(st.fallback),
+                       rel_residual=st.rel_residual,
                        true_rel_residual=st.true_rel_residual, solve_time_s=st.solve_time_s)
 
     def sensitivities(self, u):
@@ -313,6 +316,26 @@ class TopologyOptimizer:
     def sync(self):
         self._check(self._L.amt_sync(self._h))
 
+    # -- learned preconditioner --
+    def attach_unet(self, net):
+        self._check(self._L.amt_attach_unet(self._h, net._h if net is not None else None))
+        self._net = net  # keep it alive while attached
+
+    def collect_samples(self, max_samples):
+        n = C.c_int()
+        self._check(self._L.amt_unet_collect(self._h, int(max_samples), C.byref(n)))
+        return n.value
+
+    def train_unet(self, net, steps, lr=1e-3, with_jacobi=False):
+        """Adam steps on the collected samples; returns each step's energy loss (and, with
+        with_jacobi, the loss of the Jacobi term alone on the same sample)."""
+        losses = np.zeros(max(int(steps), 1))
+        jl = np.zeros_like(losses)
+        self._check(self._L.amt_unet_train(self._h, net._h, int(steps), float(lr),
+                                           losses.ctypes.data, jl.ctypes.data))
+        n = int(steps)
+        return (losses[:n], jl[:n]) if with_jacobi else losses[:n]
+
     def profile_enable(self, on=True):
         self._check(self._L.amt_profile_enable(self._h, int(bool(on))))
 
@@ -329,6 +352,88 @@ def element_stiffness(E=1.0, nu=0.3, h=1.0):
     if rc != _ffi.AMT_OK:
         _raise(rc, _ffi.load().amt_last_error(None).decode())
     return K.reshape(24, 24)
+
+
+class UNet:
+    """BF16 3D U-Net of the learned preconditioner (4 levels, 16/32/64/128 channels,
+    3x3x3 convolutions) on a padded voxel grid; tensors are [px*py*pz, 16] channels-last."""
+
+    TENSORS = ("X0", "A0a", "S0", "P1", "A1a", "S1", "P2", "A2a", "S2", "P3", "B3a", "B3", "U2",
+               "C2a", "C2", "U1", "C1a", "C1", "U0", "C0a", "C0", "OUT")
+    TENSOR_LEVEL_CHANNELS = {
+        "X0": (0, 16), "A0a": (0, 16), "S0": (0, 16), "P1": (1, 16), "A1a": (1, 32),
+        "S1": (1, 32), "P2": (2, 32), "A2a": (2, 64), "S2": (2, 64), "P3": (3, 64),
+        "B3a": (3, 128), "B3": (3, 128), "U2": (2, 128), "C2a": (2, 64), "C2": (2, 64),
+        "U1": (1, 64), "C1a": (1, 32), "C1": (1, 32), "U0": (0, 32), "C0a": (0, 16),
+        "C0": (0, 16), "OUT": (0, 16)}
+
+    def __init__(self, px=None, py=None, pz=None, seed=1, optimizer=None):
+        self._L = _ffi.load()
+        self._h = C.c_void_p()
+        if optimizer is not None:
+            rc = self._L.amt_unet_create_for(optimizer._h, int(seed), C.byref(self._h))
+            px, py, pz = (-(-(n + 1) // 8) * 8 for n in (optimizer.nx, optimizer.ny, optimizer.nz))
+        else:
+            rc = self._L.amt_unet_create(int(px), int(py), int(pz), int(seed), C.byref(self._h))
+        if rc != _ffi.AMT_OK:
+            _raise(rc, self._L.amt_last_error(None).decode())
+        self.n_params = int(self._L.amt_unet_num_params(self._h))
+        self.shape = (int(px), int(py), int(pz))
+
+    def _check(self, rc):
+        if rc != _ffi.AMT_OK:
+            _raise(rc, self._L.amt_last_error(None).decode())
+
+    def close(self):
+        if self._h:
+            self._L.amt_unet_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        out = np.zeros(self.n_params, dtype=np.float32)
+        self._check(self._L.amt_unet_get_params(self._h, out, out.size))
+        return out
+
+    def set_params(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self._L.amt_unet_set_params(self._h, params, params.size))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros_like(x)
+        self._check(self._L.amt_unet_forward(self._h, x, y))
+        return y
+
+    def backward(self, dy):
+        dy = np.ascontiguousarray(dy, dtype=np.float32)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        self._check(self._L.amt_unet_backward(self._h, dy, g))
+        return g
+
+    @property
+    def gain(self):
+        g = C.c_double()
+        self._check(self._L.amt_unet_get_gain(self._h, C.byref(g)))
+        return g.value
+
+    @gain.setter
+    def gain(self, value):
+        self._check(self._L.amt_unet_set_gain(self._h, float(value)))
+
+    def activation(self, name):
+        """Stored activation of the last forward, shape (px>>l, py>>l, pz>>l, channels)."""
+        i = self.TENSORS.index(name)
+        lev, ch = self.TENSOR_LEVEL_CHANNELS[name]
+        shape = tuple(n >> lev for n in self.shape) + (ch,)
+        out = np.zeros(shape, dtype=np.float32)
+        self._check(self._L.amt_unet_activation(self._h, i, out, out.size))
+        return out
 
 
 # notebook binding names (cell 23 lines 9-12)

@@ -74,6 +74,7 @@ typedef struct amt_iteration_result {
   int solver_converged;
   int bisect_passes;
   int outer_iterations;  /* FP64 refinement steps (1 for plain FP64) */
+  int solver_fallback;   /* 1: the learned preconditioner stalled, Jacobi-PCG finished */
   double true_rel_residual; /* |F_bc - K u| / |F_bc|, recomputed in FP64 */
   double assembly_time_s, solve_time_s, sensitivity_time_s, update_time_s;
 } amt_iteration_result;
@@ -127,7 +128,7 @@ int amt_apply_operator(amt_problem* p, const double* x, double* y, int with_bc);
 int amt_get_diagonal(amt_problem* p, double* diag, int with_bc, double* penalty);
 
 typedef struct amt_solve_stats {
-  int iterations, converged, outer_iterations;
+  int iterations, converged, outer_iterations, fallback;
   double rel_residual, true_rel_residual, solve_time_s;
 } amt_solve_stats;
 /* solveWithBCs (linear_solver.cpp:186-203) for the current density and loads; u (n_dof)
@@ -146,6 +147,47 @@ int amt_filter_neighbors(amt_problem* p, int64_t e, int64_t* idx, double* w, int
 /* updateDensitiesOC (topology_optimizer.cpp:205-287) on the device density with dcf (n_e)
  * and the current design mask. */
 int amt_oc_update(amt_problem* p, const double* dcf, double* max_change, int* passes);
+
+/* ---- learned preconditioner (BF16 3D U-Net; no counterpart in the reference, whose
+ *      src/am/ai/model.py:48-51 raises NotImplementedError) ----
+ * M(r) = D^-1 r + gain * rms * net(density feature, D^-1 r / rms) on free rows (D^-1 r on
+ * penalty rows) inside flexible CG, with plain Jacobi-PCG taking over when |r| stagnates
+ * (16 four-iteration chunks without a 10% drop) or the iteration breaks down. With warm_start
+ * and no previous displacement, u0 = gamma M(F_bc), gamma energy-optimal. A new network has
+ * gain 0 (plain Jacobi); training sets it. Networks are fully convolutional: parameters and
+ * gain move between grids with get/set_params and get/set_gain. */
+typedef struct amt_unet amt_unet;
+/* Network for a padded voxel grid (each dimension a positive multiple of 8). */
+int amt_unet_create(int px, int py, int pz, uint64_t seed, amt_unet** out);
+/* Network sized for p's node grid padded to multiples of 8. */
+int amt_unet_create_for(amt_problem* p, uint64_t seed, amt_unet** out);
+int amt_unet_destroy(amt_unet* net);
+int64_t amt_unet_num_params(const amt_unet* net);
+int amt_unet_get_params(amt_unet* net, float* params, int64_t n);
+int amt_unet_set_params(amt_unet* net, const float* params, int64_t n);
+/* Forward on host tensors: x, y are [px*py*pz][16] FP32 (stored BF16 on the device). */
+int amt_unet_forward(amt_unet* net, const float* x, float* y);
+/* Parameter gradient of sum(y * dy) for the last forward; dparams has num_params entries. */
+int amt_unet_backward(amt_unet* net, const float* dy, float* dparams);
+/* Copy a stored activation of the last forward to the host as FP32. tensor indexes
+ * X0 A0a S0 P1 A1a S1 P2 A2a S2 P3 B3a B3 U2 C2a C2 U1 C1a C1 U0 C0a C0 OUT (0..21), the
+ * network's tensors in forward order; n = voxels of its level * its channel count. */
+int amt_unet_activation(amt_unet* net, int tensor, float* out, int64_t n);
+/* Attach (or detach with NULL) the network used by AMT_PRECOND_UNET. */
+int amt_attach_unet(amt_problem* p, amt_unet* net);
+/* Appends training samples: residuals of a plain Jacobi-PCG solve of p's current problem
+ * (every 32 passes, up to max_samples). *n_total = samples held. */
+int amt_unet_collect(amt_problem* p, int max_samples, int* n_total);
+/* Adam steps on the held samples (cycled in order). Each step forms U = D^-1 r + theta N
+ * (N the network's correction, theta the energy-optimal weight, folded into the network's gain
+ * as a running mean) and descends the energy (U.KU - 2 U.r) / (r.D^-1 r); two FP32 operator
+ * applications per step. losses / jacobi_losses (optional, steps entries) get each step's
+ * energy and that of U = D^-1 r alone. lr = 0 evaluates: parameters and gain stay as they are. */
+int amt_unet_train(amt_problem* p, amt_unet* net, int steps, double lr, double* losses,
+                   double* jacobi_losses);
+/* Weight of the correction used by the preconditioner (0 for a new network). */
+int amt_unet_get_gain(amt_unet* net, double* gain);
+int amt_unet_set_gain(amt_unet* net, double gain);
 
 /* Wait for all work of this handle. */
 int amt_sync(amt_problem* p);
