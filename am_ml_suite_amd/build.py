@@ -1,17 +1,19 @@
 """Builds libamt.so (gfx950 HIP + host C++) in-tree with hipcc.
 
-    python -m am_ml_suite_amd.build
+    python -m am_ml_suite_amd.build                  # every source
+    python -m am_ml_suite_amd.build --incremental    # only sources with a changed dependency
 """
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libamt.so")
 
-SOURCES = ["amt.hip", "ke0.cpp", "topology_optimizer.cpp"]
+SOURCES = ["amt.hip", "unet.hip", "ke0.cpp", "topology_optimizer.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # Packed FP32 instructions are disabled for all device code.
@@ -32,21 +34,38 @@ def _run(cmd, verbose):
         print(out)
 
 
-def build(verbose=False):
-    objs = []
+def _up_to_date(obj, dep):
+    """True when obj is newer than every file its dependency list (from -MD) names."""
+    if not (os.path.exists(obj) and os.path.exists(dep)):
+        return False
+    text = open(dep).read().replace("\\\n", " ")
+    files = text.split(":", 1)[1].split() if ":" in text else []
+    t = os.path.getmtime(obj)
+    return all(os.path.exists(f) and os.path.getmtime(f) <= t for f in files)
+
+
+def _compile(src, bdir, verbose, force):
+    path = os.path.join(CSRC, src)
+    obj = os.path.join(bdir, src + ".o")
+    dep = obj + ".d"
+    if not force and _up_to_date(obj, dep):
+        return obj
+    cmd = [HIPCC] + COMMON
+    if src.endswith(".hip"):
+        cmd += ["-x", "hip", "--offload-arch=" + ARCH] + DEVICE_FLAGS
+    else:
+        cmd += ["-x", "c++"]
+    cmd += ["-MD", "-MF", dep, "-c", path, "-o", obj]
+    _run(cmd, verbose)
+    return obj
+
+
+def build(verbose=False, force=True):
+    """Compiles every source (force) or only those whose object is older than a dependency."""
     bdir = os.path.join(PKG, "_build")
     os.makedirs(bdir, exist_ok=True)
-    for src in SOURCES:
-        path = os.path.join(CSRC, src)
-        obj = os.path.join(bdir, src + ".o")
-        cmd = [HIPCC] + COMMON
-        if src.endswith(".hip"):
-            cmd += ["-x", "hip", "--offload-arch=" + ARCH] + DEVICE_FLAGS
-        else:
-            cmd += ["-x", "c++"]
-        cmd += ["-c", path, "-o", obj]
-        _run(cmd, verbose)
-        objs.append(obj)
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
+        objs = list(ex.map(lambda s: _compile(s, bdir, verbose, force), SOURCES))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB)
@@ -54,4 +73,4 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, force="--incremental" not in sys.argv))

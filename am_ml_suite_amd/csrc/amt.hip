@@ -22,19 +22,26 @@
 #include "../../include/amt.h"
 #include "kernels.hpp"
 #include "multigrid.hpp"
+#include "unet_api.hpp"
 #include "unet_glue.hpp"
-#include "unet_net.hpp"
 #include <cstddef>
 
 namespace amt {
 void hex8_stiffness(double E, double nu, double h, double* Ke);
 }
 
+namespace {
+thread_local std::string g_create_error;
+}
+
+int amt::set_global_error(int code, const std::string& msg) {
+  g_create_error = msg;
+  return code;
+}
+
 using namespace amt;
 
 namespace {
-
-thread_local std::string g_create_error;
 
 constexpr int kOcLevels = 6;  // bisection levels evaluated per OC sweep launch
 constexpr int kOcBlocks = 256;  // OC sweep grid (partials per candidate)
@@ -1382,101 +1389,11 @@ int amt_oc_update(amt_problem* p, const double* dcf, double* max_change, int* pa
   return AMT_OK;
 }
 
-int amt_unet_create(int px, int py, int pz, uint64_t seed, amt_unet** out) {
-  if (!out) return fail(nullptr, AMT_EINVAL, "null argument");
-  *out = nullptr;
-  const int rc = amt_unet_impl::net_create(px, py, pz, seed, out);
-  if (rc != AMT_OK)
-    return fail(nullptr, rc, rc == AMT_EINVAL ? "U-Net grid must be positive multiples of 8"
-                                              : "U-Net allocation failed");
-  return AMT_OK;
-}
-
 int amt_unet_create_for(amt_problem* p, uint64_t seed, amt_unet** out) {
   if (!p || !out) return AMT_EINVAL;
   int px, py, pz;
   nn_dims(p, px, py, pz);
   return amt_unet_create(px, py, pz, seed, out);
-}
-
-int amt_unet_destroy(amt_unet* net) {
-  amt_unet_impl::net_destroy(net);
-  return AMT_OK;
-}
-
-int64_t amt_unet_num_params(const amt_unet* net) { return net ? net->n_params : 0; }
-
-int amt_unet_get_params(amt_unet* net, float* params, int64_t n) {
-  if (!net || !params || n != net->n_params) return fail(nullptr, AMT_EINVAL, "bad buffer");
-  if (hipStreamSynchronize(net->stream) != hipSuccess ||
-      hipMemcpy(params, net->master, n * 4, hipMemcpyDeviceToHost) != hipSuccess)
-    return fail(nullptr, AMT_EHIP, "copy failed");
-  return AMT_OK;
-}
-
-int amt_unet_set_params(amt_unet* net, const float* params, int64_t n) {
-  if (!net || !params || n != net->n_params) return fail(nullptr, AMT_EINVAL, "bad buffer");
-  if (hipStreamSynchronize(net->stream) != hipSuccess ||
-      hipMemcpy(net->master, params, n * 4, hipMemcpyHostToDevice) != hipSuccess)
-    return fail(nullptr, AMT_EHIP, "copy failed");
-  amt_unet_impl::refresh_bf16(net);
-  net->adam_t = 0;
-  if (hipMemsetAsync(net->m, 0, n * 4, net->stream) != hipSuccess ||
-      hipMemsetAsync(net->v, 0, n * 4, net->stream) != hipSuccess ||
-      hipStreamSynchronize(net->stream) != hipSuccess)
-    return fail(nullptr, AMT_EHIP, "reset failed");
-  return AMT_OK;
-}
-
-static int net_io(amt_unet* net, int act, const float* in, float* out) {
-  const int64_t n = net->d[net->act_l[act]].nv * net->act_c[act];
-  float* tmp = nullptr;
-  if (hipMalloc(&tmp, n * 4) != hipSuccess) return fail(nullptr, AMT_EHIP, "hipMalloc failed");
-  int rc = AMT_OK;
-  if (in) {
-    if (hipMemcpyAsync(tmp, in, n * 4, hipMemcpyHostToDevice, net->stream) != hipSuccess)
-      rc = AMT_EHIP;
-    hipLaunchKernelGGL(k_f32_to_bf16, dim3(nblk(n)), dim3(kBlock), 0, net->stream, tmp,
-                       act == amt_unet::OUT ? net->grd[act] : net->act[act], n);
-  } else {
-    hipLaunchKernelGGL(k_bf16_to_f32, dim3(nblk(n)), dim3(kBlock), 0, net->stream,
-                       net->act[act], tmp, n);
-    if (hipMemcpyAsync(out, tmp, n * 4, hipMemcpyDeviceToHost, net->stream) != hipSuccess)
-      rc = AMT_EHIP;
-  }
-  if (hipStreamSynchronize(net->stream) != hipSuccess) rc = AMT_EHIP;
-  (void)hipFree(tmp);
-  return rc == AMT_OK ? AMT_OK : fail(nullptr, rc, "U-Net transfer failed");
-}
-
-int amt_unet_forward(amt_unet* net, const float* x, float* y) {
-  if (!net || !x || !y) return fail(nullptr, AMT_EINVAL, "null argument");
-  net->stream = net->own_stream;
-  AMT_CHECK(net_io(net, amt_unet::X0, x, nullptr));
-  amt_unet_impl::forward(net);
-  return net_io(net, amt_unet::OUT, nullptr, y);
-}
-
-int amt_unet_activation(amt_unet* net, int tensor, float* out, int64_t n) {
-  if (!net || !out) return fail(nullptr, AMT_EINVAL, "null argument");
-  if (tensor < 0 || tensor >= amt_unet::NT_) return fail(nullptr, AMT_ERANGE, "tensor out of range");
-  if (n != net->d[net->act_l[tensor]].nv * net->act_c[tensor])
-    return fail(nullptr, AMT_EINVAL, "size does not match the tensor");
-  net->stream = net->own_stream;
-  return net_io(net, tensor, nullptr, out);
-}
-
-int amt_unet_backward(amt_unet* net, const float* dy, float* dparams) {
-  if (!net || !dy || !dparams) return fail(nullptr, AMT_EINVAL, "null argument");
-  net->stream = net->own_stream;
-  AMT_CHECK(net_io(net, amt_unet::OUT, dy, nullptr));
-  hipLaunchKernelGGL(k_fill<float>, dim3(nblk(net->n_params)), dim3(kBlock), 0, net->stream,
-                     net->grad, 0.f, net->n_params);
-  amt_unet_impl::backward(net);
-  if (hipStreamSynchronize(net->stream) != hipSuccess ||
-      hipMemcpy(dparams, net->grad, net->n_params * 4, hipMemcpyDeviceToHost) != hipSuccess)
-    return fail(nullptr, AMT_EHIP, "backward failed");
-  return AMT_OK;
 }
 
 int amt_attach_unet(amt_problem* p, amt_unet* net) {
@@ -1558,25 +1475,6 @@ int amt_unet_train(amt_problem* p, amt_unet* net, int steps, double lr, double* 
   HIP_TRY(p, hipStreamSynchronize(p->stream));
   p->net = saved;
   if (saved) saved->stream = p->stream;
-  return AMT_OK;
-}
-
-int amt_unet_get_gain(amt_unet* net, double* gain) {
-  if (!net || !gain) return fail(nullptr, AMT_EINVAL, "null argument");
-  if (hipStreamSynchronize(net->stream) != hipSuccess ||
-      hipMemcpy(gain, net->gain, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
-    return fail(nullptr, AMT_EHIP, "gain read failed");
-  return AMT_OK;
-}
-
-int amt_unet_set_gain(amt_unet* net, double gain) {
-  if (!net) return fail(nullptr, AMT_EINVAL, "null argument");
-  if (!std::isfinite(gain)) return fail(nullptr, AMT_EINVAL, "gain must be finite");
-  // restart the running means at this weight
-  const double g3[3] = {gain, gain, 1.0};
-  if (hipStreamSynchronize(net->stream) != hipSuccess ||
-      hipMemcpy(net->gain, g3, sizeof(g3), hipMemcpyHostToDevice) != hipSuccess)
-    return fail(nullptr, AMT_EHIP, "gain write failed");
   return AMT_OK;
 }
 

@@ -14,32 +14,12 @@
 //   D (16x16, M x N): lane l holds D[4*(l/16) + e][l % 16]
 #pragma once
 
-#include <hip/hip_runtime.h>
-#include <stdint.h>
+#include "unet_api.hpp"
 
 namespace amt {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
-// round to nearest even (finite inputs)
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
-
-struct VoxDims {
-  int px, py, pz;
-  int64_t nv;
-};
-
-__device__ __forceinline__ void vox_decode(int64_t v, const VoxDims& d, int& i, int& j, int& k) {
-  k = (int)(v % d.pz);
-  const int64_t t = v / d.pz;
-  j = (int)(t % d.py);
-  i = (int)(t / d.py);
-}
 
 // ---------------------------------------------------------------------------------------
 // Convolution (TAPS = 27: 3x3x3 with zero padding; TAPS = 1: pointwise) by MFMA.

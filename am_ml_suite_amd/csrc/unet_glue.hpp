@@ -13,7 +13,7 @@
 #pragma once
 
 #include "kernels.hpp"
-#include "unet.hpp"
+#include "unet_api.hpp"
 
 namespace amt {
 

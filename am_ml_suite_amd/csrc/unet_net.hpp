@@ -15,31 +15,6 @@
 
 #include "unet.hpp"
 
-struct amt_unet {
-  struct Layer {
-    int taps, c1, c2, cout, relu;
-    int64_t w_off, b_off, n_w;
-  };
-  amt::VoxDims d[4];
-  std::vector<Layer> L;
-  int64_t n_params = 0;
-  float *master = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
-  // {weight of the correction in the preconditioner, running numerator, running denominator}
-  double* gain = nullptr;
-  uint16_t *wb = nullptr, *wflip = nullptr;
-  int64_t max_w = 0;
-  // activations (BF16, channels-last) and their gradients
-  enum { X0, A0a, S0, P1, A1a, S1, P2, A2a, S2, P3, B3a, B3, U2, C2a, C2, U1, C1a, C1, U0, C0a,
-         C0, OUT, NT_ };
-  uint16_t* act[NT_] = {};
-  uint16_t* grd[NT_] = {};
-  int act_c[NT_] = {}, act_l[NT_] = {};
-  int adam_t = 0;
-  hipStream_t stream = nullptr;      // stream in use (the attached problem's, or own_stream)
-  hipStream_t own_stream = nullptr;
-  std::vector<void*> allocs;
-  std::string err;
-};
 
 namespace amt_unet_impl {
 
@@ -202,7 +177,7 @@ inline void upsample(amt_unet* n, int in, int out) {
 }
 
 // Forward pass on act[X0] -> act[OUT] (stream-ordered, no sync).
-inline void forward(amt_unet* n) {
+void forward(amt_unet* n) {
   using U = amt_unet;
   forward_layer(n, 0, U::X0, -1, U::A0a);
   forward_layer(n, 1, U::A0a, -1, U::S0);
@@ -271,7 +246,7 @@ inline void pool_bwd(amt_unet* n, int out, int in, int accum) {
 
 // Backward from grd[OUT] (after forward). Parameter gradients are accumulated into n->grad
 // (zero it first for a fresh gradient).
-inline void backward(amt_unet* n) {
+void backward(amt_unet* n) {
   using U = amt_unet;
   backward_layer(n, 14, U::C0, -1, U::OUT, U::C0, -1, 0);
   backward_layer(n, 13, U::C0a, -1, U::C0, U::C0a, -1, 0);
@@ -296,7 +271,7 @@ inline void backward(amt_unet* n) {
   backward_layer(n, 0, U::X0, -1, U::A0a, -1, -1, 0);
 }
 
-inline void adam_step(amt_unet* n, float lr) {
+void adam_step(amt_unet* n, float lr) {
   ++n->adam_t;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.f - std::pow(b1, (float)n->adam_t), c2 = 1.f - std::pow(b2, (float)n->adam_t);
