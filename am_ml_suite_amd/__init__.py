@@ -5,9 +5,9 @@ as hand-written gfx950 HIP kernels behind the C-ABI in include/amt.h. This packa
 Python mirror of the reference optimizer API over that ABI (ctypes).
 """
 from .optimizer import (  # noqa: F401
-    AmtError, CPPTopologyOptimizer, IterationResult, MaterialProperties, OptimizationResult,
+    AmtError, BatchOptimizer, CPPTopologyOptimizer, IterationResult, MaterialProperties, OptimizationResult,
     SIMPConfig, TopologyOptimizer, UNet, device_name, element_stiffness, get_max_threads,
     is_available)
 from .problems import (  # noqa: F401
     LoadCase, VoxelDomain, bracket_domain, bracket_load_case, cantilever_problem,
-    bracket_like_problem)
+    bracket_like_problem, batch_problems)

@@ -24,7 +24,10 @@ EXPORTS = [
     "amt_device_info", "amt_profile_enable", "amt_profile_read", "amt_element_stiffness",
     "amt_unet_create", "amt_unet_create_for", "amt_unet_destroy", "amt_unet_num_params",
     "amt_unet_get_params", "amt_unet_set_params", "amt_unet_forward", "amt_unet_backward",
-    "amt_unet_activation", "amt_unet_get_gain", "amt_unet_set_gain", "amt_attach_unet", "amt_unet_collect", "amt_unet_train",
+    "amt_unet_activation", "amt_unet_get_gain", "amt_unet_set_gain", "amt_attach_unet",
+    "amt_batch_create", "amt_batch_destroy", "amt_batch_size", "amt_batch_set_loads",
+    "amt_batch_set_density", "amt_batch_get_density", "amt_batch_get_displacement",
+    "amt_batch_run", "amt_unet_collect", "amt_unet_train",
 ]
 
 
@@ -124,6 +127,14 @@ def load():
     L.amt_unet_collect.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
     L.amt_unet_train.argtypes = [P, P, C.c_int, C.c_double, C.c_void_p, C.c_void_p]
     L.amt_unet_get_gain.argtypes = [P, C.POINTER(C.c_double)]
+    L.amt_batch_create.argtypes = [C.c_void_p, C.c_int, C.POINTER(P)]
+    L.amt_batch_destroy.argtypes = [P]
+    L.amt_batch_size.argtypes = [P, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.amt_batch_set_loads.argtypes = [P, C.c_int, _dp, C.c_int64, _i64p, C.c_int64]
+    L.amt_batch_set_density.argtypes = [P, C.c_int, _dp, C.c_int64]
+    L.amt_batch_get_density.argtypes = [P, C.c_int, _dp, C.c_int64]
+    L.amt_batch_get_displacement.argtypes = [P, C.c_int, _dp, C.c_int64]
+    L.amt_batch_run.argtypes = [P, C.c_int, C.c_void_p, C.POINTER(C.c_double)]
     L.amt_unet_set_gain.argtypes = [P, C.c_double]
     for name in EXPORTS:
         if name not in ("amt_config_default", "amt_last_error"):

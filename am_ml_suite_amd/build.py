@@ -13,7 +13,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libamt.so")
 
-SOURCES = ["amt.hip", "unet.hip", "ke0.cpp", "topology_optimizer.cpp"]
+SOURCES = ["amt.hip", "unet.hip", "batch.hip", "ke0.cpp", "topology_optimizer.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # Packed FP32 instructions are disabled for all device code.

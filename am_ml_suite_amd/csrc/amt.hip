@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/amt.h"
+#include "host_util.hpp"
 #include "kernels.hpp"
 #include "multigrid.hpp"
 #include "unet_api.hpp"
@@ -61,10 +62,6 @@ enum Region { R_APPLY = 0, R_UPD, R_INIT, R_OC, R_MISC, R_COUNT };
 enum Scal { S_MAXDIAG = 0, S_P, S_COMPL, S_NDESIGN, S_RR, S_BB, S_FLAG, S_NNZZ, S_NNRDR,
             S_NNUAU, S_NNUR, S_NND0, S_NNTH = S_NND0 + 5, S_COUNT };
 
-inline double now_s() {
-  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
-      .count();
-}
 
 }  // namespace
 
@@ -187,21 +184,6 @@ int check_launch(amt_problem* p, const char* what) {
   return AMT_OK;
 }
 
-// Filter stencil (topology_optimizer.cpp:60-102): offsets in dz, dy, dx order with the
-// reference's distance and weight arithmetic; the per-element list is this table with
-// out-of-grid offsets skipped.
-void build_stencil(double r, std::vector<StencilEntry>& st, std::vector<double>& w) {
-  const int R = (int)std::ceil(r);
-  for (int dz = -R; dz <= R; ++dz)
-    for (int dy = -R; dy <= R; ++dy)
-      for (int dx = -R; dx <= R; ++dx) {
-        const double dist = std::sqrt((double)(dx * dx + dy * dy + dz * dz));
-        if (dist <= r) {
-          st.push_back({dx, dy, dz, 0});
-          w.push_back(std::max(0.0, r - dist));
-        }
-      }
-}
 
 int copy_in(amt_problem* p, void* dst, const void* src, size_t bytes) {
   HIP_TRY(p, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, p->stream));

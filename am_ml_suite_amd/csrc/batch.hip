@@ -1,0 +1,564 @@
+// batch.hip — the batched SIMP engine (dataset generation of many uneven problems) and its
+// C-ABI (include/amt.h, amt_batch_*). No counterpart in the reference, which optimises one
+// problem per TopologyOptimizer (topology_optimizer.cpp:391-520); each problem here follows
+// those semantics (consistent density index, Jacobi-PCG with penalty BCs, filter, OC).
+//
+// All problems advance in lockstep, one OC iteration at a time. Every phase is one persistent
+// launch over the items of all problems (batch.hpp). The pre-solve phase (SIMP scales,
+// diagonal, penalty, Jacobi, F_bc), each PCG (re)start, a chunk of kBChunk PCG iterations and
+// the post-solve phase (compliance, sensitivities, filter, all OC bisection sweeps and the
+// density update) are captured once as hipGraphs and replayed; the host only polls the
+// per-problem solver flags between chunk replays. Problems that converge drop out of the
+// chunk at once (their items exit at the top). PCG runs its matvec in T (FP32 in the mixed
+// mode) with FP64 u, r and reductions; when every problem has converged on its recursive
+// residual, the FP64 residual F_bc - K u is recomputed and problems above tolerance restart
+// from it (FP64 residual replacement), so results meet the tolerance on the true residual.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/amt.h"
+#include "batch.hpp"
+#include "host_util.hpp"
+
+namespace amt {
+void hex8_stiffness(double E, double nu, double h, double* Ke);
+int set_global_error(int code, const std::string& msg);
+}  // namespace amt
+
+using namespace amt;
+
+namespace {
+
+constexpr int kBChunk = 16;        // PCG iterations per chunk graph (even: parity returns)
+constexpr int kBMaxCycles = 8;     // FP64 residual restarts per solve
+constexpr int kBPersist = 2048;    // blocks of a persistent launch (8 per CU)
+constexpr int kBOcLevels = 6;
+constexpr int kBOcCand = (1 << kBOcLevels) - 1;
+
+int bfail(int code, const std::string& msg) { return amt::set_global_error(code, msg); }
+
+#define B_HIP(call)                                                                    \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return bfail(AMT_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+#define B_CHECK(call)              \
+  do {                             \
+    int rc_ = (call);              \
+    if (rc_ != AMT_OK) return rc_; \
+  } while (0)
+
+}  // namespace
+
+struct amt_batch {
+  int np = 0;
+  bool mixed = true, warm = true;
+  std::vector<amt_config> cfg;
+  std::vector<BProb> hp;
+  BProb* D = nullptr;
+  int n_items[BK_KINDS] = {0, 0, 0, 0};
+  int* ip[BK_KINDS] = {nullptr, nullptr, nullptr, nullptr};
+  int64_t n_e = 0, n_n = 0, n_dof = 0;
+  // packed problem data
+  double *rho = nullptr, *s64 = nullptr, *dc = nullptr, *dcf = nullptr;
+  float* s32 = nullptr;
+  uint8_t *design = nullptr, *fixcnt = nullptr;
+  double *u = nullptr, *r = nullptr, *F = nullptr, *Fb = nullptr, *dinv = nullptr,
+         *diag = nullptr;
+  void *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr;  // T
+  double* Ke64 = nullptr;
+  float* Ke32 = nullptr;
+  StencilEntry* st = nullptr;
+  double* sw = nullptr;
+  // per problem
+  BState *S = nullptr, *flags = nullptr;
+  OcState* oc = nullptr;
+  double *maxdiag = nullptr, *bb = nullptr, *pen = nullptr, *ndesign = nullptr,
+         *compl_ = nullptr;
+  // partials
+  double *part_tile = nullptr, *part_dof = nullptr, *part_elem = nullptr, *part_node = nullptr;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t g_pre = nullptr, g_start0 = nullptr, g_start1 = nullptr, g_chunk = nullptr,
+                 g_post = nullptr;
+  std::vector<uint8_t> loads_set;
+  bool design_ready = false;
+  std::vector<double> ndesign_h;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+template <typename T>
+int balloc(amt_batch* b, T** p, size_t n) {
+  void* v = nullptr;
+  if (hipMalloc(&v, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
+    return bfail(AMT_EHIP, "hipMalloc failed (batch buffers)");
+  b->allocs.push_back(v);
+  *p = (T*)v;
+  return AMT_OK;
+}
+
+unsigned grid_of(int n_items) {
+  const int g = std::min(((n_items + 7) / 8) * 8, kBPersist);
+  return (unsigned)std::max(g, 8);
+}
+
+void destroy(amt_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (hipGraphExec_t g : {b->g_pre, b->g_start0, b->g_start1, b->g_chunk, b->g_post})
+    if (g) (void)hipGraphExecDestroy(g);
+  for (void* a : b->allocs) (void)hipFree(a);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+// ---- phases (enqueued on b->stream; captured into graphs) ----
+
+template <typename T>
+void enqueue_pre(amt_batch* b) {
+  const int ne = b->n_items[BK_ELEM], nn = b->n_items[BK_NODE], nd = b->n_items[BK_DOF];
+  hipLaunchKernelGGL(k_b_simp, dim3(grid_of(ne)), dim3(kBlock), 0, b->stream, b->D,
+                     b->ip[BK_ELEM], ne, b->rho, b->s64, b->s32);
+  hipLaunchKernelGGL(k_b_diag, dim3(grid_of(nn)), dim3(kBlock), 0, b->stream, b->D,
+                     b->ip[BK_NODE], nn, b->s64, b->Ke64, b->diag, b->part_node);
+  hipLaunchKernelGGL((k_b_reduce<1, true>), dim3(b->np), dim3(kBlock), 0, b->stream, b->D,
+                     (int)BK_NODE, b->part_node, nn, b->maxdiag, 1);
+  hipLaunchKernelGGL(k_b_dinv, dim3(grid_of(nd)), dim3(kBlock), 0, b->stream, b->D,
+                     b->ip[BK_DOF], nd, b->maxdiag, b->diag, b->fixcnt, b->F, b->pen, b->dinv,
+                     b->Fb, b->part_dof);
+  hipLaunchKernelGGL((k_b_reduce<1>), dim3(b->np), dim3(kBlock), 0, b->stream, b->D,
+                     (int)BK_DOF, b->part_dof, nd, b->bb, 1);
+  if (!b->warm)
+    hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)((b->n_dof + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, b->stream, b->u, 0.0, b->n_dof);
+}
+
+// FP64 residual from u, PCG state per problem, p_old = 0.
+template <typename T>
+void enqueue_start(amt_batch* b, int first) {
+  const int nt = b->n_items[BK_TILE];
+  hipLaunchKernelGGL((k_b_residual<T>), dim3(grid_of(nt)), dim3(kBlock), 0, b->stream, b->D,
+                     b->ip[BK_TILE], nt, b->u, b->Fb, b->r, (T*)b->z, b->dinv, b->s64, b->Ke64,
+                     b->fixcnt, b->pen, b->part_tile);
+  hipLaunchKernelGGL(k_b_start, dim3(b->np), dim3(kBlock), 0, b->stream, b->D, b->part_tile,
+                     nt, b->bb, first, b->S, b->flags);
+  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((b->n_dof + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, b->stream, (T*)b->p0, T(0), b->n_dof);
+}
+
+template <typename T>
+void enqueue_chunk(amt_batch* b) {
+  const int nt = b->n_items[BK_TILE], nd = b->n_items[BK_DOF];
+  const T* s = std::is_same<T, float>::value ? (const T*)b->s32 : (const T*)b->s64;
+  const T* Ke = std::is_same<T, float>::value ? (const T*)b->Ke32 : (const T*)b->Ke64;
+  T* pp[2] = {(T*)b->p0, (T*)b->p1};
+  for (int t = 0; t < kBChunk; ++t) {
+    const int par = t & 1;
+    hipLaunchKernelGGL((k_b_apply<T>), dim3(grid_of(nt)), dim3(kBlock), 0, b->stream, par, b->D,
+                       b->ip[BK_TILE], nt, (const T*)b->z, pp[par], pp[par ^ 1], (T*)b->q, s, Ke,
+                       b->fixcnt, b->pen, b->S, b->part_dof, nd, b->part_tile);
+    hipLaunchKernelGGL((k_b_update<T>), dim3(grid_of(nd)), dim3(kBlock), 0, b->stream, par ^ 1,
+                       b->D, b->ip[BK_DOF], nd, b->u, b->r, (T*)b->z, pp[par ^ 1],
+                       (const T*)b->q, b->dinv, b->S, b->part_tile, nt, b->part_dof);
+  }
+  hipLaunchKernelGGL(k_b_check, dim3(1), dim3(kBlock), 0, b->stream, 0, b->D, b->np, b->S,
+                     b->part_dof, nd, b->flags);
+}
+
+void enqueue_post(amt_batch* b) {
+  const int ne = b->n_items[BK_ELEM], nd = b->n_items[BK_DOF];
+  // compliance with the unmodified forces (topology_optimizer.cpp:338)
+  hipLaunchKernelGGL(k_b_dot, dim3(grid_of(nd)), dim3(kBlock), 0, b->stream, b->D,
+                     b->ip[BK_DOF], nd, b->u, b->F, b->part_dof);
+  hipLaunchKernelGGL((k_b_reduce<1>), dim3(b->np), dim3(kBlock), 0, b->stream, b->D,
+                     (int)BK_DOF, b->part_dof, nd, b->compl_, 1);
+  hipLaunchKernelGGL(k_b_sens, dim3(grid_of(ne)), dim3(kBlock), 0, b->stream, b->D,
+                     b->ip[BK_ELEM], ne, b->u, b->rho, b->Ke64, b->dc);
+  hipLaunchKernelGGL(k_b_filter, dim3(grid_of(ne)), dim3(kBlock), 0, b->stream, b->D,
+                     b->ip[BK_ELEM], ne, b->dc, b->rho, b->st, b->sw, b->dcf);
+  hipLaunchKernelGGL(k_b_oc_init, dim3((b->np + 63) / 64), dim3(64), 0, b->stream, b->D, b->np,
+                     b->ndesign, b->oc);
+  // ceil(100 / LEV) + 1 sweeps always suffice: each advances LEV passes or finishes
+  for (int sw = 0; sw < (100 + kBOcLevels - 1) / kBOcLevels + 1; ++sw) {
+    hipLaunchKernelGGL((k_b_oc_sweep<kBOcLevels>), dim3(grid_of(ne)), dim3(kBlock), 0,
+                       b->stream, b->D, b->ip[BK_ELEM], ne, b->rho, b->dcf, b->design, b->oc,
+                       b->part_elem);
+    hipLaunchKernelGGL((k_b_oc_walk<kBOcLevels>), dim3(b->np), dim3(kBlock), 0, b->stream, b->D,
+                       b->oc, b->part_elem, ne);
+  }
+  hipLaunchKernelGGL(k_b_oc_final, dim3(grid_of(ne)), dim3(kBlock), 0, b->stream, b->D,
+                     b->ip[BK_ELEM], ne, b->rho, b->dcf, b->design, b->oc, b->part_elem);
+  hipLaunchKernelGGL(k_b_oc_final_reduce, dim3(b->np), dim3(kBlock), 0, b->stream, b->D, b->oc,
+                     b->part_elem, ne);
+}
+
+template <typename F>
+int capture(amt_batch* b, hipGraphExec_t* out, F enqueue) {
+  if (*out) return AMT_OK;
+  hipGraph_t g = nullptr;
+  B_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  enqueue();
+  const hipError_t e1 = hipStreamEndCapture(b->stream, &g);
+  if (e1 != hipSuccess) return bfail(AMT_EHIP, std::string("graph capture: ") + hipGetErrorString(e1));
+  const hipError_t e2 = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e2 != hipSuccess) {
+    *out = nullptr;
+    return bfail(AMT_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e2));
+  }
+  return AMT_OK;
+}
+
+int build_design(amt_batch* b) {
+  const int ne = b->n_items[BK_ELEM];
+  hipLaunchKernelGGL(k_b_design, dim3(grid_of(ne)), dim3(kBlock), 0, b->stream, b->D,
+                     b->ip[BK_ELEM], ne, b->rho, b->design, b->part_elem);
+  hipLaunchKernelGGL((k_b_reduce<1>), dim3(b->np), dim3(kBlock), 0, b->stream, b->D,
+                     (int)BK_ELEM, b->part_elem, ne, b->ndesign, 1);
+  b->ndesign_h.assign(b->np, 0.0);
+  B_HIP(hipGetLastError());
+  B_HIP(hipMemcpyAsync(b->ndesign_h.data(), b->ndesign, sizeof(double) * b->np,
+                       hipMemcpyDeviceToHost, b->stream));
+  B_HIP(hipStreamSynchronize(b->stream));
+  b->design_ready = true;
+  return AMT_OK;
+}
+
+int read_flags(amt_batch* b, std::vector<BState>& f, int* active) {
+  B_HIP(hipMemcpyAsync(f.data(), b->flags, sizeof(BState) * b->np, hipMemcpyDeviceToHost,
+                       b->stream));
+  B_HIP(hipStreamSynchronize(b->stream));
+  int a = 0;
+  for (const BState& s : f) a += !s.done;
+  *active = a;
+  return AMT_OK;
+}
+
+template <typename T>
+int run_impl(amt_batch* b, int iterations, amt_iteration_result* hist, double* elapsed) {
+  for (int i = 0; i < b->np; ++i)
+    if (!b->loads_set[i])
+      return bfail(AMT_EINVAL, "loads not set for problem " + std::to_string(i));
+  if (!b->design_ready) B_CHECK(build_design(b));
+  B_CHECK(capture(b, &b->g_pre, [&] { enqueue_pre<T>(b); }));
+  B_CHECK(capture(b, &b->g_start0, [&] { enqueue_start<T>(b, 1); }));
+  B_CHECK(capture(b, &b->g_start1, [&] { enqueue_start<T>(b, 0); }));
+  B_CHECK(capture(b, &b->g_chunk, [&] { enqueue_chunk<T>(b); }));
+  B_CHECK(capture(b, &b->g_post, [&] { enqueue_post(b); }));
+  std::vector<BState> f(b->np);
+  std::vector<OcState> oc(b->np);
+  std::vector<double> compl_(b->np);
+  const double t_begin = now_s();
+  for (int it = 0; it < iterations; ++it) {
+    const double t0 = now_s();
+    B_HIP(hipGraphLaunch(b->g_pre, b->stream));
+    B_HIP(hipStreamSynchronize(b->stream));
+    const double t1 = now_s();
+    int cycles = 0, active = 0;
+    for (; cycles <= kBMaxCycles; ++cycles) {
+      B_HIP(hipGraphLaunch(cycles == 0 ? b->g_start0 : b->g_start1, b->stream));
+      B_CHECK(read_flags(b, f, &active));
+      if (active == 0 || cycles == kBMaxCycles) break;
+      while (active > 0) {
+        B_HIP(hipGraphLaunch(b->g_chunk, b->stream));
+        B_CHECK(read_flags(b, f, &active));
+      }
+    }
+    const double t2 = now_s();
+    B_HIP(hipGraphLaunch(b->g_post, b->stream));
+    B_HIP(hipMemcpyAsync(oc.data(), b->oc, sizeof(OcState) * b->np, hipMemcpyDeviceToHost,
+                         b->stream));
+    B_HIP(hipMemcpyAsync(compl_.data(), b->compl_, sizeof(double) * b->np,
+                         hipMemcpyDeviceToHost, b->stream));
+    B_HIP(hipStreamSynchronize(b->stream));
+    const double t3 = now_s();
+    if (hist) {
+      for (int i = 0; i < b->np; ++i) {
+        amt_iteration_result& R = hist[(size_t)it * b->np + i];
+        std::memset(&R, 0, sizeof(R));
+        R.compliance = compl_[i];
+        const double nd = b->ndesign_h[i] > 0 ? b->ndesign_h[i] : (double)b->hp[i].g.n_elem;
+        R.volume_fraction = oc[i].vol_sum / nd;
+        R.max_density_change = oc[i].change;
+        R.bisect_passes = oc[i].passes;
+        R.solver_iterations = f[i].it;
+        R.solver_converged = f[i].converged;
+        R.outer_iterations = cycles + 1;
+        R.true_rel_residual = f[i].rhs2 > 0 ? std::sqrt(f[i].rr / f[i].rhs2) : 0.0;
+        R.assembly_time_s = t1 - t0;
+        R.solve_time_s = t2 - t1;
+        R.update_time_s = t3 - t2;
+      }
+    }
+  }
+  if (elapsed) *elapsed = now_s() - t_begin;
+  return AMT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amt_batch_create(const amt_config* cfgs, int n, amt_batch** out) {
+  if (!cfgs || !out || n <= 0) return bfail(AMT_EINVAL, "need at least one problem");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    const amt_config& c = cfgs[i];
+    if (c.nx <= 0 || c.ny <= 0 || c.nz <= 0)
+      return bfail(AMT_EINVAL, "problem " + std::to_string(i) + ": grid dimensions must be positive");
+    if (!(c.element_size > 0) || !(c.E > 0) || !(c.nu > -1.0 && c.nu < 0.5))
+      return bfail(AMT_EINVAL, "problem " + std::to_string(i) + ": invalid element size or material");
+    if (!(c.filter_radius >= 0) || c.filter_radius > 64)
+      return bfail(AMT_EINVAL, "problem " + std::to_string(i) + ": filter radius must be in [0, 64]");
+    if (c.density_index != AMT_INDEX_CONSISTENT || c.precond != AMT_PRECOND_JACOBI)
+      return bfail(AMT_ENOTSUP, "batch: consistent density index and Jacobi-PCG only");
+    if (c.precision != cfgs[0].precision || c.warm_start != cfgs[0].warm_start)
+      return bfail(AMT_EINVAL, "batch: precision and warm_start must agree across problems");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return bfail(AMT_EHIP, "no HIP device available");
+  amt_batch* b = new amt_batch();
+  b->np = n;
+  b->mixed = cfgs[0].precision == AMT_PREC_MIXED_FP32;
+  b->warm = cfgs[0].warm_start != 0;
+  b->cfg.assign(cfgs, cfgs + n);
+  b->loads_set.assign(n, 0);
+  auto bail = [&](int rc) {
+    destroy(b);
+    return rc;
+  };
+  // descriptors, Ke0 tables (one per distinct E, nu, h) and stencils
+  std::vector<double> ke64;
+  std::vector<float> ke32;
+  std::vector<double> ke_key;
+  std::vector<StencilEntry> st_all;
+  std::vector<double> sw_all;
+  std::vector<int> ipv[BK_KINDS];
+  for (int i = 0; i < n; ++i) {
+    const amt_config& c = cfgs[i];
+    BProb P;
+    std::memset(&P, 0, sizeof(P));
+    P.g.nx = c.nx;
+    P.g.ny = c.ny;
+    P.g.nz = c.nz;
+    P.g.nny = c.ny + 1;
+    P.g.nnz = c.nz + 1;
+    P.g.n_nodes = (int64_t)(c.nx + 1) * (c.ny + 1) * (c.nz + 1);
+    P.g.n_elem = (int64_t)c.nx * c.ny * c.nz;
+    P.tg = make_tile_grid(P.g);
+    P.e_off = b->n_e;
+    P.n_off = b->n_n;
+    b->n_e += P.g.n_elem;
+    b->n_n += P.g.n_nodes;
+    const int64_t cnt[BK_KINDS] = {P.tg.n_tiles, (P.g.n_elem + kBElemItem - 1) / kBElemItem,
+                                   (3 * P.g.n_nodes + kBDofItem - 1) / kBDofItem,
+                                   (P.g.n_nodes + kBlock - 1) / kBlock};
+    for (int k = 0; k < BK_KINDS; ++k) {
+      P.item0[k] = b->n_items[k];
+      P.nitem[k] = (int)cnt[k];
+      if ((int64_t)b->n_items[k] + cnt[k] > (int64_t)INT32_MAX / 64) {
+        delete b;
+        return bfail(AMT_EINVAL, "batch too large");
+      }
+      b->n_items[k] += (int)cnt[k];
+      ipv[k].insert(ipv[k].end(), (size_t)cnt[k], i);
+    }
+    int ke = -1;
+    for (size_t t = 0; t < ke_key.size() / 3; ++t)
+      if (ke_key[3 * t] == c.E && ke_key[3 * t + 1] == c.nu && ke_key[3 * t + 2] == c.element_size)
+        ke = (int)t;
+    if (ke < 0) {
+      ke = (int)(ke_key.size() / 3);
+      ke_key.insert(ke_key.end(), {c.E, c.nu, c.element_size});
+      double K[576];
+      hex8_stiffness(c.E, c.nu, c.element_size, K);
+      for (int q = 0; q < 576; ++q) {
+        ke64.push_back(K[q]);
+        ke32.push_back((float)K[q]);
+      }
+    }
+    P.ke = ke;
+    std::vector<StencilEntry> st;
+    std::vector<double> sw;
+    build_stencil(c.filter_radius, st, sw);
+    P.st_off = (int)st_all.size();
+    P.n_st = (int)st.size();
+    st_all.insert(st_all.end(), st.begin(), st.end());
+    sw_all.insert(sw_all.end(), sw.begin(), sw.end());
+    P.maxit = c.pcg_maxit;
+    P.penal = c.penalty;
+    P.Emin = c.E_min;
+    P.rho_min = c.rho_min;
+    P.move = c.move_limit;
+    P.vf = c.volume_fraction;
+    P.pf = c.penalty_factor;
+    P.tol = c.pcg_tol;
+    b->hp.push_back(P);
+  }
+  b->n_dof = 3 * b->n_n;
+  if (b->n_dof > (int64_t)INT32_MAX * 4) {
+    delete b;
+    return bfail(AMT_EINVAL, "batch too large");
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return bfail(AMT_EHIP, "stream creation failed");
+  }
+  const size_t tsz = b->mixed ? sizeof(float) : sizeof(double);
+  const int64_t nd = b->n_dof, ne = b->n_e;
+  int rc = AMT_OK;
+  auto A = [&](int r) {
+    if (rc == AMT_OK) rc = r;
+  };
+  A(balloc(b, &b->D, n));
+  for (int k = 0; k < BK_KINDS; ++k) A(balloc(b, &b->ip[k], ipv[k].size()));
+  A(balloc(b, &b->rho, ne));
+  A(balloc(b, &b->s64, ne));
+  A(balloc(b, &b->s32, ne));
+  A(balloc(b, &b->dc, ne));
+  A(balloc(b, &b->dcf, ne));
+  A(balloc(b, &b->design, ne));
+  A(balloc(b, &b->fixcnt, nd));
+  A(balloc(b, &b->u, nd));
+  A(balloc(b, &b->r, nd));
+  A(balloc(b, &b->F, nd));
+  A(balloc(b, &b->Fb, nd));
+  A(balloc(b, &b->dinv, nd));
+  A(balloc(b, &b->diag, nd));
+  uint8_t *zz = nullptr, *pa = nullptr, *pb = nullptr, *qq = nullptr;
+  A(balloc(b, &zz, nd * tsz));
+  A(balloc(b, &pa, nd * tsz));
+  A(balloc(b, &pb, nd * tsz));
+  A(balloc(b, &qq, nd * tsz));
+  b->z = zz;
+  b->p0 = pa;
+  b->p1 = pb;
+  b->q = qq;
+  A(balloc(b, &b->Ke64, ke64.size()));
+  A(balloc(b, &b->Ke32, ke32.size()));
+  A(balloc(b, &b->st, st_all.size()));
+  A(balloc(b, &b->sw, sw_all.size()));
+  A(balloc(b, &b->S, 2 * (size_t)n));
+  A(balloc(b, &b->flags, n));
+  A(balloc(b, &b->oc, n));
+  A(balloc(b, &b->maxdiag, n));
+  A(balloc(b, &b->bb, n));
+  A(balloc(b, &b->pen, n));
+  A(balloc(b, &b->ndesign, n));
+  A(balloc(b, &b->compl_, n));
+  A(balloc(b, &b->part_tile, 2 * (size_t)b->n_items[BK_TILE]));
+  A(balloc(b, &b->part_dof, 2 * (size_t)b->n_items[BK_DOF]));
+  A(balloc(b, &b->part_elem, (size_t)kBOcCand * b->n_items[BK_ELEM]));
+  A(balloc(b, &b->part_node, b->n_items[BK_NODE]));
+  if (rc != AMT_OK) return bail(rc);
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    if (rc == AMT_OK && bytes &&
+        hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
+      rc = bfail(AMT_EHIP, "upload failed");
+  };
+  up(b->D, b->hp.data(), sizeof(BProb) * n);
+  for (int k = 0; k < BK_KINDS; ++k) up(b->ip[k], ipv[k].data(), sizeof(int) * ipv[k].size());
+  up(b->Ke64, ke64.data(), sizeof(double) * ke64.size());
+  up(b->Ke32, ke32.data(), sizeof(float) * ke32.size());
+  up(b->st, st_all.data(), sizeof(StencilEntry) * st_all.size());
+  up(b->sw, sw_all.data(), sizeof(double) * sw_all.size());
+  // initial density: each problem's volume fraction (topology_optimizer.cpp:403-405)
+  {
+    std::vector<double> rho(ne);
+    for (int i = 0; i < n; ++i)
+      std::fill(rho.begin() + b->hp[i].e_off, rho.begin() + b->hp[i].e_off + b->hp[i].g.n_elem,
+                cfgs[i].volume_fraction);
+    up(b->rho, rho.data(), sizeof(double) * ne);
+  }
+  if (rc != AMT_OK) return bail(rc);
+  if (hipMemset(b->fixcnt, 0, nd) != hipSuccess || hipMemset(b->F, 0, nd * 8) != hipSuccess ||
+      hipMemset(b->u, 0, nd * 8) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+    return bail(bfail(AMT_EHIP, "initialisation failed"));
+  *out = b;
+  return AMT_OK;
+}
+
+int amt_batch_destroy(amt_batch* b) {
+  destroy(b);
+  return AMT_OK;
+}
+
+int amt_batch_size(const amt_batch* b, int i, int64_t* n_elements, int64_t* n_dofs) {
+  if (!b || i < 0 || i >= b->np) return bfail(AMT_ERANGE, "problem index out of range");
+  if (n_elements) *n_elements = b->hp[i].g.n_elem;
+  if (n_dofs) *n_dofs = 3 * b->hp[i].g.n_nodes;
+  return AMT_OK;
+}
+
+int amt_batch_set_loads(amt_batch* b, int i, const double* F, int64_t n_dof,
+                        const int64_t* fixed, int64_t n_fixed) {
+  if (!b || i < 0 || i >= b->np) return bfail(AMT_ERANGE, "problem index out of range");
+  const BProb& P = b->hp[i];
+  const int64_t nd = 3 * P.g.n_nodes;
+  if (!F || n_dof != nd)
+    return bfail(AMT_EINVAL, "force vector size mismatch: expected " + std::to_string(nd) +
+                                 ", got " + std::to_string(n_dof));
+  if (n_fixed < 0 || (n_fixed > 0 && !fixed)) return bfail(AMT_EINVAL, "bad fixed list");
+  std::vector<uint8_t> cnt(nd, 0);
+  for (int64_t t = 0; t < n_fixed; ++t) {
+    const int64_t d = fixed[t];
+    if (d < 0 || d >= nd)
+      return bfail(AMT_ERANGE, "Fixed DOF index out of range: " + std::to_string(d));
+    if (cnt[d] == 255) return bfail(AMT_EINVAL, "DOF listed more than 255 times");
+    cnt[d]++;
+  }
+  for (int64_t t = 0; t < nd; ++t)
+    if (!std::isfinite(F[t])) return bfail(AMT_ENAN, "non-finite force");
+  B_HIP(hipStreamSynchronize(b->stream));
+  B_HIP(hipMemcpy(b->F + 3 * P.n_off, F, sizeof(double) * nd, hipMemcpyHostToDevice));
+  B_HIP(hipMemcpy(b->fixcnt + 3 * P.n_off, cnt.data(), nd, hipMemcpyHostToDevice));
+  B_HIP(hipMemset(b->u + 3 * P.n_off, 0, sizeof(double) * nd));
+  b->loads_set[i] = 1;
+  return AMT_OK;
+}
+
+int amt_batch_set_density(amt_batch* b, int i, const double* rho, int64_t n_e) {
+  if (!b || i < 0 || i >= b->np) return bfail(AMT_ERANGE, "problem index out of range");
+  const BProb& P = b->hp[i];
+  if (!rho || n_e != P.g.n_elem)
+    return bfail(AMT_EINVAL, "Density vector size mismatch: expected " +
+                                 std::to_string(P.g.n_elem) + ", got " + std::to_string(n_e));
+  for (int64_t t = 0; t < n_e; ++t)
+    if (!std::isfinite(rho[t])) return bfail(AMT_ENAN, "non-finite density");
+  B_HIP(hipStreamSynchronize(b->stream));
+  B_HIP(hipMemcpy(b->rho + P.e_off, rho, sizeof(double) * n_e, hipMemcpyHostToDevice));
+  return AMT_OK;
+}
+
+int amt_batch_get_density(amt_batch* b, int i, double* rho, int64_t n_e) {
+  if (!b || i < 0 || i >= b->np) return bfail(AMT_ERANGE, "problem index out of range");
+  const BProb& P = b->hp[i];
+  if (!rho || n_e != P.g.n_elem) return bfail(AMT_EINVAL, "density buffer size mismatch");
+  B_HIP(hipStreamSynchronize(b->stream));
+  B_HIP(hipMemcpy(rho, b->rho + P.e_off, sizeof(double) * n_e, hipMemcpyDeviceToHost));
+  return AMT_OK;
+}
+
+int amt_batch_get_displacement(amt_batch* b, int i, double* u, int64_t n_dof) {
+  if (!b || i < 0 || i >= b->np) return bfail(AMT_ERANGE, "problem index out of range");
+  const BProb& P = b->hp[i];
+  if (!u || n_dof != 3 * P.g.n_nodes) return bfail(AMT_EINVAL, "displacement buffer size mismatch");
+  B_HIP(hipStreamSynchronize(b->stream));
+  B_HIP(hipMemcpy(u, b->u + 3 * P.n_off, sizeof(double) * n_dof, hipMemcpyDeviceToHost));
+  return AMT_OK;
+}
+
+int amt_batch_run(amt_batch* b, int iterations, amt_iteration_result* hist, double* elapsed_s) {
+  if (!b || iterations < 0) return bfail(AMT_EINVAL, "bad argument");
+  return b->mixed ? run_impl<float>(b, iterations, hist, elapsed_s)
+                  : run_impl<double>(b, iterations, hist, elapsed_s);
+}
+
+}  // extern "C"

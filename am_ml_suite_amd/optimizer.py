@@ -436,6 +436,74 @@ class UNet:
         return out
 
 
+class BatchOptimizer:
+    """Many independent problems optimised in lockstep on one GPU (amt_batch_*): every phase
+    one persistent launch over all problems, the OC step replayed from hipGraphs.
+    problems: list of (nx, ny, nz, element_size, MaterialProperties, SIMPConfig); precision and
+    warm_start must agree; density index consistent, Jacobi-PCG."""
+
+    def __init__(self, problems):
+        self._L = _ffi.load()
+        cfgs = (_ffi.AmtConfig * len(problems))()
+        for t, (nx, ny, nz, h, mat, cfg) in enumerate(problems):
+            cfgs[t] = make_amt_config(nx, ny, nz, h, mat or MaterialProperties(),
+                                      cfg or SIMPConfig())
+        self._h = C.c_void_p()
+        rc = self._L.amt_batch_create(C.cast(cfgs, C.c_void_p), len(problems), C.byref(self._h))
+        if rc != _ffi.AMT_OK:
+            _raise(rc, self._L.amt_last_error(None).decode())
+        self.n = len(problems)
+        self.sizes = []
+        for t in range(self.n):
+            ne, nd = C.c_int64(), C.c_int64()
+            self._check(self._L.amt_batch_size(self._h, t, C.byref(ne), C.byref(nd)))
+            self.sizes.append((ne.value, nd.value))
+
+    def _check(self, rc):
+        if rc != _ffi.AMT_OK:
+            _raise(rc, self._L.amt_last_error(None).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.amt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_loads(self, i, forces, fixed_dofs):
+        F = np.ascontiguousarray(forces, dtype=np.float64)
+        fx = np.ascontiguousarray(fixed_dofs, dtype=np.int64)
+        self._check(self._L.amt_batch_set_loads(self._h, int(i), F, F.size, fx, fx.size))
+
+    def set_density(self, i, rho):
+        r = np.ascontiguousarray(rho, dtype=np.float64)
+        self._check(self._L.amt_batch_set_density(self._h, int(i), r, r.size))
+
+    def get_density(self, i):
+        out = np.zeros(self.sizes[i][0])
+        self._check(self._L.amt_batch_get_density(self._h, int(i), out, out.size))
+        return out
+
+    def get_displacement(self, i):
+        out = np.zeros(self.sizes[i][1])
+        self._check(self._L.amt_batch_get_displacement(self._h, int(i), out, out.size))
+        return out
+
+    def run(self, iterations):
+        """Runs OC iterations; returns (history[iteration][problem] of IterationResult,
+        elapsed seconds)."""
+        hist = (_ffi.AmtIterationResult * max(int(iterations) * self.n, 1))()
+        el = C.c_double()
+        self._check(self._L.amt_batch_run(self._h, int(iterations), C.cast(hist, C.c_void_p),
+                                          C.byref(el)))
+        return ([[_convert(hist[it * self.n + t]) for t in range(self.n)]
+                 for it in range(int(iterations))], el.value)
+
+
 # notebook binding names (cell 23 lines 9-12)
 CPPTopologyOptimizer = TopologyOptimizer
 

@@ -8,6 +8,8 @@
 - cantilever_problem / bracket_like_problem generate the synthetic benchmark problems
   (30x10x10 cantilever; 120x60x60 bracket-like domain with solid pads, a void box and seeded
   point loads) from a splitmix64 stream, so every run sees the same arrays.
+- batch_problems generates the dataset-generation set: many problems of uneven sizes, each
+  with one clamped face and 1-4 point loads on the other faces.
 """
 from dataclasses import dataclass, field
 from typing import List, Tuple
@@ -276,3 +278,45 @@ def make_config(problem, **overrides):
     for k, v in overrides.items():
         setattr(c, k, v)
     return c
+
+
+def _face_nodes(nx, ny, nz, face):
+    """Node ids (z fastest) of one boundary face: 0/1 x=0/x=nx, 2/3 y=0/y=ny, 4/5 z=0/z=nz."""
+    axis, side = face // 2, face % 2
+    dims = (nx + 1, ny + 1, nz + 1)
+    idx = [np.arange(d) for d in dims]
+    idx[axis] = np.array([0 if side == 0 else dims[axis] - 1])
+    I, J, K = np.meshgrid(*idx, indexing="ij")
+    return (I * dims[1] * dims[2] + J * dims[2] + K).ravel()
+
+
+def batch_problems(n=64, seed=3, lo=16, hi=64):
+    """The dataset-generation set: n problems, grid size per axis U{lo..hi}, one random face
+    clamped (all DOFs of its nodes), 1-4 point loads on random free nodes of the other faces
+    (uniform unit direction, magnitude U(0.5, 1)), volume fraction U(0.2, 0.5), filter radius
+    U(1.2, 2.5); E=1, nu=0.3, Emin=1e-9, p=3, move=0.2, rho_min=0.01; initial density = the
+    volume fraction everywhere (all design). Drawn from one splitmix64 stream."""
+    rng = SplitMix64(seed)
+    out = []
+    for t in range(n):
+        nx, ny, nz = (rng.randint(lo, hi) for _ in range(3))
+        n_dofs = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+        clamp = rng.randint(0, 5)
+        clamped = _face_nodes(nx, ny, nz, clamp)
+        fixed = np.sort(np.concatenate([3 * clamped + c for c in range(3)])).astype(np.int64)
+        F = np.zeros(n_dofs)
+        others = [f for f in range(6) if f != clamp]
+        for _ in range(rng.randint(1, 4)):
+            nodes = np.setdiff1d(_face_nodes(nx, ny, nz, others[rng.randint(0, 4)]), clamped)
+            node = int(nodes[rng.randint(0, len(nodes) - 1)])
+            d = rng.unit_vector()
+            mag = rng.uniform(0.5, 1.0)
+            for c in range(3):
+                F[3 * node + c] += mag * d[c]
+        vf = rng.uniform(0.2, 0.5)
+        rmin = rng.uniform(1.2, 2.5)
+        params = dict(volume_fraction=vf, penalty=3.0, filter_radius=rmin, move_limit=0.2,
+                      rho_min=0.01, E_min=1e-9)
+        out.append(Problem("batch%d_%dx%dx%d" % (t, nx, ny, nz), nx, ny, nz,
+                           np.full(nx * ny * nz, vf), F, fixed, params))
+    return out

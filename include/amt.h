@@ -189,6 +189,34 @@ int amt_unet_train(amt_problem* p, amt_unet* net, int steps, double lr, double* 
 int amt_unet_get_gain(amt_unet* net, double* gain);
 int amt_unet_set_gain(amt_unet* net, double gain);
 
+/* ---- batched dataset generation (no counterpart in the reference, which optimises one
+ *      problem per TopologyOptimizer, topology_optimizer.cpp:391-520) ----
+ * n independent problems of different sizes, each with its own amt_config (grid, material,
+ * volume fraction, filter radius, ...), advanced in lockstep: every phase is one persistent
+ * launch over all problems, and the OC step is replayed from hipGraphs. Each problem follows
+ * the single-problem semantics with the consistent density index and Jacobi-PCG:
+ * precision AMT_PREC_MIXED_FP32 runs the PCG matvec in FP32 with FP64 displacement, residual
+ * and reductions and restarts from the FP64 residual until |F_bc - K u| <= pcg_tol |F_bc|;
+ * AMT_PREC_FP64 is plain FP64 Jacobi-PCG. precision and warm_start must agree across
+ * problems. Errors of these calls are read with amt_last_error(NULL). */
+typedef struct amt_batch amt_batch;
+int amt_batch_create(const amt_config* configs, int n, amt_batch** out);
+int amt_batch_destroy(amt_batch* b);
+/* Element and DOF counts of problem i. */
+int amt_batch_size(const amt_batch* b, int i, int64_t* n_elements, int64_t* n_dofs);
+/* As amt_set_loads / amt_set_density / amt_get_density / amt_get_displacement, for problem i.
+ * Densities default to each problem's volume fraction. */
+int amt_batch_set_loads(amt_batch* b, int i, const double* F, int64_t n_dof,
+                        const int64_t* fixed, int64_t n_fixed);
+int amt_batch_set_density(amt_batch* b, int i, const double* rho, int64_t n_e);
+int amt_batch_get_density(amt_batch* b, int i, double* rho, int64_t n_e);
+int amt_batch_get_displacement(amt_batch* b, int i, double* u, int64_t n_dof);
+/* `iterations` OC iterations of every problem (design masks from the densities at the first
+ * call). hist (optional) gets iterations * n results, hist[it * n + i] for problem i;
+ * solver_iterations counts PCG iterations over all restarts, outer_iterations the (re)starts,
+ * and the phase times are the batch's. elapsed_s (optional): wall time of the call. */
+int amt_batch_run(amt_batch* b, int iterations, amt_iteration_result* hist, double* elapsed_s);
+
 /* Wait for all work of this handle. */
 int amt_sync(amt_problem* p);
 /* Device name / CU count of the handle's GPU. */

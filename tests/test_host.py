@@ -76,3 +76,32 @@ def test_bracket_like_problem_deterministic():
     assert (a.rho0 == 1.0).sum() == 2000 and (a.rho0 == 0.0).sum() > 0
     c = A.bracket_like_problem(seed=3)
     assert not np.array_equal(a.F, c.F)
+
+
+def test_batch_problem_generator():
+    """The dataset-generation set: deterministic, sizes in U{16..64}, one whole clamped face
+    (all three DOFs of its nodes), 1-4 point loads of magnitude in [0.5, 1] on other faces,
+    volume fraction in [0.2, 0.5], filter radius in [1.2, 2.5]."""
+    ps = A.batch_problems()
+    qs = A.batch_problems()
+    assert len(ps) == 64
+    for p, q in zip(ps, qs):
+        assert (p.nx, p.ny, p.nz) == (q.nx, q.ny, q.nz)
+        assert np.array_equal(p.F, q.F) and np.array_equal(p.fixed, q.fixed)
+        assert all(16 <= d <= 64 for d in (p.nx, p.ny, p.nz))
+        nn = np.array([p.nx + 1, p.ny + 1, p.nz + 1])
+        nodes = np.unique(p.fixed // 3)
+        assert p.fixed.size == 3 * nodes.size
+        I, J, K = nodes // (nn[1] * nn[2]), (nodes // nn[2]) % nn[1], nodes % nn[2]
+        on = [np.all(I == 0), np.all(I == p.nx), np.all(J == 0), np.all(J == p.ny),
+              np.all(K == 0), np.all(K == p.nz)]
+        face = on.index(True)
+        assert nodes.size == np.prod(np.delete(nn, face // 2))
+        loaded = np.unique(np.nonzero(p.F)[0] // 3)
+        assert 1 <= loaded.size <= 4
+        assert not np.any(np.isin(loaded, nodes))
+        mags = np.linalg.norm(p.F.reshape(-1, 3)[loaded], axis=1)
+        assert np.all(mags >= 0.5 - 1e-12) and np.all(mags <= 4.0)
+        assert 0.2 <= p.params["volume_fraction"] <= 0.5
+        assert 1.2 <= p.params["filter_radius"] <= 2.5
+        assert np.all(p.rho0 == p.params["volume_fraction"])
