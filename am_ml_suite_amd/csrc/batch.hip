@@ -10,9 +10,10 @@
 // density update) are captured once as hipGraphs and replayed; the host only polls the
 // per-problem solver flags between chunk replays. Problems that converge drop out of the
 // chunk at once (their items exit at the top). PCG runs its matvec in T (FP32 in the mixed
-// mode) with FP64 u, r and reductions; when every problem has converged on its recursive
-// residual, the FP64 residual F_bc - K u is recomputed and problems above tolerance restart
-// from it (FP64 residual replacement), so results meet the tolerance on the true residual.
+// mode) with FP64 u, r and reductions. After every chunk the FP64 residual F_bc - K u replaces
+// the recursive one (van der Vorst-Ye residual replacement: the search direction is kept), so
+// convergence is judged on the true residual and the FP32 matvec error cannot let the two
+// drift apart.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -34,7 +35,6 @@ using namespace amt;
 namespace {
 
 constexpr int kBChunk = 16;        // PCG iterations per chunk graph (even: parity returns)
-constexpr int kBMaxCycles = 8;     // FP64 residual restarts per solve
 constexpr int kBPersist = 2048;    // blocks of a persistent launch (8 per CU)
 constexpr int kBOcLevels = 6;
 constexpr int kBOcCand = (1 << kBOcLevels) - 1;
@@ -137,6 +137,20 @@ void enqueue_pre(amt_batch* b) {
   if (!b->warm)
     hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)((b->n_dof + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, b->stream, b->u, 0.0, b->n_dof);
+}
+
+// Residual replacement after a chunk loop: FP64 residual from u, per-problem decision
+// (k_b_replace), last directions moved to buffer 0.
+template <typename T>
+void enqueue_replace(amt_batch* b) {
+  const int nt = b->n_items[BK_TILE], nd = b->n_items[BK_DOF];
+  hipLaunchKernelGGL((k_b_residual<T>), dim3(grid_of(nt)), dim3(kBlock), 0, b->stream, b->D,
+                     b->ip[BK_TILE], nt, b->u, b->Fb, b->r, (T*)b->z, b->dinv, b->s64, b->Ke64,
+                     b->fixcnt, b->pen, b->part_tile);
+  hipLaunchKernelGGL(k_b_replace, dim3(b->np), dim3(kBlock), 0, b->stream, 0, b->D,
+                     b->part_tile, nt, b->S, b->part_dof, nd, b->flags);
+  hipLaunchKernelGGL((k_b_carry_p<T>), dim3(grid_of(nd)), dim3(kBlock), 0, b->stream, b->D,
+                     b->ip[BK_DOF], nd, b->flags, (T*)b->p0, (const T*)b->p1);
 }
 
 // FP64 residual from u, PCG state per problem, p_old = 0.
@@ -248,7 +262,7 @@ int run_impl(amt_batch* b, int iterations, amt_iteration_result* hist, double* e
   if (!b->design_ready) B_CHECK(build_design(b));
   B_CHECK(capture(b, &b->g_pre, [&] { enqueue_pre<T>(b); }));
   B_CHECK(capture(b, &b->g_start0, [&] { enqueue_start<T>(b, 1); }));
-  B_CHECK(capture(b, &b->g_start1, [&] { enqueue_start<T>(b, 0); }));
+  B_CHECK(capture(b, &b->g_start1, [&] { enqueue_replace<T>(b); }));
   B_CHECK(capture(b, &b->g_chunk, [&] { enqueue_chunk<T>(b); }));
   B_CHECK(capture(b, &b->g_post, [&] { enqueue_post(b); }));
   std::vector<BState> f(b->np);
@@ -260,15 +274,15 @@ int run_impl(amt_batch* b, int iterations, amt_iteration_result* hist, double* e
     B_HIP(hipGraphLaunch(b->g_pre, b->stream));
     B_HIP(hipStreamSynchronize(b->stream));
     const double t1 = now_s();
-    int cycles = 0, active = 0;
-    for (; cycles <= kBMaxCycles; ++cycles) {
-      B_HIP(hipGraphLaunch(cycles == 0 ? b->g_start0 : b->g_start1, b->stream));
+    // fresh start, then chunks of kBChunk iterations, each followed by residual replacement
+    int chunks = 0, active = 0;
+    B_HIP(hipGraphLaunch(b->g_start0, b->stream));
+    B_CHECK(read_flags(b, f, &active));
+    while (active > 0) {
+      B_HIP(hipGraphLaunch(b->g_chunk, b->stream));
+      B_HIP(hipGraphLaunch(b->g_start1, b->stream));
       B_CHECK(read_flags(b, f, &active));
-      if (active == 0 || cycles == kBMaxCycles) break;
-      while (active > 0) {
-        B_HIP(hipGraphLaunch(b->g_chunk, b->stream));
-        B_CHECK(read_flags(b, f, &active));
-      }
+      ++chunks;
     }
     const double t2 = now_s();
     B_HIP(hipGraphLaunch(b->g_post, b->stream));
@@ -289,7 +303,7 @@ int run_impl(amt_batch* b, int iterations, amt_iteration_result* hist, double* e
         R.bisect_passes = oc[i].passes;
         R.solver_iterations = f[i].it;
         R.solver_converged = f[i].converged;
-        R.outer_iterations = cycles + 1;
+        R.outer_iterations = chunks;  // residual replacements
         R.true_rel_residual = f[i].rhs2 > 0 ? std::sqrt(f[i].rr / f[i].rhs2) : 0.0;
         R.assembly_time_s = t1 - t0;
         R.solve_time_s = t2 - t1;

@@ -36,9 +36,10 @@ struct BProb {
 
 // Per-problem PCG state (two copies alternate by iteration parity, as in the single-problem
 // solver). fresh: set by the start kernel; the first iteration after it does no bookkeeping.
+// lastp: parity of the search-direction buffer holding the last p when convergence was seen.
 struct BState {
   double rz, rr, rhs2, thresh;
-  int it, done, converged, maxit, fresh, pad;
+  int it, done, converged, maxit, fresh, lastp;
 };
 
 __device__ __forceinline__ int bp_first(int n) {
@@ -240,15 +241,16 @@ __global__ __launch_bounds__(kBlock) void k_b_start(const BProb* __restrict__ D,
     st.converged = (st.rhs2 == 0.0) || (t[0] < st.thresh);
     st.done = st.converged || st.it >= st.maxit;
     st.fresh = 1;
-    st.pad = 0;
+    st.lastp = 0;
     S[2 * pi] = st;
     flags[pi] = st;
   }
 }
 
-// Bookkeeping at the top of an iteration (pcg_step_state with the fresh flag for t == 0).
-__device__ __forceinline__ BState b_step_state(const BState& cur, const double* upart, int nbu,
-                                               int stride, double& beta) {
+// Bookkeeping at the top of an iteration of parity par (pcg_step_state with the fresh flag
+// for t == 0).
+__device__ __forceinline__ BState b_step_state(const BState& cur, int par, const double* upart,
+                                               int nbu, int stride, double& beta) {
   BState nx = cur;
   beta = 0.0;
   nx.fresh = 0;
@@ -263,6 +265,7 @@ __device__ __forceinline__ BState b_step_state(const BState& cur, const double* 
   if (v[0] < cur.thresh) {
     nx.done = 1;
     nx.converged = 1;
+    nx.lastp = par;  // this iteration would have read its p_old from buffer par
     return nx;
   }
   beta = v[1] / cur.rz;
@@ -288,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_b_apply(
     const BState cur = S[2 * pi + par];
     double beta_d;
     const BState nx =
-        b_step_state(cur, upart + P.item0[BK_DOF], P.nitem[BK_DOF], ustride, beta_d);
+        b_step_state(cur, par, upart + P.item0[BK_DOF], P.nitem[BK_DOF], ustride, beta_d);
     if (w == P.item0[BK_TILE] && threadIdx.x == 0) S[2 * pi + (par ^ 1)] = nx;
     if (nx.done) continue;  // uniform over the block
     const GridDims g = P.g;
@@ -371,10 +374,80 @@ __global__ __launch_bounds__(kBlock) void k_b_check(int par, const BProb* __rest
   for (int pi = 0; pi < np; ++pi) {
     const BProb& P = D[pi];
     double beta;
-    const BState nx =
-        b_step_state(S[2 * pi + par], upart + P.item0[BK_DOF], P.nitem[BK_DOF], ustride, beta);
+    const BState nx = b_step_state(S[2 * pi + par], par, upart + P.item0[BK_DOF],
+                                   P.nitem[BK_DOF], ustride, beta);
     if (threadIdx.x == 0) flags[pi] = nx;
     __syncthreads();
+  }
+}
+
+// Residual replacement (van der Vorst and Ye), one block per problem, after every chunk:
+// k_b_residual has recomputed r = F_bc - K u in FP64 and z = D^-1 r. par: parity of the state
+// slot the chunk left current.
+//  - Running problem: the DOF-item partials its next bookkeeping sums are set to the true
+//    |r|^2 and r.z (first partial, rest 0), so the next iteration judges convergence on the
+//    true residual and forms beta = r.z / rz against its last direction; the state is kept.
+//  - Problem stopped as converged on its recursive residual: final if the true one meets the
+//    tolerance, else reopened the same way (its last p is carried to buffer 0 when it lies in
+//    buffer 1: flags.lastp = 1 asks k_b_carry_p for it).
+//  - Problem stopped otherwise (iteration limit, breakdown): stays stopped.
+// Writes S[2p + par] and flags[p]; flags.rr is the true |r|^2.
+__global__ __launch_bounds__(kBlock) void k_b_replace(int par, const BProb* __restrict__ D,
+                                                      const double* __restrict__ tpart,
+                                                      int tstride, BState* __restrict__ S,
+                                                      double* __restrict__ upart, int ustride,
+                                                      BState* __restrict__ flags) {
+  const int pi = blockIdx.x;
+  const BProb& P = D[pi];
+  double t[2];
+  reduce_partials<2>(tpart + P.item0[BK_TILE], P.nitem[BK_TILE], tstride, t);
+  BState cur = S[2 * pi + par];
+  const bool reopen = cur.done && cur.converged && !(t[0] < cur.thresh) && cur.it < cur.maxit;
+  const bool run = !cur.done || reopen;
+  if (run) {
+    for (int b = threadIdx.x; b < P.nitem[BK_DOF]; b += kBlock) {
+      double* u0 = upart + P.item0[BK_DOF] + b;
+      u0[0] = b == 0 ? t[0] : 0.0;
+      u0[ustride] = b == 0 ? t[1] : 0.0;
+    }
+  }
+  if (threadIdx.x == 0) {
+    BState f = cur;
+    if (reopen) {
+      cur.done = 0;
+      cur.converged = 0;
+      cur.fresh = 0;
+      f = cur;             // flags.lastp keeps the buffer of the last p for k_b_carry_p
+      cur.lastp = 0;
+    } else {
+      f.lastp = 0;
+    }
+    f.rr = t[0];
+    S[2 * pi + par] = cur;
+    flags[pi] = f;
+  }
+}
+
+// p_old buffer 0 <- buffer 1 for the problems continuing after a replacement whose last
+// direction lies in buffer 1 (the next chunk starts at parity 0).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_carry_p(const BProb* __restrict__ D,
+                                                      const int* __restrict__ ip, int n,
+                                                      const BState* __restrict__ flags,
+                                                      T* __restrict__ p0,
+                                                      const T* __restrict__ p1) {
+  AMT_ITEMS(w, n) {
+    const int pi = ip[w];
+    const BState f = flags[pi];
+    if (f.done || f.lastp != 1) continue;
+    const BProb P = D[pi];
+    const int64_t nd = 3 * P.g.n_nodes, off = 3 * P.n_off;
+    const int64_t base = (int64_t)(w - P.item0[BK_DOF]) * kBDofItem;
+    for (int l = 0; l < kBDofItem / kBlock; ++l) {
+      const int64_t i = base + l * kBlock + threadIdx.x;
+      if (i >= nd) break;
+      p0[off + i] = p1[off + i];
+    }
   }
 }
 

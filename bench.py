@@ -12,6 +12,9 @@ the same W + K OC iterations:
             FP64 residual (the true residual is recomputed and reported).
 `value` is the fast path's OC iterations/s; the plain path and the speed-up are reported
 beside it, with the fast path's compliance/density error against the plain FP64 path.
+The "batch" object reports the dataset-generation workload: 64 generated problems of sizes
+U{16..64}^3 (13.0M DOFs together) advanced in lockstep by the batched engine (FP32 matvec,
+FP64 vectors and reductions, residual replacement, every solve to the 1e-8 true residual).
 """
 import argparse
 import json
@@ -68,6 +71,32 @@ def run_path(problem, warmup, steps, precision, warm_start, profile, precond="ja
     return dict(time=dt, results=res, density=dens, prof=prof)
 
 
+def run_batch(warmup, steps):
+    import am_ml_suite_amd as A
+    from am_ml_suite_amd.problems import make_config
+    ps = A.batch_problems()
+    B = A.BatchOptimizer([(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3),
+                           make_config(p, precision="mixed_fp32", warm_start=True))
+                          for p in ps])
+    for i, p in enumerate(ps):
+        B.set_loads(i, p.F, p.fixed)
+        B.set_density(i, p.rho0)
+    if warmup:
+        B.run(warmup)
+    hist, el = B.run(steps)
+    n_dof = sum(p.n_dofs for p in ps)
+    flat = [r for row in hist for r in row]
+    log("  [batch] %d OC iterations of %d problems in %.3fs" % (steps, len(ps), el))
+    return dict(workload="batch_problems_n64_seed3", n_problems=len(ps), n_dofs_total=n_dof,
+                n_elements_total=sum(p.n_elements for p in ps), precision="fp32 matvec + fp64",
+                steps=steps, warmup=warmup, time_s=el, oc_iter_per_s=steps / el,
+                problem_oc_iter_per_s=steps * len(ps) / el, dof_oc_iter_per_s=n_dof * steps / el,
+                max_pcg_iters=max(r.solver_iterations for r in flat),
+                mean_pcg_iters=float(np.mean([r.solver_iterations for r in flat])),
+                all_converged=all(r.solver_converged for r in flat),
+                max_true_rel_residual=max(r.true_rel_residual for r in flat))
+
+
 def cpu_baseline(problem, gpu_iters_per_oc, max_pcg=150):
     """Oracle (C++/OpenMP restatement of the reference path, assembled CSR) on the host: one
     OC iteration with PCG capped at max_pcg passes; the solve time is scaled to the PCG
@@ -103,7 +132,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--paths", default="plain,fast")
+    ap.add_argument("--paths", default="plain,fast,batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -122,6 +151,11 @@ def main():
     if "fast" in paths:
         log("fast: FP64 flexible CG + FP32 multigrid V-cycle + warm start")
         out["fast"] = run_path(p, args.warmup, args.steps, "fp64", True, True, "multigrid")
+
+    batch = None
+    if "batch" in paths:
+        log("batch: 64 generated problems in lockstep")
+        batch = run_batch(1, 3)
 
     main_key = "fast" if "fast" in out else "plain"
     m = out[main_key]
@@ -180,7 +214,7 @@ def main():
                    "n_elements": n_e, "pcg_tol": 1e-8, "path": main_key,
                    "solver": ("FP64 flexible CG + FP32 Galerkin multigrid V-cycle, warm start"
                               if main_key == "fast" else "FP64 Jacobi-PCG")},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "batch": batch,
     }
     line.update(extra)
     print(json.dumps(line), flush=True)

@@ -196,7 +196,8 @@ int amt_unet_set_gain(amt_unet* net, double gain);
  * launch over all problems, and the OC step is replayed from hipGraphs. Each problem follows
  * the single-problem semantics with the consistent density index and Jacobi-PCG:
  * precision AMT_PREC_MIXED_FP32 runs the PCG matvec in FP32 with FP64 displacement, residual
- * and reductions and restarts from the FP64 residual until |F_bc - K u| <= pcg_tol |F_bc|;
+ * and reductions, replacing the recursive residual by the FP64 one every 16 iterations, until
+ * |F_bc - K u| <= pcg_tol |F_bc|;
  * AMT_PREC_FP64 is plain FP64 Jacobi-PCG. precision and warm_start must agree across
  * problems. Errors of these calls are read with amt_last_error(NULL). */
 typedef struct amt_batch amt_batch;
@@ -213,7 +214,7 @@ int amt_batch_get_density(amt_batch* b, int i, double* rho, int64_t n_e);
 int amt_batch_get_displacement(amt_batch* b, int i, double* u, int64_t n_dof);
 /* `iterations` OC iterations of every problem (design masks from the densities at the first
  * call). hist (optional) gets iterations * n results, hist[it * n + i] for problem i;
- * solver_iterations counts PCG iterations over all restarts, outer_iterations the (re)starts,
+ * solver_iterations counts PCG iterations, outer_iterations the residual replacements,
  * and the phase times are the batch's. elapsed_s (optional): wall time of the call. */
 int amt_batch_run(amt_batch* b, int iterations, amt_iteration_result* hist, double* elapsed_s);
 
