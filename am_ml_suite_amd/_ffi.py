@@ -12,14 +12,15 @@ LIB_PATH = os.path.join(PKG, "libamt.so")
 
 AMT_OK, AMT_EINVAL, AMT_ERANGE, AMT_EHIP, AMT_ENAN, AMT_ENOTSUP = range(6)
 AMT_INDEX_CONSISTENT, AMT_INDEX_REF_COMPAT = 0, 1
-AMT_PREC_FP64, AMT_PREC_MIXED_FP32 = 0, 1
+AMT_PREC_FP64, AMT_PREC_MIXED_FP32, AMT_PREC_MIXED_BF16 = 0, 1, 2
 AMT_PRECOND_JACOBI, AMT_PRECOND_UNET, AMT_PRECOND_MULTIGRID = 0, 1, 2
 
 # every symbol include/amt.h declares
 EXPORTS = [
     "amt_config_default", "amt_create", "amt_destroy", "amt_last_error", "amt_set_loads",
     "amt_set_density", "amt_get_density", "amt_get_displacement", "amt_begin_run",
-    "amt_run_iteration", "amt_run", "amt_apply_operator", "amt_get_diagonal", "amt_solve",
+    "amt_run_iteration", "amt_run", "amt_apply_operator", "amt_apply_operator_prec",
+    "amt_get_diagonal", "amt_solve",
     "amt_sensitivities", "amt_filter", "amt_filter_neighbors", "amt_oc_update", "amt_sync",
     "amt_device_info", "amt_profile_enable", "amt_profile_read", "amt_element_stiffness",
     "amt_unet_create", "amt_unet_create_for", "amt_unet_destroy", "amt_unet_num_params",
@@ -101,6 +102,7 @@ def load():
     L.amt_run.argtypes = [P, C.c_int, C.POINTER(AmtIterationResult), C.POINTER(C.c_int),
                           C.POINTER(C.c_int), C.POINTER(C.c_double), ITER_CB, C.c_void_p]
     L.amt_apply_operator.argtypes = [P, _dp, _dp, C.c_int]
+    L.amt_apply_operator_prec.argtypes = [P, _dp, _dp, C.c_int, C.c_int]
     L.amt_get_diagonal.argtypes = [P, _dp, C.c_int, C.POINTER(C.c_double)]
     L.amt_solve.argtypes = [P, _dp, C.POINTER(AmtSolveStats)]
     L.amt_sensitivities.argtypes = [P, _dp, _dp]

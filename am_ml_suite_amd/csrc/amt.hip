@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -23,6 +24,7 @@
 #include "host_util.hpp"
 #include "kernels.hpp"
 #include "multigrid.hpp"
+#include "mfma_op.hpp"
 #include "unet_api.hpp"
 #include "unet_glue.hpp"
 #include <cstddef>
@@ -76,6 +78,7 @@ struct amt_problem {
 
   double* Ke64 = nullptr;
   float* Ke32 = nullptr;
+  uint16_t *Kh16 = nullptr, *Kl16 = nullptr;  // split-BF16 Ke0 of the MFMA operator
   double *rho = nullptr, *s64 = nullptr;
   float* s32 = nullptr;
   uint8_t *design = nullptr, *fixcnt = nullptr;
@@ -397,9 +400,15 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q) {
   auto& mg = p->mg;
   const int* done = (const int*)((const char*)p->fcg + offsetof(FcgState, done));
   const size_t L = mg.size();
-  hipLaunchKernelGGL(k_mg0_pre, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g, p->tg,
-                     p->s32, p->Ke32, p->fixcnt, p->scal + S_P, mg[0].wd, mg[0].x, mg[0].t,
-                     done);
+  const bool mf = p->cfg.precision == AMT_PREC_MIXED_BF16;
+  if (mf)
+    hipLaunchKernelGGL(k_mg0_pre_mfma, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g,
+                       p->tg, p->s32, p->Kh16, p->Kl16, p->fixcnt, p->scal + S_P, mg[0].wd,
+                       mg[0].x, mg[0].t, done);
+  else
+    hipLaunchKernelGGL(k_mg0_pre, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g, p->tg,
+                       p->s32, p->Ke32, p->fixcnt, p->scal + S_P, mg[0].wd, mg[0].x, mg[0].t,
+                       done);
   hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[1].g.n_nodes)), dim3(kBlock), 0, p->stream,
                      mg[0].t, mg[0].g, mg[1].rhs, mg[1].g, done);
   for (size_t l = 1; l + 1 < L; ++l) {
@@ -420,9 +429,14 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q) {
   }
   hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, res,
                      mg[1].g, mg[0].x, mg[0].g, done);
-  hipLaunchKernelGGL(k_mg0_post, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g, p->tg,
-                     p->s32, p->Ke32, p->fixcnt, p->scal + S_P, mg[0].wd, mg[0].x, z, q,
-                     p->part(R_INIT), p->stride_red, done);
+  if (mf)
+    hipLaunchKernelGGL(k_mg0_post_mfma, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g,
+                       p->tg, p->s32, p->Kh16, p->Kl16, p->fixcnt, p->scal + S_P, mg[0].wd,
+                       mg[0].x, z, q, p->part(R_INIT), p->stride_red, done);
+  else
+    hipLaunchKernelGGL(k_mg0_post, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g, p->tg,
+                       p->s32, p->Ke32, p->fixcnt, p->scal + S_P, mg[0].wd, mg[0].x, z, q,
+                       p->part(R_INIT), p->stride_red, done);
 }
 
 // kMgChunk flexible-CG iterations (apply, update, V-cycle, scalar). kMgChunk is even, so
@@ -568,7 +582,7 @@ void prof_harvest(amt_problem* p) {
 template <typename T, typename TK, typename TX>
 int pcg_run(amt_problem* p, const double* b, TX* x, T* r, T* z, T** pp, T* q, const T* s,
             const TK* Ke, const T* dinv, double tol, int maxit, PcgState* out,
-            const std::function<bool()>* on_chunk = nullptr) {
+            const std::function<bool()>* on_chunk = nullptr, bool mfma = false) {
   const GridDims g = p->g;
   const int nbu = (int)nred(p->n_dof), nba = (int)p->nb_tile;
   hipLaunchKernelGGL(k_fill<T>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, pp[0], T(0),
@@ -586,10 +600,23 @@ int pcg_run(amt_problem* p, const double* b, TX* x, T* r, T* z, T** pp, T* q, co
     for (int c = 0; c < kChunk; ++c, ++t) {
       hipEvent_t ev;
       prof_begin(p, &ev);
-      hipLaunchKernelGGL((k_pcg_apply<T, TK>), dim3(nba), dim3(kBlock), 0, p->stream, t, z,
-                         pp[t & 1], pp[(t + 1) & 1], q, s, Ke, p->fixcnt, p->scal + S_P, g,
-                         p->tg, p->pcg, (const double*)p->part(R_UPD), nbu, p->part(R_APPLY),
-                         p->stride_red);
+      if constexpr (std::is_same<T, float>::value) {
+        if (mfma)
+          hipLaunchKernelGGL(k_pcg_apply_mfma, dim3(nba), dim3(kBlock), 0, p->stream, t, z,
+                             pp[t & 1], pp[(t + 1) & 1], q, s, p->Kh16, p->Kl16, p->fixcnt,
+                             p->scal + S_P, g, p->tg, p->pcg, (const double*)p->part(R_UPD),
+                             nbu, p->part(R_APPLY), p->stride_red);
+        else
+          hipLaunchKernelGGL((k_pcg_apply<T, TK>), dim3(nba), dim3(kBlock), 0, p->stream, t, z,
+                             pp[t & 1], pp[(t + 1) & 1], q, s, Ke, p->fixcnt, p->scal + S_P, g,
+                             p->tg, p->pcg, (const double*)p->part(R_UPD), nbu,
+                             p->part(R_APPLY), p->stride_red);
+      } else {
+        hipLaunchKernelGGL((k_pcg_apply<T, TK>), dim3(nba), dim3(kBlock), 0, p->stream, t, z,
+                           pp[t & 1], pp[(t + 1) & 1], q, s, Ke, p->fixcnt, p->scal + S_P, g,
+                           p->tg, p->pcg, (const double*)p->part(R_UPD), nbu, p->part(R_APPLY),
+                           p->stride_red);
+      }
       prof_end(p, ev);
       hipLaunchKernelGGL((k_pcg_update<T, TX>), dim3(nbu), dim3(kBlock), 0, p->stream, t, x, r,
                          z, pp[(t + 1) & 1], q, dinv, p->n_dof, p->pcg,
@@ -689,7 +716,8 @@ int solve_mixed(amt_problem* p, amt_solve_stats* st) {
     const int budget = std::min(p->cfg.inner_maxit, p->cfg.pcg_maxit - inner_total);
     AMT_CHECK((pcg_run<float, float, float>(p, nullptr, p->d32, p->r32, p->z32, p->p32,
                                             p->q32, p->s32, p->Ke32, p->dinv32,
-                                            p->cfg.inner_tol, budget, &hs)));
+                                            p->cfg.inner_tol, budget, &hs, nullptr,
+                                            p->cfg.precision == AMT_PREC_MIXED_BF16)));
     inner_total += std::max(hs.it, 1);
     hipLaunchKernelGGL(k_axpy_cast, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->u,
                        p->d32, nr, p->n_dof);
@@ -898,7 +926,7 @@ int do_solve(amt_problem* p, amt_solve_stats* st) {
   else if (p->cfg.precond == AMT_PRECOND_UNET)
     rc = solve_unet(p, st);
   else
-    rc = p->cfg.precision == AMT_PREC_MIXED_FP32 ? solve_mixed(p, st) : solve_fp64(p, st);
+    rc = p->cfg.precision == AMT_PREC_FP64 ? solve_fp64(p, st) : solve_mixed(p, st);
   if (rc == AMT_OK) p->have_u = true;
   return rc;
 }
@@ -1050,6 +1078,9 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
   if (cfg->precond != AMT_PRECOND_JACOBI && cfg->precond != AMT_PRECOND_MULTIGRID &&
       cfg->precond != AMT_PRECOND_UNET)
     return fail(nullptr, AMT_EINVAL, "unknown preconditioner");
+  if (cfg->precision != AMT_PREC_FP64 && cfg->precision != AMT_PREC_MIXED_FP32 &&
+      cfg->precision != AMT_PREC_MIXED_BF16)
+    return fail(nullptr, AMT_EINVAL, "unknown precision");
   const int64_t n_n = (int64_t)(cfg->nx + 1) * (cfg->ny + 1) * (cfg->nz + 1);
   if (3 * n_n > (int64_t)INT32_MAX) return fail(nullptr, AMT_EINVAL, "grid too large");
   int ndev = 0;
@@ -1085,6 +1116,8 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
   const int64_t nd = p->n_dof, ne = p->n_e;
   CREATE_TRY(dalloc(p, &p->Ke64, 576));
   CREATE_TRY(dalloc(p, &p->Ke32, 576));
+  CREATE_TRY(dalloc(p, &p->Kh16, 576));
+  CREATE_TRY(dalloc(p, &p->Kl16, 576));
   CREATE_TRY(dalloc(p, &p->rho, ne));
   CREATE_TRY(dalloc(p, &p->s64, ne));
   CREATE_TRY(dalloc(p, &p->s32, ne));
@@ -1101,7 +1134,7 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
   CREATE_TRY(dalloc(p, &p->p[0], nd));
   CREATE_TRY(dalloc(p, &p->p[1], nd));
   CREATE_TRY(dalloc(p, &p->q, nd));
-  if (cfg->precision == AMT_PREC_MIXED_FP32) {
+  if (cfg->precision == AMT_PREC_MIXED_FP32 || cfg->precision == AMT_PREC_MIXED_BF16) {
     CREATE_TRY(dalloc(p, &p->r32, nd));
     CREATE_TRY(dalloc(p, &p->z32, nd));
     CREATE_TRY(dalloc(p, &p->p32[0], nd));
@@ -1149,6 +1182,27 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
   for (int i = 0; i < 576; ++i) Kf[i] = (float)Ke[i];
   CREATE_TRY(copy_in(p, p->Ke64, Ke, sizeof(Ke)));
   CREATE_TRY(copy_in(p, p->Ke32, Kf, sizeof(Kf)));
+  {
+    // v = hi + lo, hi = bf16(v), lo = bf16(v - hi), round to nearest even from FP32
+    auto bf = [](float f) {
+      uint32_t u;
+      std::memcpy(&u, &f, 4);
+      return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+    };
+    auto fb = [](uint16_t b) {
+      const uint32_t u = (uint32_t)b << 16;
+      float f;
+      std::memcpy(&f, &u, 4);
+      return f;
+    };
+    uint16_t kh[576], kl[576];
+    for (int i = 0; i < 576; ++i) {
+      kh[i] = bf(Kf[i]);
+      kl[i] = bf(Kf[i] - fb(kh[i]));
+    }
+    CREATE_TRY(copy_in(p, p->Kh16, kh, sizeof(kh)));
+    CREATE_TRY(copy_in(p, p->Kl16, kl, sizeof(kl)));
+  }
   if (cfg->precond == AMT_PRECOND_MULTIGRID) CREATE_TRY(mg_build_hierarchy(p, Ke));
   if (!st.empty()) {
     CREATE_TRY(copy_in(p, p->stencil, st.data(), st.size() * sizeof(StencilEntry)));
@@ -1284,6 +1338,41 @@ int amt_apply_operator(amt_problem* p, const double* x, double* y, int with_bc) 
                      with_bc ? (const double*)(p->scal + S_P) : nullptr, p->g, p->tg);
   AMT_CHECK(check_launch(p, "k_apply"));
   return copy_out(p, y, p->q, sizeof(double) * p->n_dof);
+}
+
+int amt_apply_operator_prec(amt_problem* p, const double* x, double* y, int with_bc,
+                            int precision) {
+  if (!p || !x || !y) return AMT_EINVAL;
+  if (precision == AMT_PREC_FP64) return amt_apply_operator(p, x, y, with_bc);
+  if (precision != AMT_PREC_MIXED_FP32 && precision != AMT_PREC_MIXED_BF16)
+    return fail(p, AMT_EINVAL, "unknown precision");
+  if (with_bc && !p->loads_set) return fail(p, AMT_EINVAL, "loads not set (amt_set_loads)");
+  if (!p->coeffs_ready) AMT_CHECK(prepare_coeffs(p));
+  float *xf = nullptr, *yf = nullptr;
+  HIP_TRY(p, hipMalloc(&xf, sizeof(float) * p->n_dof));
+  if (hipMalloc(&yf, sizeof(float) * p->n_dof) != hipSuccess) {
+    (void)hipFree(xf);
+    return fail(p, AMT_EHIP, "hipMalloc failed");
+  }
+  int rc = copy_in(p, p->z, x, sizeof(double) * p->n_dof);
+  if (rc == AMT_OK) {
+    hipLaunchKernelGGL(k_d2f, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->z, xf,
+                       p->n_dof);
+    const double* P = with_bc ? (const double*)(p->scal + S_P) : nullptr;
+    if (precision == AMT_PREC_MIXED_BF16)
+      hipLaunchKernelGGL(k_apply_mfma, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, xf, yf,
+                         p->s32, p->Kh16, p->Kl16, p->fixcnt, P, p->g, p->tg);
+    else
+      hipLaunchKernelGGL((k_apply<float, float>), dim3(p->nb_tile), dim3(kBlock), 0, p->stream,
+                         xf, yf, p->s32, p->Ke32, p->fixcnt, P, p->g, p->tg);
+    hipLaunchKernelGGL(k_f2d, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, yf, p->q,
+                       p->n_dof);
+    rc = check_launch(p, "operator");
+  }
+  if (rc == AMT_OK) rc = copy_out(p, y, p->q, sizeof(double) * p->n_dof);
+  (void)hipFree(xf);
+  (void)hipFree(yf);
+  return rc;
 }
 
 int amt_get_diagonal(amt_problem* p, double* diag, int with_bc, double* penalty) {

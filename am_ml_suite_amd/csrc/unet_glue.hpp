@@ -228,4 +228,9 @@ __global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, int64
   if (i < n) b[i] = (float)a[i];
 }
 
+__global__ void k_f2d(const float* __restrict__ a, double* __restrict__ b, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
+}
+
 }  // namespace amt

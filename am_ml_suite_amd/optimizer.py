@@ -60,7 +60,7 @@ class SIMPConfig:
     solver_max_iterations: int = 5000
     penalty_factor: float = 1e8
     density_index: str = "reference"  # or "consistent"
-    precision: str = "fp64"           # or "mixed_fp32"
+    precision: str = "fp64"           # or "mixed_fp32", "mixed_bf16" (operator on MFMA)
     preconditioner: str = "jacobi"    # or "multigrid" (FP32 V-cycle inside FP64 flexible CG)
     warm_start: bool = False
     inner_tolerance: float = 1e-4
@@ -130,7 +130,8 @@ def make_amt_config(nx, ny, nz, element_size, material, config):
     c.density_index = {"reference": _ffi.AMT_INDEX_REF_COMPAT,
                        "consistent": _ffi.AMT_INDEX_CONSISTENT}[config.density_index]
     c.precision = {"fp64": _ffi.AMT_PREC_FP64,
-                   "mixed_fp32": _ffi.AMT_PREC_MIXED_FP32}[config.precision]
+                   "mixed_fp32": _ffi.AMT_PREC_MIXED_FP32,
+                   "mixed_bf16": _ffi.AMT_PREC_MIXED_BF16}[config.precision]
     c.precond = {"jacobi": _ffi.AMT_PRECOND_JACOBI, "unet": _ffi.AMT_PRECOND_UNET,
                  "multigrid": _ffi.AMT_PRECOND_MULTIGRID}[config.preconditioner]
     c.warm_start = int(bool(config.warm_start))
@@ -265,10 +266,14 @@ class TopologyOptimizer:
             iteration_results=res)
 
     # -- phase entry points --
-    def apply_operator(self, x, with_bc=True):
+    def apply_operator(self, x, with_bc=True, precision="fp64"):
+        """y = K x (penalty rows with with_bc) with the FP64 vector operator, or the FP32
+        vector ("mixed_fp32") or split-BF16 MFMA ("mixed_bf16") one."""
         x = np.ascontiguousarray(x, dtype=np.float64)
-        y = np.zeros(self.n_dofs)
-        self._check(self._L.amt_apply_operator(self._h, x, y, int(bool(with_bc))))
+        y = np.zeros_like(x)
+        prec = {"fp64": _ffi.AMT_PREC_FP64, "mixed_fp32": _ffi.AMT_PREC_MIXED_FP32,
+                "mixed_bf16": _ffi.AMT_PREC_MIXED_BF16}[precision]
+        self._check(self._L.amt_apply_operator_prec(self._h, x, y, int(bool(with_bc)), prec))
         return y
 
     def diagonal(self, with_bc=True):

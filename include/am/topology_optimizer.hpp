@@ -49,7 +49,7 @@ using Scalar = double;
 using Vector = std::vector<double>;
 
 enum class DensityIndex { Consistent = 0, Reference = 1 };
-enum class Precision { FP64 = 0, MixedFP32 = 1 };
+enum class Precision { FP64 = 0, MixedFP32 = 1, MixedBF16 = 2 };
 /// Jacobi: the reference's CG with Jacobi in place of IncompleteCholesky. Multigrid: FP64
 /// flexible CG around an FP32 Galerkin geometric-multigrid V-cycle.
 enum class Preconditioner { Jacobi = 0, Multigrid = 2 };

@@ -36,10 +36,14 @@ extern "C" {
 #define AMT_ENOTSUP 5  /* option not available in this build */
 
 enum { AMT_INDEX_CONSISTENT = 0, AMT_INDEX_REF_COMPAT = 1 };
-enum { AMT_PREC_FP64 = 0, AMT_PREC_MIXED_FP32 = 1 };
+/* Precision of the FP32 work. MIXED_FP32: FP32 vector operator. MIXED_BF16: the operator on
+ * the matrix cores (element-batched Ke0 x_e with x and Ke0 split into two BF16 terms each,
+ * FP32 accumulation, ~1e-5 relative). Results are always held to the FP64 tolerance. */
+enum { AMT_PREC_FP64 = 0, AMT_PREC_MIXED_FP32 = 1, AMT_PREC_MIXED_BF16 = 2 };
 /* Preconditioners. JACOBI: the reference's solve with Jacobi in place of IC (FP64, or FP32
- * inner solves in FP64 refinement with AMT_PREC_MIXED_FP32). MULTIGRID: flexible CG with FP64
- * vectors, operator and reductions around an FP32 Galerkin geometric-multigrid V-cycle. */
+ * inner solves in FP64 refinement with a MIXED precision). MULTIGRID: flexible CG with FP64
+ * vectors, operator and reductions around an FP32 Galerkin geometric-multigrid V-cycle (its
+ * fine-level smoothing on the matrix cores with AMT_PREC_MIXED_BF16). */
 enum { AMT_PRECOND_JACOBI = 0, AMT_PRECOND_UNET = 1, AMT_PRECOND_MULTIGRID = 2 };
 
 typedef struct amt_problem amt_problem;
@@ -124,6 +128,10 @@ int amt_run(amt_problem* p, int max_it, amt_iteration_result* hist, int* iterati
 /* y = K x with the current density (global_assembler.cpp:61-222 + SpMV). with_bc=1 adds
  * the penalty rows (linear_solver.cpp:35-91) exactly as the solve sees them. */
 int amt_apply_operator(amt_problem* p, const double* x, double* y, int with_bc);
+/* As amt_apply_operator with the operator of a given precision (AMT_PREC_*): FP64 vector,
+ * FP32 vector, or split-BF16 MFMA; x and y are FP64 on the host either way. */
+int amt_apply_operator_prec(amt_problem* p, const double* x, double* y, int with_bc,
+                            int precision);
 /* Diagonal of K (+ penalty if with_bc) and P = penalty_factor * max|K_ii|. */
 int amt_get_diagonal(amt_problem* p, double* diag, int with_bc, double* penalty);
 
