@@ -67,29 +67,26 @@ __device__ __forceinline__ void load_kefrag(KeFrag& f, const uint16_t* __restric
       }
 }
 
-// Halo node values f(dof) (0 outside the grid) split into BF16 hi/lo, and element scales.
+// Halo node values f(dof) (0 outside the grid) split into BF16 hi/lo, and element scales
+// (all global loads issued first, as tile_load).
 template <typename F>
 __device__ __forceinline__ void mfma_tile_load(MfmaSmem& sm, const GridDims& g, int i0, int j0,
                                                int k0, F f, const float* __restrict__ s) {
-  for (int q = threadIdx.x; q < kHalo * 3; q += kBlock) {
-    const int c = q % 3;
-    const int node = q / 3;
-    const int hk = node % kHK, hj = (node / kHK) % kHJ, hi = node / (kHK * kHJ);
-    const int i = i0 + hi - 1, j = j0 + hj - 1, k = k0 + hk - 1;
-    float v = 0.f;
-    if (i >= 0 && i <= g.nx && j >= 0 && j <= g.ny && k >= 0 && k <= g.nz)
-      v = f(3 * (((int64_t)i * g.nny + j) * g.nnz + k) + c);
-    const uint16_t h = mf_bf16(v);
-    sm.xh[q] = h;
-    sm.xl[q] = mf_bf16(v - mf_f32(h));
+  float v[kXPer], sv[kSPer];
+  tile_gather(v, sv, g, i0, j0, k0, f, s);
+#pragma unroll
+  for (int t = 0; t < kXPer; ++t) {
+    const int q = threadIdx.x + t * kBlock;
+    if (q < kHalo * 3) {
+      const uint16_t h = mf_bf16(v[t]);
+      sm.xh[q] = h;
+      sm.xl[q] = mf_bf16(v[t] - mf_f32(h));
+    }
   }
-  for (int q = threadIdx.x; q < kETile; q += kBlock) {
-    const int ek = q % kEK, ej = (q / kEK) % kEJ, ei = q / (kEK * kEJ);
-    const int i = i0 + ei - 1, j = j0 + ej - 1, k = k0 + ek - 1;
-    float v = 0.f;
-    if (i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz)
-      v = s[((int64_t)i * g.ny + j) * g.nz + k];
-    sm.s[q] = v;
+#pragma unroll
+  for (int t = 0; t < kSPer; ++t) {
+    const int q = threadIdx.x + t * kBlock;
+    if (q < kETile) sm.s[q] = sv[t];
   }
 }
 

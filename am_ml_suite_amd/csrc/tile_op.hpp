@@ -61,26 +61,52 @@ __device__ __forceinline__ bool tile_origin(const TileGrid& tg, int& i0, int& j0
 }
 
 // Cooperative load: halo node values f(dof) (0 outside the grid) and element scales.
+// Fully unrolled, with every global load of the thread issued before the first LDS store, so
+// the block pays the memory latency once instead of once per loop trip (the loads are
+// independent; a rolled loop serialises them behind its stores).
+constexpr int kXPer = (kHalo * 3 + kBlock - 1) / kBlock;  // halo values per thread (8)
+constexpr int kSPer = (kETile + kBlock - 1) / kBlock;     // element scales per thread (2)
+
 template <typename T, typename F>
-__device__ __forceinline__ void tile_load(TileSmem<T>& sm, const GridDims& g, int i0, int j0,
-                                          int k0, F f, const T* __restrict__ s) {
-  for (int q = threadIdx.x; q < kHalo * 3; q += kBlock) {
+__device__ __forceinline__ void tile_gather(T (&v)[kXPer], T (&sv)[kSPer], const GridDims& g,
+                                            int i0, int j0, int k0, F f,
+                                            const T* __restrict__ s) {
+#pragma unroll
+  for (int t = 0; t < kXPer; ++t) {
+    const int q = threadIdx.x + t * kBlock;
     const int c = q % 3;
     const int node = q / 3;
     const int hk = node % kHK, hj = (node / kHK) % kHJ, hi = node / (kHK * kHJ);
     const int i = i0 + hi - 1, j = j0 + hj - 1, k = k0 + hk - 1;
-    T v = T(0);
-    if (i >= 0 && i <= g.nx && j >= 0 && j <= g.ny && k >= 0 && k <= g.nz)
-      v = f(3 * (((int64_t)i * g.nny + j) * g.nnz + k) + c);
-    sm.x[q] = v;
+    v[t] = T(0);
+    if (q < kHalo * 3 && i >= 0 && i <= g.nx && j >= 0 && j <= g.ny && k >= 0 && k <= g.nz)
+      v[t] = f(3 * (((int64_t)i * g.nny + j) * g.nnz + k) + c);
   }
-  for (int q = threadIdx.x; q < kETile; q += kBlock) {
+#pragma unroll
+  for (int t = 0; t < kSPer; ++t) {
+    const int q = threadIdx.x + t * kBlock;
     const int ek = q % kEK, ej = (q / kEK) % kEJ, ei = q / (kEK * kEJ);
     const int i = i0 + ei - 1, j = j0 + ej - 1, k = k0 + ek - 1;
-    T v = T(0);
-    if (i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz)
-      v = s[((int64_t)i * g.ny + j) * g.nz + k];
-    sm.s[q] = v;
+    sv[t] = T(0);
+    if (q < kETile && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz)
+      sv[t] = s[((int64_t)i * g.ny + j) * g.nz + k];
+  }
+}
+
+template <typename T, typename F>
+__device__ __forceinline__ void tile_load(TileSmem<T>& sm, const GridDims& g, int i0, int j0,
+                                          int k0, F f, const T* __restrict__ s) {
+  T v[kXPer], sv[kSPer];
+  tile_gather(v, sv, g, i0, j0, k0, f, s);
+#pragma unroll
+  for (int t = 0; t < kXPer; ++t) {
+    const int q = threadIdx.x + t * kBlock;
+    if (q < kHalo * 3) sm.x[q] = v[t];
+  }
+#pragma unroll
+  for (int t = 0; t < kSPer; ++t) {
+    const int q = threadIdx.x + t * kBlock;
+    if (q < kETile) sm.s[q] = sv[t];
   }
 }
 

@@ -12,6 +12,10 @@ the same W + K OC iterations:
             FP64 residual (the true residual is recomputed and reported).
 `value` is the fast path's OC iterations/s; the plain path and the speed-up are reported
 beside it, with the fast path's compliance/density error against the plain FP64 path.
+With --paths ...,large the "large" object reports the 256x128x128 bracket (12,830,211 DOFs):
+100 OC iterations with the V-cycle's fine-level smoothing on the matrix cores (split-BF16
+MFMA operator, FP32 accumulation) against the same run with FP32 vector smoothing (FP64
+flexible CG either way), with the density max-abs between them.
 The "batch" object reports the dataset-generation workload: 64 generated problems of sizes
 U{16..64}^3 (13.0M DOFs together) advanced in lockstep by the batched engine (FP32 matvec,
 FP64 vectors and reductions, residual replacement, every solve to the 1e-8 true residual).
@@ -30,6 +34,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per k_fcg_apply launch on this workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+# scaled by a plain 256 MiB copy (profiles/fcg_apply_bracket120x60x60_hbm_traffic.md)
+FCG_APPLY_TRAFFIC_BYTES = 49.82e6
 
 
 def log(msg):
@@ -69,6 +76,28 @@ def run_path(problem, warmup, steps, precision, warm_start, profile, precond="ja
     dens = opt.get_density()
     opt.close()
     return dict(time=dt, results=res, density=dens, prof=prof)
+
+
+def run_large(steps):
+    import am_ml_suite_amd as A
+    p = A.bracket_like_problem(256, 128, 128, seed=5)
+    out = {}
+    for name, prec in (("mfma", "mixed_bf16"), ("fp32_smoother", "fp64")):
+        log("large [%s]: %s, %d DOFs" % (name, p.name, p.n_dofs))
+        out[name] = run_path(p, 1, steps, prec, True, False, "multigrid")
+    m, f = out["mfma"], out["fp32_smoother"]
+    t_m = m["results"][1:]
+    cm = np.array([r.compliance for r in m["results"]])
+    cf = np.array([r.compliance for r in f["results"]])
+    return dict(workload=p.name, n_dofs=p.n_dofs, steps=steps,
+                oc_iter_per_s=steps / m["time"], dof_per_s=p.n_dofs * steps / m["time"],
+                fp32_smoother_oc_iter_per_s=steps / f["time"],
+                density_maxabs_vs_fp64_path=float(np.abs(m["density"] - f["density"]).max()),
+                max_rel_compliance_diff=float(np.abs(cm / cf - 1).max()),
+                mean_fcg_iters=float(np.mean([r.solver_iterations for r in t_m])),
+                max_true_rel_residual=float(max(r.true_rel_residual for r in t_m)),
+                solver="FP64 flexible CG + FP32 V-cycle, fine-level smoothing on MFMA "
+                       "(split-BF16 operator, FP32 accumulation)")
 
 
 def run_batch(warmup, steps):
@@ -133,7 +162,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--paths", default="plain,fast,batch")
+    ap.add_argument("--large-steps", type=int, default=100)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (a plain copy kernel of "
+                         "known size, to scale the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy << 20
+        x = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "copies": 5}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -152,6 +195,9 @@ def main():
         log("fast: FP64 flexible CG + FP32 multigrid V-cycle + warm start")
         out["fast"] = run_path(p, args.warmup, args.steps, "fp64", True, True, "multigrid")
 
+    large = None
+    if "large" in paths:
+        large = run_large(args.large_steps)
     batch = None
     if "batch" in paths:
         log("batch: 64 generated problems in lockstep")
@@ -173,7 +219,8 @@ def main():
         avg_s = ms / launches / 1e3
         achieved = bytes_per_launch / avg_s / 1e9
         roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=achieved / HBM_PEAK_GBS, traffic=None,
+                        frac=achieved / HBM_PEAK_GBS,
+                        traffic=FCG_APPLY_TRAFFIC_BYTES if main_key == "fast" else None,
                         kernel="k_fcg_apply" if main_key == "fast" else "k_pcg_apply<double>",
                         avg_launch_us=avg_s * 1e6, timed_launches=launches,
                         algorithmic_bytes_per_launch=bytes_per_launch,
@@ -216,6 +263,8 @@ def main():
                               if main_key == "fast" else "FP64 Jacobi-PCG")},
         "roofline": roofline, "cpu_baseline": cpu, "batch": batch,
     }
+    if large is not None:
+        line["large"] = large
     line.update(extra)
     print(json.dumps(line), flush=True)
 
