@@ -13,7 +13,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libamt.so")
 
-SOURCES = ["amt.hip", "unet.hip", "batch.hip", "ke0.cpp", "topology_optimizer.cpp"]
+SOURCES = ["amt.hip", "unet.hip", "batch.hip", "ke0.cpp", "topology_optimizer.cpp", "problem.cpp"]
+CLI = os.path.join(PKG, "am_topopt")  # the reference's standalone program on this library
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # Packed FP32 instructions are disabled for all device code.
@@ -69,6 +70,11 @@ def build(verbose=False, force=True):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB)
+    tmp = CLI + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+          os.path.join(CSRC, "am_topopt.cpp"), "-o", tmp, "-L", PKG, "-lamt",
+          "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(tmp, CLI)
     return LIB
 
 

@@ -472,18 +472,15 @@ __global__ __launch_bounds__(kBlock) void k_fcg_apply(
   double acc[1] = {0.0};
   if (has && i <= g.nx && j <= g.ny && k <= g.nz) {
     const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
-    Nbr<double> nb;
-    double s8[8];
-    tile_fetch(sm, li, lj, lk, nb, s8);
-    double yv[3];
-    apply_node(nb, s8, Ke, yv);
+    double yv[3], pv[3];
+    tile_apply(sm, li, lj, lk, Ke, yv, pv);
     const double pen = *P;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      yv[c] = fma((double)fixcnt[3 * n + c] * pen, nb.v[13][c], yv[c]);
-      pnew[3 * n + c] = nb.v[13][c];
+      yv[c] = fma((double)fixcnt[3 * n + c] * pen, pv[c], yv[c]);
+      pnew[3 * n + c] = pv[c];
       q[3 * n + c] = yv[c];
-      acc[0] = fma(nb.v[13][c], yv[c], acc[0]);
+      acc[0] = fma(pv[c], yv[c], acc[0]);
     }
   }
   store_partials<1>(acc, partials, gridDim.x);

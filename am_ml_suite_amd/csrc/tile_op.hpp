@@ -139,4 +139,40 @@ __device__ __forceinline__ void tile_fetch(const TileSmem<T>& sm, int li, int lj
   for (int a = 0; a < 8; ++a) s8[a] = sm.s[eb - (kOX[a] * kEJ + kOY[a]) * kEK - kOZ[a]];
 }
 
+// Element-loop form of tile_fetch + apply_node: y = sum_a s_a Ke0[3a.., :] x_e(a), reading
+// each element's 24 nodal values from LDS as it is used (the empty asm with a memory clobber
+// keeps the compiler from merging the eight elements' loads into one 81-value live set), so
+// a thread holds 24 values instead of 81: more waves per SIMD for more LDS reads. own: the
+// thread's node value.
+template <typename T, typename TK>
+__device__ __forceinline__ void tile_apply(const TileSmem<T>& sm, int li, int lj, int lk,
+                                           const TK* __restrict__ Ke, T (&y)[3], T (&own)[3]) {
+  const int nb0 = ((li + 1) * kHJ + (lj + 1)) * kHK + (lk + 1);
+  const int eb = ((li + 1) * kEJ + (lj + 1)) * kEK + (lk + 1);
+  y[0] = y[1] = y[2] = T(0);
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    asm volatile("" ::: "memory");
+    const T se = sm.s[eb - (kOX[a] * kEJ + kOY[a]) * kEK - kOZ[a]];
+    T t0 = T(0), t1 = T(0), t2 = T(0);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int dn = ((kOX[b] - kOX[a]) * kHJ + (kOY[b] - kOY[a])) * kHK + (kOZ[b] - kOZ[a]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const T xv = sm.x[3 * (nb0 + dn) + d];
+        t0 = fma((T)Ke[(3 * a + 0) * 24 + 3 * b + d], xv, t0);
+        t1 = fma((T)Ke[(3 * a + 1) * 24 + 3 * b + d], xv, t1);
+        t2 = fma((T)Ke[(3 * a + 2) * 24 + 3 * b + d], xv, t2);
+      }
+    }
+    y[0] = fma(se, t0, y[0]);
+    y[1] = fma(se, t1, y[1]);
+    y[2] = fma(se, t2, y[2]);
+  }
+  asm volatile("" ::: "memory");
+#pragma unroll
+  for (int c = 0; c < 3; ++c) own[c] = sm.x[3 * nb0 + c];
+}
+
 }  // namespace amt

@@ -105,3 +105,43 @@ def test_batch_problem_generator():
         assert 0.2 <= p.params["volume_fraction"] <= 0.5
         assert 1.2 <= p.params["filter_radius"] <= 2.5
         assert np.all(p.rho0 == p.params["volume_fraction"])
+
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CLI = os.path.join(os.path.dirname(HERE), "am_ml_suite_amd", "am_topopt")
+
+
+def test_cli_setup_matches_python_and_reference_counts():
+    """am_topopt's C++ BRK-A-01 set-up (problem.cpp) against the Python restatement and the
+    reference's data/cpp_test_run/metadata.json counts (8151 / 784 / 65 at 4 mm)."""
+    import json
+    import subprocess
+    G = json.load(open(os.path.join(HERE, "golden", "reference_runs.json")))["cpp_test_run"]
+    for res in (4.0, 12.0, 6.0):
+        out = subprocess.run([CLI, "--dump-domain", "--resolution", str(res)],
+                             capture_output=True, text=True, check=True).stdout
+        d = json.loads(out)
+        dom = A.bracket_domain(res)
+        lc = A.bracket_load_case(dom)
+        assert tuple(d["domain_shape"]) == dom.shape
+        assert (d["n_design"], d["n_fixed"], d["n_void"]) == (dom.n_design, dom.n_fixed,
+                                                               dom.n_void)
+        assert d["n_dofs"] == lc.n_dofs and d["n_load_nodes"] == len(lc.forces)
+        assert d["first_load_node"] == lc.forces[0][0]
+        assert d["total_fz"] == pytest.approx(-15000.0)
+        assert d["n_fixed_dofs"] == len(lc.constrained)
+        if res == 4.0:
+            assert [d["n_design"], d["n_fixed"], d["n_void"]] == [G["n_design"], G["n_fixed"],
+                                                                  G["n_void"]]
+            assert d["domain_shape"] == G["domain_shape"]
+
+
+def test_cli_arguments():
+    import subprocess
+    r = subprocess.run([CLI, "--help"], capture_output=True, text=True)
+    assert r.returncode == 1 and "--resolution <mm>" in r.stdout and "--output_dir" in r.stdout
+    r = subprocess.run([CLI, "--bogus"], capture_output=True, text=True)
+    assert r.returncode == 1 and "Unknown option: --bogus" in r.stderr
+    r = subprocess.run([CLI, "--resolution", "abc", "--dump-domain"], capture_output=True,
+                       text=True)
+    assert r.returncode == 1
