@@ -54,7 +54,8 @@ constexpr int kChunk = 32;  // PCG iterations launched between two host polls
 constexpr int kMgChunk = 4;  // flexible-CG iterations per captured graph (even)
 constexpr int kNnPatience = 16;  // learned FCG: chunks allowed without a 10% |r| drop
 constexpr double kNnGainMomentum = 0.05;  // running means behind the correction's weight
-constexpr float kMgOmega = 0.6f;  // damped-Jacobi weight of the V-cycle smoother
+constexpr float kMgOmega = 0.6f;  // damped-Jacobi weight of the V-cycle smoother...
+constexpr float kMgTheta = 1.9f;  // ...capped per row at theta / sum_j |A_ij| (spectrum < 2)
 constexpr int64_t kMgCoarseDofs = 300;  // coarsen until a level has at most this many DOFs
 
 // Partials regions (doubles) inside one buffer; see amt_problem::part()
@@ -292,7 +293,7 @@ int mg_build_hierarchy(amt_problem* p, const double* Ke) {
     amt_problem::MgLevel L;
     L.g = mg_grid((f.ex + 1) / 2, (f.ey + 1) / 2, (f.ez + 1) / 2);
     const int64_t nd = 3 * L.g.n_nodes;
-    AMT_CHECK(dalloc(p, &L.A, 243 * L.g.n_nodes));
+    AMT_CHECK(dalloc(p, &L.A, (size_t)9 * kStencilBlocks * L.g.n_nodes));
     AMT_CHECK(dalloc(p, &L.wd, nd));
     AMT_CHECK(dalloc(p, &L.rhs, nd));
     AMT_CHECK(dalloc(p, &L.x, nd));
@@ -324,7 +325,9 @@ void mg_dense_inverse(const float* st, const MgGrid& g, std::vector<float>& inv)
       const int64_t M = ((int64_t)MI * g.nny + MJ) * g.nnz + MK;
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
-          A[(size_t)(3 * N + i) * n + 3 * M + j] = st[(int64_t)(d * 9 + 3 * i + j) * g.n_nodes + N];
+          A[(size_t)(3 * N + i) * n + 3 * M + j] =
+              d >= 13 ? st[(int64_t)((d - 13) * 9 + 3 * i + j) * g.n_nodes + N]
+                      : st[(int64_t)((13 - d) * 9 + 3 * j + i) * g.n_nodes + M];
     }
   }
   std::vector<double> diag(n);
@@ -374,22 +377,23 @@ void mg_dense_inverse(const float* st, const MgGrid& g, std::vector<float>& inv)
 // Coarse operators for the current element scales and penalty (after k_dinv).
 int mg_setup(amt_problem* p) {
   auto& mg = p->mg;
-  hipLaunchKernelGGL(k_mg_wd0, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->diag,
-                     p->fixcnt, p->scal + S_P, kMgOmega, mg[0].wd, p->n_dof);
-  hipLaunchKernelGGL(k_mg_galerkin1, dim3(nblk(mg[1].g.n_nodes), 27, 3), dim3(kBlock), 0,
+  hipLaunchKernelGGL(k_mg_wd0, dim3(nblk(p->n_n), 3), dim3(kBlock), 0, p->stream, p->s64,
+                     p->Ke64, p->g, p->fixcnt, p->scal + S_P, kMgOmega, kMgTheta, mg[0].wd);
+  hipLaunchKernelGGL(k_mg_galerkin1, dim3(nblk(mg[1].g.n_nodes), kStencilBlocks, 3), dim3(kBlock), 0,
                      p->stream, p->s64, p->g, p->mg_G, p->fixcnt, p->scal + S_P, mg[1].g,
                      mg[1].A);
   for (size_t l = 1; l < mg.size(); ++l) {
     if (l >= 2)
-      hipLaunchKernelGGL(k_mg_coarsen, dim3(nblk(mg[l].g.n_nodes), 27, 3), dim3(kBlock), 0,
+      hipLaunchKernelGGL(k_mg_coarsen, dim3(nblk(mg[l].g.n_nodes), kStencilBlocks, 3), dim3(kBlock), 0,
                          p->stream, mg[l - 1].A, mg[l - 1].g, mg[l].g, mg[l].A);
     hipLaunchKernelGGL(k_mg_stencil_wdinv, dim3(nblk(3 * mg[l].g.n_nodes)), dim3(kBlock), 0,
-                       p->stream, mg[l].A, mg[l].g, kMgOmega, mg[l].wd);
+                       p->stream, mg[l].A, mg[l].g, kMgOmega, kMgTheta, mg[l].wd);
   }
   AMT_CHECK(check_launch(p, "mg_setup"));
   const auto& C = mg.back();
-  p->mg_host_stencil.resize(243 * C.g.n_nodes);
-  AMT_CHECK(copy_out(p, p->mg_host_stencil.data(), C.A, sizeof(float) * 243 * C.g.n_nodes));
+  p->mg_host_stencil.resize((size_t)9 * kStencilBlocks * C.g.n_nodes);
+  AMT_CHECK(copy_out(p, p->mg_host_stencil.data(), C.A,
+                     sizeof(float) * 9 * kStencilBlocks * C.g.n_nodes));
   std::vector<float> inv;
   mg_dense_inverse(p->mg_host_stencil.data(), C.g, inv);
   return copy_in(p, p->mg_ainv, inv.data(), inv.size() * sizeof(float));

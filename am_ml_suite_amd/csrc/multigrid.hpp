@@ -4,8 +4,11 @@
 // Hierarchy: level 0 is the fine voxel grid, applied matrix-free (s_e * Ke0). Level l+1 has
 // ceil(e/2) elements per axis of level l; the prolongation P is trilinear (1, 1/2 weights per
 // axis; a last odd fine node interpolates from a coarse node one layer past the fine grid).
-// Coarse operators are Galerkin, A_{l+1} = P^T A_l P, stored as 27-point block stencils in
-// SoA layout A[(d*9 + 3i + j) * n_nodes + node] (d = neighbour offset, i/j = components):
+// Coarse operators are Galerkin, A_{l+1} = P^T A_l P, 27-point block stencils. They are
+// symmetric, A(N, N+d) = A(N+d, N)^T, so only the diagonal block and the 13 forward offsets
+// d = 13..26 are stored, in SoA layout A[((d-13)*9 + 3i + j) * n_nodes + node] (i/j =
+// components); a backward block is read transposed from the neighbour (stencil_coef). This
+// halves the coarse levels' traffic and makes the stored operator exactly symmetric:
 //  - level 1 straight from the element densities: a coarse element E holds 8 fine children c,
 //    and P restricted to E is the fixed interpolation Q_c, so P^T A_0 P over E equals
 //    sum_c s_c G_c with G_c = Q_c^T Ke0 Q_c (eight 24x24 tables built on the host); the
@@ -41,11 +44,21 @@ __device__ __forceinline__ float pw1(int f, int c) {
   return d == 0 ? 1.0f : ((d == 1 || d == -1) ? 0.5f : 0.0f);
 }
 
+constexpr int kStencilBlocks = 14;  // stored 3x3 blocks per node: d = 13..26
+
+// Block (r, c) of A(n, n + d); the neighbour n + d must exist. sj, si: node strides along j, i.
+__device__ __forceinline__ float stencil_coef(const float* __restrict__ A, int64_t nn, int64_t n,
+                                              int d, int r, int c, int64_t sj, int64_t si) {
+  if (d >= 13) return A[(int64_t)((d - 13) * 9 + 3 * r + c) * nn + n];
+  const int64_t m = n + (d / 9 - 1) * si + ((d / 3) % 3 - 1) * sj + (d % 3 - 1);
+  return A[(int64_t)((13 - d) * 9 + 3 * c + r) * nn + m];
+}
+
 // ---------------------------------------------------------------------------------------
 // Coarse operator construction (once per OC iteration)
 // ---------------------------------------------------------------------------------------
 
-// Level 1, thread (coarse node N, row component i = blockIdx.z, offset D = blockIdx.y).
+// Level 1, thread (coarse node N, row component i = blockIdx.z, offset D = 13 + blockIdx.y).
 // G: 8 x 576 doubles (child c, row 3a+i, col 3b+j). s: fine element scales, C order.
 __global__ __launch_bounds__(kBlock) void k_mg_galerkin1(const double* __restrict__ s,
                                                          GridDims fg,
@@ -55,8 +68,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_galerkin1(const double* __restric
                                                          MgGrid cg, float* __restrict__ A) {
   const int64_t N = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (N >= cg.n_nodes) return;
-  const int dI = (int)blockIdx.y / 9 - 1, dJ = ((int)blockIdx.y / 3) % 3 - 1,
-            dK = (int)blockIdx.y % 3 - 1;
+  const int D = 13 + (int)blockIdx.y;
+  const int dI = D / 9 - 1, dJ = (D / 3) % 3 - 1, dK = D % 3 - 1;
   const int i = blockIdx.z;
   int I, J, K;
   mg_decode(N, cg, I, J, K);
@@ -102,18 +115,19 @@ __global__ __launch_bounds__(kBlock) void k_mg_galerkin1(const double* __restric
       }
     }
   }
-  const int d = blockIdx.y;
+  const int d = blockIdx.y;  // stored block index D - 13
 #pragma unroll
   for (int j = 0; j < 3; ++j) A[(int64_t)(d * 9 + 3 * i + j) * cg.n_nodes + N] = (float)acc[j];
 }
 
-// Level l+1 from level l: thread (coarse N, offset D = blockIdx.y, row i = blockIdx.z).
+// Level l+1 from level l: thread (coarse N, offset D = 13 + blockIdx.y, row i = blockIdx.z).
 __global__ __launch_bounds__(kBlock) void k_mg_coarsen(const float* __restrict__ Af, MgGrid fg,
                                                        MgGrid cg, float* __restrict__ Ac) {
   const int64_t N = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (N >= cg.n_nodes) return;
-  const int dI = (int)blockIdx.y / 9 - 1, dJ = ((int)blockIdx.y / 3) % 3 - 1,
-            dK = (int)blockIdx.y % 3 - 1;
+  const int D = 13 + (int)blockIdx.y;
+  const int dI = D / 9 - 1, dJ = (D / 3) % 3 - 1, dK = D % 3 - 1;
+  const int64_t fsj = fg.nnz, fsi = (int64_t)fg.nny * fg.nnz;
   const int i = blockIdx.z;
   int I, J, K;
   mg_decode(N, cg, I, J, K);
@@ -142,10 +156,9 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(const float* __restrict__
                 if (wz == 0.f || mz + ddz < 0 || mz + ddz >= fg.nnz) continue;
                 const int d = (ddx + 1) * 9 + (ddy + 1) * 3 + (ddz + 1);
                 const float w = wN * wx * wy * wz;
-                const float* row = Af + (int64_t)(d * 9 + 3 * i) * fg.n_nodes + m;
-                acc[0] = fmaf(w, row[0], acc[0]);
-                acc[1] = fmaf(w, row[fg.n_nodes], acc[1]);
-                acc[2] = fmaf(w, row[2 * fg.n_nodes], acc[2]);
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                  acc[j] = fmaf(w, stencil_coef(Af, fg.n_nodes, m, d, i, j, fsj, fsi), acc[j]);
               }
             }
           }
@@ -153,49 +166,88 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(const float* __restrict__
       }
     }
   }
-  const int d = blockIdx.y;
+  const int d = blockIdx.y;  // stored block index D - 13
 #pragma unroll
   for (int j = 0; j < 3; ++j) Ac[(int64_t)(d * 9 + 3 * i + j) * cg.n_nodes + N] = acc[j];
 }
 
-// Level-0 damped-Jacobi weights: omega / (K_ii + cnt_i P); 0 on rows no element touches.
-__global__ void k_mg_wd0(const double* __restrict__ diag, const uint8_t* __restrict__ fixcnt,
-                         const double* __restrict__ P, float omega, float* __restrict__ wd,
-                         int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  double d = diag[i];
-  for (int t = 0; t < fixcnt[i]; ++t) d += *P;
-  wd[i] = d > 0.0 ? (float)(omega / d) : 0.f;
+// Smoother weights. Damped Jacobi w_i = omega / A_ii diverges once omega * lambda_max(D^-1 A)
+// exceeds 2: for hex8 elasticity lambda_max(D^-1 K) is ~3.1 on a uniform grid but ~4.8 on a
+// high-contrast solid/void design (nu = 0.3), so omega = 0.6 can amplify some modes and turn the
+// V-cycle into a poor preconditioner. Each row therefore gets
+//   w_i = min(omega / A_ii, theta / sum_j |A_ij|),
+// which by Gershgorin bounds the spectrum of W A by theta < 2 (a convergent, symmetric
+// smoother) while rows with a tame neighbourhood keep close to omega. 0 on empty rows.
+__device__ __forceinline__ float smoother_weight(double diag, double rowabs, float omega,
+                                                 float theta) {
+  if (!(diag > 0.0)) return 0.f;
+  const double w = fmin((double)omega / diag, (double)theta / rowabs);
+  return (float)w;
 }
 
-// Damped-Jacobi weights of a stencil level: wd = omega / A_ii (0 on empty rows).
+// Level 0, thread (node, row r = blockIdx.y): the assembled row of K (the 27 neighbour blocks
+// summed from the node's 8 elements, plus the penalty on the diagonal) gives A_rr and
+// sum_j |A_rj|.
+__global__ __launch_bounds__(kBlock) void k_mg_wd0(const double* __restrict__ s,
+                                                   const double* __restrict__ Ke, GridDims g,
+                                                   const uint8_t* __restrict__ fixcnt,
+                                                   const double* __restrict__ P, float omega,
+                                                   float theta, float* __restrict__ wd) {
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  const int r = blockIdx.y;
+  int i, j, k;
+  node_decode(n, g, i, j, k);
+  double s8[8];
+  load_element_scales(s, g, i, j, k, s8);
+  // neighbour block d = (di, dj, dk) gathers the element pairs (a, b) with b - a = d
+  double rowabs = 0.0, diag = 0.0;
+  for (int d = 0; d < 27; ++d) {
+    const int di = d / 9 - 1, dj = (d / 3) % 3 - 1, dk = d % 3 - 1;
+    double blk[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int bx = kOX[a] + di, by = kOY[a] + dj, bz = kOZ[a] + dk;
+      if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+      const int b = 4 * bz + 2 * by + (bx ^ by);  // corner at offsets (bx, by, bz)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) blk[c] = fma(s8[a], Ke[(3 * a + r) * 24 + 3 * b + c], blk[c]);
+    }
+    if (d == 13) {
+      blk[r] += (double)fixcnt[3 * n + r] * (*P);
+      diag = blk[r];
+    }
+    rowabs += fabs(blk[0]) + fabs(blk[1]) + fabs(blk[2]);
+  }
+  wd[3 * n + r] = smoother_weight(diag, rowabs, omega, theta);
+}
+
+// Stencil level, thread per (node, row): the same weights from the stored stencil.
 __global__ void k_mg_stencil_wdinv(const float* __restrict__ A, MgGrid g, float omega,
-                                   float* __restrict__ wd) {
+                                   float theta, float* __restrict__ wd) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= 3 * g.n_nodes) return;
   const int64_t n = t / 3;
-  const int i = (int)(t % 3);
-  const float d = A[(int64_t)(13 * 9 + 4 * i) * g.n_nodes + n];
-  wd[t] = d > 0.f ? omega / d : 0.f;
+  const int r = (int)(t % 3);
+  int i, j, k;
+  mg_decode(n, g, i, j, k);
+  const int64_t sj = g.nnz, si = (int64_t)g.nny * g.nnz;
+  double rowabs = 0.0;
+#pragma unroll
+  for (int d = 0; d < 27; ++d) {
+    const int di = d / 9 - 1, dj = (d / 3) % 3 - 1, dk = d % 3 - 1;
+    if ((i + di) < 0 || (i + di) >= g.nnx || (j + dj) < 0 || (j + dj) >= g.nny || (k + dk) < 0 ||
+        (k + dk) >= g.nnz)
+      continue;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rowabs += fabs((double)stencil_coef(A, g.n_nodes, n, d, r, c, sj, si));
+  }
+  wd[t] = smoother_weight((double)A[(int64_t)(4 * r) * g.n_nodes + n], rowabs, omega, theta);
 }
 
 // ---------------------------------------------------------------------------------------
 // V-cycle kernels. Every kernel returns at once when the outer solve is done.
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ void stencil_row_apply(const float* __restrict__ A,
-                                                  const MgGrid& g, int64_t n,
-                                                  const float (&nb)[27][3], float (&y)[3]) {
-  y[0] = y[1] = y[2] = 0.f;
-#pragma unroll
-  for (int d = 0; d < 27; ++d)
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        y[i] = fmaf(A[(int64_t)(d * 9 + 3 * i + j) * g.n_nodes + n], nb[d][j], y[i]);
-}
-
 // Loads f(q) for the 27 neighbours of node n (0 outside the grid); f maps a DOF to a value.
 template <typename F>
 __device__ __forceinline__ void mg_gather(const MgGrid& g, int64_t n, int i, int j, int k, F f,
@@ -216,15 +268,22 @@ __device__ __forceinline__ void mg_gather(const MgGrid& g, int64_t n, int i, int
       }
 }
 
-// Row i of the stencil product at node n: sum_d sum_j A[d][i][j] nb[d][j].
+// Row r of the stencil product at node n = (i, j, k): sum_d sum_c A(n, n+d)[r][c] nb[d][c]
+// over the neighbours inside the grid.
 __device__ __forceinline__ float stencil_row(const float* __restrict__ A, const MgGrid& g,
-                                             int64_t n, int i, const float (&nb)[27][3]) {
+                                             int64_t n, int i, int j, int k, int r,
+                                             const float (&nb)[27][3]) {
+  const int64_t sj = g.nnz, si = (int64_t)g.nny * g.nnz;
   float y = 0.f;
 #pragma unroll
-  for (int d = 0; d < 27; ++d)
+  for (int d = 0; d < 27; ++d) {
+    const int di = d / 9 - 1, dj = (d / 3) % 3 - 1, dk = d % 3 - 1;
+    const bool ok = (i + di) >= 0 && (i + di) < g.nnx && (j + dj) >= 0 && (j + dj) < g.nny &&
+                    (k + dk) >= 0 && (k + dk) < g.nnz;
+    if (!ok) continue;
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
-      y = fmaf(A[(int64_t)(d * 9 + 3 * i + j) * g.n_nodes + n], nb[d][j], y);
+    for (int c = 0; c < 3; ++c) y = fmaf(stencil_coef(A, g.n_nodes, n, d, r, c, sj, si), nb[d][c], y);
+  }
   return y;
 }
 
@@ -243,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_pre(const float* __restrict__ A, 
   mg_decode(n, g, i, j, k);
   float nb[27][3];
   mg_gather(g, n, i, j, k, [&](int64_t q) { return wd[q] * rhs[q]; }, nb);
-  const float y = stencil_row(A, g, n, r, nb);
+  const float y = stencil_row(A, g, n, i, j, k, r, nb);
   const int64_t q = 3 * n + r;
   x0[q] = wd[q] * rhs[q];  // == nb[13][r]; read directly to keep nb in registers
   t[q] = rhs[q] - y;
@@ -263,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_post(const float* __restrict__ A,
   mg_decode(n, g, i, j, k);
   float nb[27][3];
   mg_gather(g, n, i, j, k, [&](int64_t q) { return x[q]; }, nb);
-  const float y = stencil_row(A, g, n, r, nb);
+  const float y = stencil_row(A, g, n, i, j, k, r, nb);
   const int64_t q = 3 * n + r;
   out[q] = x[q] + wd[q] * (rhs[q] - y);
 }
