@@ -47,7 +47,7 @@ using namespace amt;
 namespace {
 
 constexpr int kOcLevels = 6;  // bisection levels evaluated per OC sweep launch
-constexpr int kOcBlocks = 256;  // OC sweep grid (partials per candidate)
+constexpr int kOcBlocks = 512;  // OC sweep grid (partials per candidate)
 constexpr int kOcCand = (1 << kOcLevels) - 1;
 constexpr int kMaxReduceBlocks = 1024;
 constexpr int kChunk = 32;  // PCG iterations launched between two host polls
@@ -421,8 +421,8 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q) {
     hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[l + 1].g.n_nodes)), dim3(kBlock), 0,
                        p->stream, mg[l].t, mg[l].g, mg[l + 1].rhs, mg[l + 1].g, done);
   }
-  hipLaunchKernelGGL(k_mg_dense, dim3((p->mg_nc + kWaves - 1) / kWaves), dim3(kBlock), 0,
-                     p->stream, p->mg_ainv, p->mg_nc, mg[L - 1].rhs, mg[L - 1].x, done);
+  hipLaunchKernelGGL(k_mg_dense, dim3((p->mg_nc + 8 * kWaves - 1) / (8 * kWaves)), dim3(kBlock),
+                     0, p->stream, p->mg_ainv, p->mg_nc, mg[L - 1].rhs, mg[L - 1].x, done);
   const float* res = mg[L - 1].x;  // correction of the level below the current one
   for (size_t l = L - 2; l >= 1; --l) {
     hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,

@@ -102,22 +102,88 @@ __device__ __forceinline__ void store_partials(const double (&v)[NV], double* pa
   }
 }
 
+// Block totals of many values (NV <= 64) at once. A wave reduce-scatter by recursive
+// halving: at distance d each lane keeps d of its 2d running sums and adds the partner's copy
+// of the same d, so after 32 + 16 + ... + 1 = 63 shuffles lane l holds the wave total of
+// value l (instead of 6 shuffles per value); the waves' totals are then summed in wave order
+// into tot[0..NV). All threads of the block call it; tot is valid after it returns.
+template <int NV>
+__device__ __forceinline__ void block_reduce_many(const double (&v)[NV], double* tot) {
+  static_assert(NV <= 64, "block_reduce_many holds one value per lane");
+  __shared__ double red[kWaves][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double a[64];
+#pragma unroll
+  for (int i = 0; i < 64; ++i) a[i] = i < NV ? v[i] : 0.0;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const bool up = lane & d;
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+      const double send = up ? a[i] : a[d + i];
+      const double keep = up ? a[d + i] : a[i];
+      a[i] = keep + __shfl_xor(send, d, 64);
+    }
+  }
+  red[w][lane] = a[0];
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double t = red[0][threadIdx.x];
+#pragma unroll
+    for (int k = 1; k < kWaves; ++k) t += red[k][threadIdx.x];
+    tot[threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
+// partials[q*stride + blockIdx.x] = block total of v[q], for many values (NV <= 64).
+template <int NV>
+__device__ __forceinline__ void store_partials_many(const double (&v)[NV], double* partials,
+                                                    int stride) {
+  __shared__ double tot[NV];
+  block_reduce_many<NV>(v, tot);
+  if (threadIdx.x < NV) partials[(size_t)threadIdx.x * stride + blockIdx.x] = tot[threadIdx.x];
+}
+
 // out[q] = fixed-order reduction of partials[q*stride + 0..nb). All threads of the block
-// call it and all receive the result.
+// call it and all receive the result. Thread t folds partials t, t + kBlock, ... in ascending
+// order; the loads of U strides and all NV values are issued before the first add, so a
+// reduction over a few thousand partials waits on memory once or twice instead of once per
+// value.
 template <int NV, bool IS_MAX = false>
 __device__ __forceinline__ void reduce_partials(const double* partials, int nb, int stride,
                                                 double (&out)[NV]) {
+  constexpr int U = NV == 1 ? 8 : (NV <= 5 ? 4 : 1);
   double v[NV];
 #pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    double t = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kBlock) {
-      const double x = partials[(size_t)q * stride + b];
-      t = IS_MAX ? fmax(t, x) : t + x;
-    }
-    v[q] = t;
+  for (int q = 0; q < NV; ++q) v[q] = 0.0;
+  int b = threadIdx.x;
+  for (; b + (U - 1) * kBlock < nb; b += U * kBlock) {
+    double x[U][NV];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < NV; ++q) x[u][q] = partials[(size_t)q * stride + b + u * kBlock];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < NV; ++q) v[q] = IS_MAX ? fmax(v[q], x[u][q]) : v[q] + x[u][q];
   }
-  block_reduce<NV, IS_MAX>(v, out);
+  for (; b < nb; b += kBlock) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const double x = partials[(size_t)q * stride + b];
+      v[q] = IS_MAX ? fmax(v[q], x) : v[q] + x;
+    }
+  }
+  if constexpr (NV > 8 && !IS_MAX) {
+    __shared__ double tot[NV];
+    block_reduce_many<NV>(v, tot);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] = tot[q];
+  } else {
+    block_reduce<NV, IS_MAX>(v, out);
+  }
 }
 
 template <int NV, bool IS_MAX = false>

@@ -445,7 +445,7 @@ __global__ __launch_bounds__(kBlock) void k_oc_sweep(const double* __restrict__ 
     for (int c = 0; c < NC; ++c)
       vol[c] += std_max(rho_min, std_max(lo, std_min(1.0, std_min(hi, rs * gfac[c]))));
   }
-  store_partials<NC>(vol, partials, stride);
+  store_partials_many<NC>(vol, partials, stride);
 }
 
 // One block: reduce the candidate volumes and walk the bisection tree with the reference's

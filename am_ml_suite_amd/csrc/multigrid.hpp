@@ -262,42 +262,48 @@ __device__ __forceinline__ void mg_gather(const MgGrid& g, int64_t n, int i, int
         const int m = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
         const bool ok = (i + di) >= 0 && (i + di) < g.nnx && (j + dj) >= 0 &&
                         (j + dj) < g.nny && (k + dk) >= 0 && (k + dk) < g.nnz;
-        const int64_t q = 3 * (n + di * si + dj * sj + dk);
+        // out-of-grid neighbours load their own node (always valid) and are zeroed after the
+        // load, so no load sits behind a branch and all 81 can be in flight together
+        const int64_t q = 3 * (ok ? n + di * si + dj * sj + dk : n);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) nb[m][c] = ok ? f(q + c) : 0.f;
+        for (int c = 0; c < 3; ++c) {
+          const float v = f(q + c);
+          nb[m][c] = ok ? v : 0.f;
+        }
       }
 }
 
-// Row r of the stencil product at node n = (i, j, k): sum_d sum_c A(n, n+d)[r][c] nb[d][c]
-// over the neighbours inside the grid.
+// Row r of the stencil product at node n = (i, j, k): sum_d sum_c A(n, n+d)[r][c] nb[d][c],
+// nb zero outside the grid (mg_gather). A backward block whose neighbour is outside the grid
+// is read at n instead (finite, times nb = 0): every load is unconditional.
 __device__ __forceinline__ float stencil_row(const float* __restrict__ A, const MgGrid& g,
                                              int64_t n, int i, int j, int k, int r,
                                              const float (&nb)[27][3]) {
-  const int64_t sj = g.nnz, si = (int64_t)g.nny * g.nnz;
+  const int64_t sj = g.nnz, si = (int64_t)g.nny * g.nnz, nn = g.n_nodes;
   float y = 0.f;
 #pragma unroll
   for (int d = 0; d < 27; ++d) {
     const int di = d / 9 - 1, dj = (d / 3) % 3 - 1, dk = d % 3 - 1;
-    const bool ok = (i + di) >= 0 && (i + di) < g.nnx && (j + dj) >= 0 && (j + dj) < g.nny &&
-                    (k + dk) >= 0 && (k + dk) < g.nnz;
-    if (!ok) continue;
+    if (d >= 13) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) y = fmaf(stencil_coef(A, g.n_nodes, n, d, r, c, sj, si), nb[d][c], y);
+      for (int c = 0; c < 3; ++c) y = fmaf(A[(int64_t)((d - 13) * 9 + 3 * r + c) * nn + n], nb[d][c], y);
+    } else {
+      const bool ok = (i + di) >= 0 && (i + di) < g.nnx && (j + dj) >= 0 &&
+                      (j + dj) < g.nny && (k + dk) >= 0 && (k + dk) < g.nnz;
+      const int64_t m = ok ? n + di * si + dj * sj + dk : n;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) y = fmaf(A[(int64_t)((13 - d) * 9 + 3 * c + r) * nn + m], nb[d][c], y);
+    }
   }
   return y;
 }
 
-// Stencil level, thread (node, row i = blockIdx.y): x0 = wd*rhs (pre-smoothing from zero);
-// t = rhs - A x0. Three threads per node keep the small coarse grids busy.
-__global__ __launch_bounds__(kBlock) void k_mg_pre(const float* __restrict__ A, MgGrid g,
-                                                   const float* __restrict__ wd,
-                                                   const float* __restrict__ rhs,
-                                                   float* __restrict__ x0,
-                                                   float* __restrict__ t, const int* done) {
-  if (*done) return;
-  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (n >= g.n_nodes) return;
-  const int r = blockIdx.y;
+// Stencil level, row (node n, component r): x0 = wd*rhs (pre-smoothing from zero);
+// t = rhs - A x0.
+__device__ __forceinline__ void mg_pre_row(const float* __restrict__ A, const MgGrid& g,
+                                           const float* __restrict__ wd,
+                                           const float* __restrict__ rhs, float* __restrict__ x0,
+                                           float* __restrict__ t, int64_t n, int r) {
   int i, j, k;
   mg_decode(n, g, i, j, k);
   float nb[27][3];
@@ -308,16 +314,12 @@ __global__ __launch_bounds__(kBlock) void k_mg_pre(const float* __restrict__ A, 
   t[q] = rhs[q] - y;
 }
 
-// Stencil level, thread (node, row): out = x + wd*(rhs - A x) (post-smoothing).
-__global__ __launch_bounds__(kBlock) void k_mg_post(const float* __restrict__ A, MgGrid g,
-                                                    const float* __restrict__ wd,
-                                                    const float* __restrict__ rhs,
-                                                    const float* __restrict__ x,
-                                                    float* __restrict__ out, const int* done) {
-  if (*done) return;
-  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (n >= g.n_nodes) return;
-  const int r = blockIdx.y;
+// Stencil level, row (node n, component r): out = x + wd*(rhs - A x) (post-smoothing).
+__device__ __forceinline__ void mg_post_row(const float* __restrict__ A, const MgGrid& g,
+                                            const float* __restrict__ wd,
+                                            const float* __restrict__ rhs,
+                                            const float* __restrict__ x, float* __restrict__ out,
+                                            int64_t n, int r) {
   int i, j, k;
   mg_decode(n, g, i, j, k);
   float nb[27][3];
@@ -327,13 +329,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_post(const float* __restrict__ A,
   out[q] = x[q] + wd[q] * (rhs[q] - y);
 }
 
-// Restriction r_c = P^T r_f (coarse node per thread).
-__global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict__ rf,
-                                                        MgGrid fg, float* __restrict__ rc,
-                                                        MgGrid cg, const int* done) {
-  if (*done) return;
-  const int64_t N = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (N >= cg.n_nodes) return;
+// Restriction r_c = P^T r_f at coarse node N.
+__device__ __forceinline__ void mg_restrict_node(const float* __restrict__ rf, const MgGrid& fg,
+                                                 float* __restrict__ rc, const MgGrid& cg,
+                                                 int64_t N) {
   int I, J, K;
   mg_decode(N, cg, I, J, K);
   float acc[3] = {0.f, 0.f, 0.f};
@@ -343,10 +342,13 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict_
     for (int ay = -1; ay <= 1; ++ay)
 #pragma unroll
       for (int az = -1; az <= 1; ++az) {
+        // a fine node outside the grid is read at the clamped position with weight 0, so the
+        // 81 loads carry no branch and issue together
         const int mx = 2 * I + ax, my = 2 * J + ay, mz = 2 * K + az;
-        if (mx < 0 || mx >= fg.nnx || my < 0 || my >= fg.nny || mz < 0 || mz >= fg.nnz) continue;
-        const float w = (ax ? 0.5f : 1.f) * (ay ? 0.5f : 1.f) * (az ? 0.5f : 1.f);
-        const int64_t m = mg_node(fg, mx, my, mz);
+        const bool ok = mx >= 0 && mx < fg.nnx && my >= 0 && my < fg.nny && mz >= 0 && mz < fg.nnz;
+        const float w = ok ? (ax ? 0.5f : 1.f) * (ay ? 0.5f : 1.f) * (az ? 0.5f : 1.f) : 0.f;
+        const int64_t m = mg_node(fg, min(max(mx, 0), fg.nnx - 1), min(max(my, 0), fg.nny - 1),
+                                  min(max(mz, 0), fg.nnz - 1));
 #pragma unroll
         for (int c = 0; c < 3; ++c) acc[c] = fmaf(w, rf[3 * m + c], acc[c]);
       }
@@ -354,18 +356,18 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict_
   for (int c = 0; c < 3; ++c) rc[3 * N + c] = acc[c];
 }
 
-// Prolongation x_f += P x_c (fine node per thread).
-__global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__ xc, MgGrid cg,
-                                                       float* __restrict__ xf, MgGrid fg,
-                                                       const int* done) {
-  if (*done) return;
-  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (n >= fg.n_nodes) return;
+// Prolongation x_f += P x_c at fine node n.
+__device__ __forceinline__ void mg_prolong_node(const float* __restrict__ xc, const MgGrid& cg,
+                                                float* __restrict__ xf, const MgGrid& fg,
+                                                int64_t n) {
   int i, j, k;
   mg_decode(n, fg, i, j, k);
   const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
   const int i1 = (i & 1) ? i0 + 1 : i0, j1 = (j & 1) ? j0 + 1 : j0, k1 = (k & 1) ? k0 + 1 : k0;
-  const float wi = (i & 1) ? 0.5f : 1.f, wj = (j & 1) ? 0.5f : 1.f, wk = (k & 1) ? 0.5f : 1.f;
+  // an even coordinate has one parent: its second parent repeats the first with weight 0
+  const float wi[2] = {(i & 1) ? 0.5f : 1.f, (i & 1) ? 0.5f : 0.f};
+  const float wj[2] = {(j & 1) ? 0.5f : 1.f, (j & 1) ? 0.5f : 0.f};
+  const float wk[2] = {(k & 1) ? 0.5f : 1.f, (k & 1) ? 0.5f : 0.f};
   float acc[3] = {0.f, 0.f, 0.f};
   const int ci[2] = {i0, i1}, cj[2] = {j0, j1}, ck[2] = {k0, k1};
 #pragma unroll
@@ -374,9 +376,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        if ((a && !(i & 1)) || (b && !(j & 1)) || (c && !(k & 1))) continue;
         const int64_t N = mg_node(cg, ci[a], cj[b], ck[c]);
-        const float w = wi * wj * wk;
+        const float w = wi[a] * wj[b] * wk[c];
 #pragma unroll
         for (int q = 0; q < 3; ++q) acc[q] = fmaf(w, xc[3 * N + q], acc[q]);
       }
@@ -384,20 +385,83 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__
   for (int q = 0; q < 3; ++q) xf[3 * n + q] += acc[q];
 }
 
-// Coarsest level: x = Ainv r (dense, n <= 1024); one wave per row, lanes split the columns,
-// four rows per block so the rows' loads are in flight together.
+// Dense rows x[row0 + u*rs] = Ainv[row, :] r, u < 8, for the calling wave: lanes split the
+// columns and the eight rows' loads are in flight together.
+__device__ __forceinline__ void mg_dense_rows8(const float* __restrict__ Ainv, int n,
+                                               const float* __restrict__ r,
+                                               float* __restrict__ x, int row0, int rs) {
+  const int lane = threadIdx.x & 63;
+  float acc[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) acc[u] = 0.f;
+  for (int c = lane; c < n; c += 64) {
+    const float rc = r[c];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int row = min(row0 + u * rs, n - 1);  // rows past n repeat the last, unused
+      acc[u] = fmaf(Ainv[(int64_t)row * n + c], rc, acc[u]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc[u] += __shfl_xor(acc[u], o, 64);
+    if (lane == 0 && row0 + u * rs < n) x[row0 + u * rs] = acc[u];
+  }
+}
+
+// Stencil level, thread (node, row = blockIdx.y). Three threads per node keep the coarse
+// grids busy.
+__global__ __launch_bounds__(kBlock) void k_mg_pre(const float* __restrict__ A, MgGrid g,
+                                                   const float* __restrict__ wd,
+                                                   const float* __restrict__ rhs,
+                                                   float* __restrict__ x0,
+                                                   float* __restrict__ t, const int* done) {
+  if (*done) return;
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  mg_pre_row(A, g, wd, rhs, x0, t, n, blockIdx.y);
+}
+
+__global__ __launch_bounds__(kBlock) void k_mg_post(const float* __restrict__ A, MgGrid g,
+                                                    const float* __restrict__ wd,
+                                                    const float* __restrict__ rhs,
+                                                    const float* __restrict__ x,
+                                                    float* __restrict__ out, const int* done) {
+  if (*done) return;
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  mg_post_row(A, g, wd, rhs, x, out, n, blockIdx.y);
+}
+
+// Restriction (coarse node per thread).
+__global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict__ rf,
+                                                        MgGrid fg, float* __restrict__ rc,
+                                                        MgGrid cg, const int* done) {
+  if (*done) return;
+  const int64_t N = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (N >= cg.n_nodes) return;
+  mg_restrict_node(rf, fg, rc, cg, N);
+}
+
+// Prolongation (fine node per thread).
+__global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__ xc, MgGrid cg,
+                                                       float* __restrict__ xf, MgGrid fg,
+                                                       const int* done) {
+  if (*done) return;
+  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= fg.n_nodes) return;
+  mg_prolong_node(xc, cg, xf, fg, n);
+}
+
+// Coarsest level: x = Ainv r (dense, n <= 1024); eight rows per wave, four waves per block.
 __global__ __launch_bounds__(kBlock) void k_mg_dense(const float* __restrict__ Ainv, int n,
                                                      const float* __restrict__ r,
                                                      float* __restrict__ x, const int* done) {
   if (*done) return;
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * kWaves + (threadIdx.x >> 6);
-  if (row >= n) return;
-  float acc = 0.f;
-  for (int c = lane; c < n; c += 64) acc = fmaf(Ainv[(int64_t)row * n + c], r[c], acc);
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
-  if (lane == 0) x[row] = acc;
+  const int row0 = blockIdx.x * 8 * kWaves + (threadIdx.x >> 6);
+  if (row0 >= n) return;
+  mg_dense_rows8(Ainv, n, r, x, row0, kWaves);
 }
 
 // Level 0 (matrix-free, FP32 s and Ke0, LDS tile): x0 = wd*r, t = r - K x0; r is the FP64
