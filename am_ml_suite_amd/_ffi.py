@@ -29,6 +29,8 @@ EXPORTS = [
     "amt_batch_create", "amt_batch_destroy", "amt_batch_size", "amt_batch_set_loads",
     "amt_batch_set_density", "amt_batch_get_density", "amt_batch_get_displacement",
     "amt_batch_run", "amt_unet_collect", "amt_unet_train",
+    "amt_simp_create", "amt_simp_destroy", "amt_simp_add_load_case", "amt_simp_set_density",
+    "amt_simp_get_density", "amt_simp_step", "amt_simp_filter", "amt_simp_get_displacement",
 ]
 
 
@@ -64,6 +66,23 @@ class AmtSolveStats(C.Structure):
         ("iterations", C.c_int), ("converged", C.c_int), ("outer_iterations", C.c_int),
         ("fallback", C.c_int), ("rel_residual", C.c_double), ("true_rel_residual", C.c_double),
         ("solve_time_s", C.c_double),
+    ]
+
+
+class AmtSimpParams(C.Structure):
+    _fields_ = [
+        ("volume_fraction", C.c_double), ("penalty", C.c_double),
+        ("filter_radius", C.c_double), ("move_limit", C.c_double), ("rho_min", C.c_double),
+        ("convergence_tol", C.c_double), ("E_min", C.c_double), ("max_iterations", C.c_int),
+        ("solve_tol", C.c_double), ("solve_maxit", C.c_int),
+    ]
+
+
+class AmtSimpIteration(C.Structure):
+    _fields_ = [
+        ("compliance", C.c_double), ("volume", C.c_double), ("change", C.c_double),
+        ("solver_iterations", C.c_int), ("max_true_rel_residual", C.c_double),
+        ("solver_converged", C.c_int),
     ]
 
 
@@ -138,6 +157,16 @@ def load():
     L.amt_batch_get_displacement.argtypes = [P, C.c_int, _dp, C.c_int64]
     L.amt_batch_run.argtypes = [P, C.c_int, C.c_void_p, C.POINTER(C.c_double)]
     L.amt_unet_set_gain.argtypes = [P, C.c_double]
+    _i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+    L.amt_simp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                  _i8p, C.POINTER(AmtSimpParams), C.POINTER(P)]
+    L.amt_simp_destroy.argtypes = [P]
+    L.amt_simp_add_load_case.argtypes = [P, _dp, C.c_int64, _i64p, C.c_int64]
+    L.amt_simp_set_density.argtypes = [P, _dp, C.c_int64]
+    L.amt_simp_get_density.argtypes = [P, _dp, C.c_int64]
+    L.amt_simp_step.argtypes = [P, C.POINTER(AmtSimpIteration)]
+    L.amt_simp_filter.argtypes = [P, _dp, _dp, C.c_int64]
+    L.amt_simp_get_displacement.argtypes = [P, C.c_int, _dp, C.c_int64]
     for name in EXPORTS:
         if name not in ("amt_config_default", "amt_last_error"):
             getattr(L, name).restype = C.c_int

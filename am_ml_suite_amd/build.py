@@ -13,7 +13,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libamt.so")
 
-SOURCES = ["amt.hip", "unet.hip", "batch.hip", "ke0.cpp", "topology_optimizer.cpp", "problem.cpp"]
+SOURCES = ["amt.hip", "unet.hip", "batch.hip", "simp_py.hip", "ke0.cpp", "topology_optimizer.cpp", "problem.cpp"]
 CLI = os.path.join(PKG, "am_topopt")  # the reference's standalone program on this library
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -226,6 +226,60 @@ int amt_batch_get_displacement(amt_batch* b, int i, double* u, int64_t n_dof);
  * and the phase times are the batch's. elapsed_s (optional): wall time of the call. */
 int amt_batch_run(amt_batch* b, int iterations, amt_iteration_result* hist, double* elapsed_s);
 
+/* ---- The Python SIMPOptimizer flavour (src/am/numerical/topopt.py:84-411) ----
+ * The suite's second SIMP loop, on the same matrix-free operator. It differs from
+ * TopologyOptimizer in every phase around the solve:
+ *   - design variable rho (nx, ny, nz) C-order; each iteration filters it,
+ *     rho~ = clip(convolve(rho, w, mode='reflect'), rho_min, 1) (:154-156, :302-303), with
+ *     w = cone kernel r - dist on the (2 ceil(r) + 1)^3 box, normalised to sum 1 (:134-151);
+ *   - element scale s_e = E_eff / E: void 0 (skipped), fixed 1, design
+ *     (E_min + rho~^p (E - E_min)) / E (fem.py:181-207);
+ *   - per load case, K_ff u_f = F_f with the case's constrained DOFs and the orphan DOFs
+ *     (|K_ii| < 1e-15) eliminated (fem.py:262-304; the reference solves directly, here FP64
+ *     Jacobi-PCG to solve_tol), compliance sum_e E_ratio_e u_e^T Ke0 u_e (fem.py:356-384),
+ *     dc_e += -p rho~_e^(p-1) u_e^T Ke0 u_e on design elements (topopt.py:158-198);
+ *   - dcf = convolve(dc * rho, w, 'reflect') / max(rho, 1e-3) (:363);
+ *   - OC bisection on [1e-9, 1e9], stop when (l2-l1)/max(l1+l2, 1e-10) <= 1e-3, Be =
+ *     sqrt(-dcf / (lmid + 1e-10)), density of the last evaluated lmid (:200-262);
+ *   - change = max |rho_new - rho|; volume = mean rho_new over the design elements. */
+typedef struct amt_simp amt_simp;
+typedef struct {
+  double volume_fraction; /* SIMPParams defaults: 0.25 */
+  double penalty;         /* 3.0 */
+  double filter_radius;   /* 2.0 (elements) */
+  double move_limit;      /* 0.2 */
+  double rho_min;         /* 0.01 */
+  double convergence_tol; /* 0.01 */
+  double E_min;           /* 1e-9 */
+  int max_iterations;     /* 100 */
+  double solve_tol;       /* relative residual of the per-load-case solve, e.g. 1e-10 */
+  int solve_maxit;
+} amt_simp_params;
+typedef struct {
+  double compliance; /* sum over load cases */
+  double volume;     /* mean density over design elements */
+  double change;     /* max |rho_new - rho| */
+  int solver_iterations;      /* summed over load cases */
+  double max_true_rel_residual; /* worst load case, |M(F - K u)| / |M F| */
+  int solver_converged;       /* all load cases reached solve_tol */
+} amt_simp_iteration;
+/* grid: (nx, ny, nz) C-order int8 (-1 void, 0 design, 1 fixed). Density starts at
+ * volume_fraction everywhere (topopt.py:124). */
+int amt_simp_create(int nx, int ny, int nz, double element_size, double E, double nu,
+                    const int8_t* grid, const amt_simp_params* prm, amt_simp** out);
+int amt_simp_destroy(amt_simp* h);
+/* Appends a load case: F[3 (nx+1)(ny+1)(nz+1)] and its constrained DOFs (any order). */
+int amt_simp_add_load_case(amt_simp* h, const double* F, int64_t n_dof,
+                           const int64_t* constrained, int64_t n_constrained);
+int amt_simp_set_density(amt_simp* h, const double* rho, int64_t n_e);
+int amt_simp_get_density(amt_simp* h, double* rho, int64_t n_e);
+/* One iteration of SIMPOptimizer.run's loop (topopt.py:294-383). */
+int amt_simp_step(amt_simp* h, amt_simp_iteration* out);
+/* y = convolve(x, w, mode='reflect') on the element grid. */
+int amt_simp_filter(amt_simp* h, const double* x, double* y, int64_t n_e);
+/* Displacement of load case lc from the last step. */
+int amt_simp_get_displacement(amt_simp* h, int lc, double* u, int64_t n_dof);
+
 /* Wait for all work of this handle. */
 int amt_sync(amt_problem* p);
 /* Device name / CU count of the handle's GPU. */
