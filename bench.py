@@ -102,6 +102,55 @@ def run_large(steps):
                        "(split-BF16 operator, FP32 accumulation)")
 
 
+def run_simp_py(steps, cpu):
+    """The Python SIMPOptimizer flavour (topopt.py:84-411) on the BRK-A-01 bracket: 4 mm
+    (30x15x20, the reference's test-run resolution) and 2 mm (60x30x40); SIMPParams defaults,
+    Ti6Al4V, each load-case solve to a 1e-10 relative residual. The oracle (numpy/scipy, sparse
+    direct solve, as the reference) is timed on the 4 mm grid."""
+    import am_ml_suite_amd as A
+    from am_ml_suite_amd.simp import SIMPOptimizer, SIMPParams
+    out = {}
+    for res_mm in (4.0, 2.0):
+        dom = A.bracket_domain(res_mm)
+        lc = A.bracket_load_case(dom)
+        opt = SIMPOptimizer(dom, [lc], params=SIMPParams(max_iterations=steps + 1),
+                            verbose=False)
+        d, it = opt.step(opt.density)  # warm-up
+        first_c = it.compliance
+        t0 = time.perf_counter()
+        comp, pcg, resid = [], [], []
+        for _ in range(steps):
+            d, it = opt.step(d)
+            comp.append(it.compliance)
+            pcg.append(it.solver_iterations)
+            resid.append(it.max_true_rel_residual)
+        dt = time.perf_counter() - t0
+        n_dof = 3 * (dom.shape[0] + 1) * (dom.shape[1] + 1) * (dom.shape[2] + 1)
+        out["bracket_%gmm" % res_mm] = dict(
+            grid=list(dom.shape), n_dofs=n_dof, steps=steps, oc_iter_per_s=steps / dt,
+            mean_pcg_iters=float(np.mean(pcg)), max_true_rel_residual=float(max(resid)),
+            first_compliance=first_c, last_compliance=comp[-1])
+        opt.close()
+        log("  [simp_py] %g mm: %.2f OC iter/s, %.0f PCG iterations/solve"
+            % (res_mm, steps / dt, np.mean(pcg)))
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import simp_py as S
+        dom = A.bracket_domain(4.0)
+        lc = A.bracket_load_case(dom)
+        P = S.Problem(dom.grid, 4.0, [(lc.get_force_vector(), lc.get_constrained_dofs())])
+        d = np.full(dom.shape, P.vf)
+        t0 = time.perf_counter()
+        d, info = S.step(P, d)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(value=1.0 / dt, unit="OC iter/s", cores=1, kind="port",
+                                   sample="oracle/simp_py.py (numpy/scipy, SuperLU direct "
+                                          "solve as the reference) on the 4 mm bracket, the "
+                                          "first OC iteration from the uniform start",
+                                   compliance=info["compliance"])
+    return out
+
+
 def run_batch(warmup, steps):
     import am_ml_suite_amd as A
     from am_ml_suite_amd.problems import make_config
@@ -163,7 +212,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--paths", default="plain,fast,batch")
+    ap.add_argument("--paths", default="plain,fast,batch,simp_py")
     ap.add_argument("--large-steps", type=int, default=100)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a plain copy kernel of "
@@ -200,6 +249,10 @@ def main():
     large = None
     if "large" in paths:
         large = run_large(args.large_steps)
+    simp_py = None
+    if "simp_py" in paths:
+        log("simp_py: the Python SIMPOptimizer flavour")
+        simp_py = run_simp_py(5, not args.no_cpu_baseline)
     batch = None
     if "batch" in paths:
         log("batch: 64 generated problems in lockstep")
@@ -209,6 +262,8 @@ def main():
         line = {"metric": "auxiliary workloads", "batch": batch}
         if large is not None:
             line["large"] = large
+        if simp_py is not None:
+            line["simp_py"] = simp_py
         print(json.dumps(line), flush=True)
         return
 
@@ -274,6 +329,8 @@ def main():
     }
     if large is not None:
         line["large"] = large
+    if simp_py is not None:
+        line["simp_py"] = simp_py
     line.update(extra)
     print(json.dumps(line), flush=True)
 
