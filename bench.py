@@ -151,6 +151,44 @@ def run_simp_py(steps, cpu):
     return out
 
 
+def run_ai(n_train=192, n_test=16, shape=(24, 16, 16), oc_iters=40, steps=1000, batch=8):
+    """The AI density predictor: ground-truth designs from the GPU SIMP engine, U-Net training
+    (device forward/backward, host Adam), held-out evaluation against the optimised designs."""
+    from am_ml_suite_amd import ai
+    probs = ai.ai_problems(n_train + n_test, shape=shape, seed=4)
+    t0 = time.perf_counter()
+    designs = ai.optimise(probs, oc_iters)
+    t_gen = time.perf_counter() - t0
+    log("  [ai] %d designs (%d OC iterations each) in %.1fs" % (len(probs), oc_iters, t_gen))
+    samples = ai.make_samples(probs, designs)
+    A = ai.AIOptimizer(shape=shape, seed=1)
+    U = ai.AIOptimizer(shape=shape, seed=1)
+    t0 = time.perf_counter()
+    losses = A.train(samples[:n_train], steps, lr=1e-3, batch=batch, seed=0)
+    t_train = time.perf_counter() - t0
+    res_t, res_u, unif = [], [], []
+    for p, d in zip(probs[n_train:], designs[n_train:]):
+        res_t.append(A.validate_against_numerical(p, d))
+        res_u.append(U.validate_against_numerical(p, d))
+        unif.append(ai.compliance_of(p, np.full(d.shape, p.params["volume_fraction"]))
+                    / res_t[-1]["compliance_numerical"])
+    t0 = time.perf_counter()
+    for p in probs[n_train:]:
+        A.predict_density(ai._Domain(p), [ai._LC(p.F, p.fixed)], p.params["volume_fraction"])
+    t_inf = (time.perf_counter() - t0) / n_test
+    return dict(workload="ai_problems_%dx%dx%d_seed4" % shape, n_train=n_train, n_test=n_test,
+                oc_iterations_per_design=oc_iters, numerical_s_per_design=t_gen / len(probs),
+                train_steps=steps, batch=batch, train_s=t_train,
+                loss_first=float(np.mean(losses[:10])), loss_last=float(np.mean(losses[-25:])),
+                inference_ms_per_problem=1e3 * t_inf,
+                iou_trained=float(np.mean([r["iou"] for r in res_t])),
+                iou_untrained=float(np.mean([r["iou"] for r in res_u])),
+                compliance_ratio_median_trained=float(np.median([r["compliance_ratio"] for r in res_t])),
+                compliance_ratio_median_untrained=float(np.median([r["compliance_ratio"] for r in res_u])),
+                compliance_ratio_median_uniform=float(np.median(unif)),
+                compliance_ratios_trained=[round(r["compliance_ratio"], 3) for r in res_t])
+
+
 def run_batch(warmup, steps):
     import am_ml_suite_amd as A
     from am_ml_suite_amd.problems import make_config
@@ -249,6 +287,10 @@ def main():
     large = None
     if "large" in paths:
         large = run_large(args.large_steps)
+    ai_line = None
+    if "ai" in paths:
+        log("ai: density predictor")
+        ai_line = run_ai()
     simp_py = None
     if "simp_py" in paths:
         log("simp_py: the Python SIMPOptimizer flavour")
@@ -264,6 +306,8 @@ def main():
             line["large"] = large
         if simp_py is not None:
             line["simp_py"] = simp_py
+        if ai_line is not None:
+            line["ai"] = ai_line
         print(json.dumps(line), flush=True)
         return
 
@@ -331,6 +375,8 @@ def main():
         line["large"] = large
     if simp_py is not None:
         line["simp_py"] = simp_py
+    if ai_line is not None:
+        line["ai"] = ai_line
     line.update(extra)
     print(json.dumps(line), flush=True)
 
