@@ -406,8 +406,10 @@ __device__ __forceinline__ void mg_dense_rows8(const float* __restrict__ Ainv, i
   for (int u = 0; u < 8; ++u) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) acc[u] += __shfl_xor(acc[u], o, 64);
-    if (lane == 0 && row0 + u * rs < n) x[row0 + u * rs] = acc[u];
   }
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+    if (lane == 0 && row0 + u * rs < n) x[row0 + u * rs] = acc[u];
 }
 
 // Stencil level, thread (node, row = blockIdx.y). Three threads per node keep the coarse
@@ -478,25 +480,33 @@ __global__ __launch_bounds__(kBlock) void k_mg0_pre(const double* __restrict__ r
   __shared__ TileSmem<float> sm;
   int i0, j0, k0;
   if (!tile_origin(tg, i0, j0, k0)) return;
-  tile_load(sm, g, i0, j0, k0, [&](int64_t q) { return wd[q] * (float)r[q]; }, s);
-  __syncthreads();
   int li, lj, lk;
   tile_thread(li, lj, lk);
   const int i = i0 + li, j = j0 + lj, k = k0 + lk;
-  if (i > g.nx || j > g.ny || k > g.nz) return;
+  const bool own = i <= g.nx && j <= g.ny && k <= g.nz;
   const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
-  Nbr<float> nb;
-  float s8[8];
-  tile_fetch(sm, li, lj, lk, nb, s8);
-  float y[3];
-  apply_node(nb, s8, Ke, y);
+  // the epilogue's own-node loads are issued with the tile's, so their latency hides behind
+  // the tile load and the operator instead of following them
+  float rv[3] = {0.f, 0.f, 0.f}, fc[3] = {0.f, 0.f, 0.f};
+  if (own) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      rv[c] = (float)r[3 * n + c];
+      fc[c] = (float)fixcnt[3 * n + c];
+    }
+  }
+  tile_load(sm, g, i0, j0, k0, [&](int64_t q) { return wd[q] * (float)r[q]; }, s);
+  __syncthreads();
+  if (!own) return;
+  float y[3], xo[3];
+  tile_apply(sm, li, lj, lk, Ke, y, xo);
   const float pen = (float)(*P);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int64_t q = 3 * n + c;
-    y[c] = fmaf((float)fixcnt[q] * pen, nb.v[13][c], y[c]);
-    x0[q] = nb.v[13][c];
-    t[q] = (float)r[q] - y[c];
+    y[c] = fmaf(fc[c] * pen, xo[c], y[c]);
+    x0[q] = xo[c];
+    t[q] = rv[c] - y[c];
   }
 }
 
@@ -517,32 +527,41 @@ __global__ __launch_bounds__(kBlock) void k_mg0_post(const double* __restrict__ 
   __shared__ TileSmem<float> sm;
   int i0, j0, k0;
   const bool has = tile_origin(tg, i0, j0, k0);
-  if (has) tile_load(sm, g, i0, j0, k0, [&](int64_t qq) { return x[qq]; }, s);
-  __syncthreads();
   int li, lj, lk;
   tile_thread(li, lj, lk);
   const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+  const bool own = has && i <= g.nx && j <= g.ny && k <= g.nz;
+  const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
+  // own-node epilogue loads issued with the tile's (their latency hides behind the operator)
+  double rv[3] = {0.0, 0.0, 0.0}, qv[3] = {0.0, 0.0, 0.0};
+  float wv[3] = {0.f, 0.f, 0.f}, fc[3] = {0.f, 0.f, 0.f};
+  if (own) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      rv[c] = r[3 * n + c];
+      wv[c] = wd[3 * n + c];
+      fc[c] = (float)fixcnt[3 * n + c];
+      if (q) qv[c] = q[3 * n + c];
+    }
+  }
+  if (has) tile_load(sm, g, i0, j0, k0, [&](int64_t qq) { return x[qq]; }, s);
+  __syncthreads();
   double acc[2] = {0.0, 0.0};
-  if (has && i <= g.nx && j <= g.ny && k <= g.nz) {
-    const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
-    Nbr<float> nb;
-    float s8[8];
-    tile_fetch(sm, li, lj, lk, nb, s8);
-    float y[3];
-    apply_node(nb, s8, Ke, y);
+  if (own) {
+    float y[3], xo[3];
+    tile_apply(sm, li, lj, lk, Ke, y, xo);
     const float pen = (float)(*P);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t qq = 3 * n + c;
-      y[c] = fmaf((float)fixcnt[qq] * pen, nb.v[13][c], y[c]);
-      const double rv = r[qq];
+      y[c] = fmaf(fc[c] * pen, xo[c], y[c]);
       // wd == 0 marks a DOF no element touches (zero row): its correction stays 0
-      const double zv = wd[qq] == 0.f
+      const double zv = wv[c] == 0.f
                             ? 0.0
-                            : (double)nb.v[13][c] + (double)wd[qq] * (rv - (double)y[c]);
+                            : (double)xo[c] + (double)wv[c] * (rv[c] - (double)y[c]);
       z[qq] = zv;
-      acc[0] = fma(zv, rv, acc[0]);
-      if (q) acc[1] = fma(zv, q[qq], acc[1]);
+      acc[0] = fma(zv, rv[c], acc[0]);
+      acc[1] = fma(zv, qv[c], acc[1]);
     }
   }
   store_partials<2>(acc, partials, stride);
@@ -586,21 +605,27 @@ __global__ __launch_bounds__(kBlock) void k_fcg_apply(
   __shared__ TileSmem<double> sm;
   int i0, j0, k0;
   const bool has = tile_origin(tg, i0, j0, k0);
-  if (has)
-    tile_load(sm, g, i0, j0, k0, [&](int64_t qq) { return fma(beta, pold[qq], z[qq]); }, s);
-  __syncthreads();
   int li, lj, lk;
   tile_thread(li, lj, lk);
   const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+  const bool own = has && i <= g.nx && j <= g.ny && k <= g.nz;
+  const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
+  double fc[3] = {0.0, 0.0, 0.0};  // issued with the tile's loads
+  if (own) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fc[c] = (double)fixcnt[3 * n + c];
+  }
+  if (has)
+    tile_load(sm, g, i0, j0, k0, [&](int64_t qq) { return fma(beta, pold[qq], z[qq]); }, s);
+  __syncthreads();
   double acc[1] = {0.0};
-  if (has && i <= g.nx && j <= g.ny && k <= g.nz) {
-    const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
+  if (own) {
     double yv[3], pv[3];
     tile_apply(sm, li, lj, lk, Ke, yv, pv);
     const double pen = *P;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      yv[c] = fma((double)fixcnt[3 * n + c] * pen, pv[c], yv[c]);
+      yv[c] = fma(fc[c] * pen, pv[c], yv[c]);
       pnew[3 * n + c] = pv[c];
       q[3 * n + c] = yv[c];
       acc[0] = fma(pv[c], yv[c], acc[0]);
@@ -617,9 +642,9 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(double* __restrict__ x,
                                                        const FcgState* st,
                                                        const double* __restrict__ apart,
                                                        int nba, double* __restrict__ upart) {
-  if (st->done) return;
   double pq[1];
-  reduce_partials<1>(apart, nba, nba, pq);
+  reduce_partials<1>(apart, nba, nba, pq);  // before the flag test: the loads overlap it
+  if (st->done) return;
   const double a = st->rz / pq[0];
   double acc[1] = {0.0};
   AMT_GRID_STRIDE(i, n) {
@@ -639,13 +664,34 @@ __global__ __launch_bounds__(kBlock) void k_fcg_scalar(FcgState* st,
                                                        int nba, const double* __restrict__ zpart,
                                                        int nbz, int zstride, int first,
                                                        int* done_flag) {
+  // z.r, z.q, |r|^2 and p.q in one pass: every thread folds its strided partials of the four
+  // sums in ascending order (the order reduce_partials uses), four strides' loads in flight,
+  // then one block reduction; all before the flag test, so the loads overlap it
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  const int nmax = max(nbz, max(nbu, nba));
+  for (int b0 = threadIdx.x; b0 < nmax; b0 += 4 * kBlock) {
+    double x[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int b = b0 + u * kBlock;
+      x[u][0] = b < nbz ? zpart[b] : 0.0;
+      x[u][1] = b < nbz ? zpart[(size_t)zstride + b] : 0.0;
+      x[u][2] = (!first && b < nbu) ? upart[b] : 0.0;
+      x[u][3] = (!first && b < nba) ? apart[b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[c] += x[u][c];
+  }
+  double tot[4];
+  block_reduce<4>(v, tot);
+  const double zr[2] = {tot[0], tot[1]};
   FcgState s = *st;
   if (s.done) {
     if (threadIdx.x == 0) *done_flag = 1;
     return;
   }
-  double zr[2];
-  reduce_partials<2>(zpart, nbz, zstride, zr);
   if (first) {
     if (threadIdx.x == 0) {
       s.rz = zr[0];
@@ -654,9 +700,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_scalar(FcgState* st,
     }
     return;
   }
-  double rr[1], pq[1];
-  reduce_partials<1>(upart, nbu, nbu, rr);
-  reduce_partials<1>(apart, nba, nba, pq);
+  const double rr[1] = {tot[2]}, pq[1] = {tot[3]};
   if (threadIdx.x != 0) return;
   s.rr = rr[0];
   s.pq = pq[0];
