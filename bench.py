@@ -189,6 +189,64 @@ def run_ai(n_train=192, n_test=16, shape=(24, 16, 16), oc_iters=40, steps=1000, 
                 compliance_ratios_trained=[round(r["compliance_ratio"], 3) for r in res_t])
 
 
+def run_learned(shape=(96, 48, 48), train_steps=400, samples=64):
+    """The learned preconditioner on 96x48x48 bracket-like problems (698,691 DOFs): residual
+    samples of a Jacobi-PCG solve on seed 4, GPU training, then on the held-out seed-5 problem
+    the learned flexible CG (Jacobi-PCG fallback on stagnation) against plain Jacobi-PCG and
+    the multigrid FCG, each to the 1e-8 FP64 residual."""
+    import am_ml_suite_amd as A
+    from am_ml_suite_amd.problems import make_config
+
+    def make(seed, pre):
+        p = A.bracket_like_problem(*shape, seed=seed)
+        o = A.TopologyOptimizer(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3),
+                                make_config(p, preconditioner=pre))
+        o.set_loads(p.F, p.fixed)
+        o.set_density(p.rho0)
+        return o, p
+
+    tr, _ = make(4, "unet")
+    held, ph = make(5, "unet")
+    net = A.UNet(optimizer=tr, seed=3)
+    n = tr.collect_samples(samples)
+    nh = held.collect_samples(samples)
+    l0, j0 = held.train_unet(net, nh, lr=0.0, with_jacobi=True)
+    t0 = time.perf_counter()
+    losses, jl = tr.train_unet(net, train_steps, lr=1e-3, with_jacobi=True)
+    t_train = time.perf_counter() - t0
+    lh, jh = held.train_unet(net, nh, lr=0.0, with_jacobi=True)
+    held.attach_unet(net)
+    held.set_density(ph.rho0)
+    held.sync()
+    t0 = time.perf_counter()
+    u, st = held.solve()
+    t_l = time.perf_counter() - t0
+    out = dict(workload="bracket_like_%dx%dx%d_seed5" % shape, n_dofs=ph.n_dofs,
+               train_samples=n, train_steps=train_steps, train_s=t_train,
+               heldout_energy_per_application_learned=float(np.mean(lh)),
+               heldout_energy_per_application_jacobi=float(np.mean(jh)),
+               learned=dict(iterations=st["iterations"], fallback=st["fallback"], time_s=t_l,
+                            true_rel_residual=st["true_rel_residual"]))
+    c_l = float(u @ ph.F)
+    for name, pre in (("jacobi", "jacobi"), ("multigrid", "multigrid")):
+        o, p = make(5, pre)
+        o.sync()
+        t0 = time.perf_counter()
+        u2, s2 = o.solve()
+        dt = time.perf_counter() - t0
+        out[name] = dict(iterations=s2["iterations"], time_s=dt,
+                         true_rel_residual=s2["true_rel_residual"],
+                         compliance_rel_diff_learned=abs(c_l / float(u2 @ p.F) - 1))
+        o.close()
+    log("  [learned] held-out: learned %d its %.3fs (fallback %s), jacobi %d its %.3fs, "
+        "multigrid %d its %.3fs" % (st["iterations"], t_l, st["fallback"],
+                                    out["jacobi"]["iterations"], out["jacobi"]["time_s"],
+                                    out["multigrid"]["iterations"], out["multigrid"]["time_s"]))
+    tr.close()
+    held.close()
+    return out
+
+
 def run_batch(warmup, steps):
     import am_ml_suite_amd as A
     from am_ml_suite_amd.problems import make_config
@@ -287,6 +345,10 @@ def main():
     large = None
     if "large" in paths:
         large = run_large(args.large_steps)
+    learned = None
+    if "learned" in paths:
+        log("learned: U-Net preconditioner on 96x48x48")
+        learned = run_learned()
     ai_line = None
     if "ai" in paths:
         log("ai: density predictor")
@@ -306,6 +368,8 @@ def main():
             line["large"] = large
         if simp_py is not None:
             line["simp_py"] = simp_py
+        if learned is not None:
+            line["learned"] = learned
         if ai_line is not None:
             line["ai"] = ai_line
         print(json.dumps(line), flush=True)
@@ -375,6 +439,8 @@ def main():
         line["large"] = large
     if simp_py is not None:
         line["simp_py"] = simp_py
+    if learned is not None:
+        line["learned"] = learned
     if ai_line is not None:
         line["ai"] = ai_line
     line.update(extra)
