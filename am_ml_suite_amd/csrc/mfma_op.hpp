@@ -72,15 +72,18 @@ __device__ __forceinline__ void load_kefrag(KeFrag& f, const uint16_t* __restric
 template <typename F>
 __device__ __forceinline__ void mfma_tile_load(MfmaSmem& sm, const GridDims& g, int i0, int j0,
                                                int k0, F f, const float* __restrict__ s) {
-  float v[kXPer], sv[kSPer];
+  float v[kNPer][3], sv[kSPer];
   tile_gather(v, sv, g, i0, j0, k0, f, s);
 #pragma unroll
-  for (int t = 0; t < kXPer; ++t) {
-    const int q = threadIdx.x + t * kBlock;
-    if (q < kHalo * 3) {
-      const uint16_t h = mf_bf16(v[t]);
-      sm.xh[q] = h;
-      sm.xl[q] = mf_bf16(v[t] - mf_f32(h));
+  for (int t = 0; t < kNPer; ++t) {
+    const int node = threadIdx.x + t * kBlock;
+    if (node < kHalo) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const uint16_t h = mf_bf16(v[t][c]);
+        sm.xh[3 * node + c] = h;
+        sm.xl[3 * node + c] = mf_bf16(v[t][c] - mf_f32(h));
+      }
     }
   }
 #pragma unroll

@@ -64,44 +64,51 @@ __device__ __forceinline__ bool tile_origin(const TileGrid& tg, int& i0, int& j0
 // Fully unrolled, with every global load of the thread issued before the first LDS store, so
 // the block pays the memory latency once instead of once per loop trip (the loads are
 // independent; a rolled loop serialises them behind its stores).
-constexpr int kXPer = (kHalo * 3 + kBlock - 1) / kBlock;  // halo values per thread (8)
-constexpr int kSPer = (kETile + kBlock - 1) / kBlock;     // element scales per thread (2)
+constexpr int kNPer = (kHalo + kBlock - 1) / kBlock;   // halo nodes per thread (3)
+constexpr int kSPer = (kETile + kBlock - 1) / kBlock;  // element scales per thread (2)
 
+// Thread slots are whole halo nodes (three values each): the slot's tile coordinates are
+// decoded once per node, not once per value, and its global offset is a 32-bit offset from
+// the tile's (uniform, 64-bit) base, so the per-slot address arithmetic stays small.
 template <typename T, typename F>
-__device__ __forceinline__ void tile_gather(T (&v)[kXPer], T (&sv)[kSPer], const GridDims& g,
+__device__ __forceinline__ void tile_gather(T (&v)[kNPer][3], T (&sv)[kSPer], const GridDims& g,
                                             int i0, int j0, int k0, F f,
                                             const T* __restrict__ s) {
+  const int64_t nbase = ((int64_t)(i0 - 1) * g.nny + (j0 - 1)) * g.nnz + (k0 - 1);
 #pragma unroll
-  for (int t = 0; t < kXPer; ++t) {
-    const int q = threadIdx.x + t * kBlock;
-    const int c = q % 3;
-    const int node = q / 3;
+  for (int t = 0; t < kNPer; ++t) {
+    const int node = threadIdx.x + t * kBlock;
     const int hk = node % kHK, hj = (node / kHK) % kHJ, hi = node / (kHK * kHJ);
     const int i = i0 + hi - 1, j = j0 + hj - 1, k = k0 + hk - 1;
-    v[t] = T(0);
-    if (q < kHalo * 3 && i >= 0 && i <= g.nx && j >= 0 && j <= g.ny && k >= 0 && k <= g.nz)
-      v[t] = f(3 * (((int64_t)i * g.nny + j) * g.nnz + k) + c);
+    const bool ok = node < kHalo && i >= 0 && i <= g.nx && j >= 0 && j <= g.ny && k >= 0 &&
+                    k <= g.nz;
+    const int64_t q = 3 * (nbase + (hi * g.nny + hj) * g.nnz + hk);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[t][c] = ok ? f(q + c) : T(0);
   }
+  const int64_t ebase = ((int64_t)(i0 - 1) * g.ny + (j0 - 1)) * g.nz + (k0 - 1);
 #pragma unroll
   for (int t = 0; t < kSPer; ++t) {
     const int q = threadIdx.x + t * kBlock;
     const int ek = q % kEK, ej = (q / kEK) % kEJ, ei = q / (kEK * kEJ);
     const int i = i0 + ei - 1, j = j0 + ej - 1, k = k0 + ek - 1;
-    sv[t] = T(0);
-    if (q < kETile && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz)
-      sv[t] = s[((int64_t)i * g.ny + j) * g.nz + k];
+    const bool ok = q < kETile && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
+    sv[t] = ok ? s[ebase + (ei * g.ny + ej) * g.nz + ek] : T(0);
   }
 }
 
 template <typename T, typename F>
 __device__ __forceinline__ void tile_load(TileSmem<T>& sm, const GridDims& g, int i0, int j0,
                                           int k0, F f, const T* __restrict__ s) {
-  T v[kXPer], sv[kSPer];
+  T v[kNPer][3], sv[kSPer];
   tile_gather(v, sv, g, i0, j0, k0, f, s);
 #pragma unroll
-  for (int t = 0; t < kXPer; ++t) {
-    const int q = threadIdx.x + t * kBlock;
-    if (q < kHalo * 3) sm.x[q] = v[t];
+  for (int t = 0; t < kNPer; ++t) {
+    const int node = threadIdx.x + t * kBlock;
+    if (node < kHalo) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sm.x[3 * node + c] = v[t][c];
+    }
   }
 #pragma unroll
   for (int t = 0; t < kSPer; ++t) {
