@@ -426,13 +426,13 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q) {
   const float* res = mg[L - 1].x;  // correction of the level below the current one
   for (size_t l = L - 2; l >= 1; --l) {
     hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
-                       res, mg[l + 1].g, mg[l].x, mg[l].g, done);
+                       res, mg[l + 1].g, mg[l].x, mg[l].g, nullptr, done);
     hipLaunchKernelGGL(k_mg_post, dim3(nblk(mg[l].g.n_nodes), 3), dim3(kBlock), 0, p->stream,
                        mg[l].A, mg[l].g, mg[l].wd, mg[l].rhs, mg[l].x, mg[l].t, done);
     res = mg[l].t;
   }
   hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, res,
-                     mg[1].g, mg[0].x, mg[0].g, done);
+                     mg[1].g, mg[0].x, mg[0].g, p->fixcnt, done);
   if (mf)
     hipLaunchKernelGGL(k_mg0_post_mfma, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g,
                        p->tg, p->s32, p->Kh16, p->Kl16, p->fixcnt, p->scal + S_P, mg[0].wd,

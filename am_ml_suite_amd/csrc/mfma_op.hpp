@@ -271,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_mg0_pre_mfma(
     const float xv = wd[q] * (float)r[q];
     y[c] = fmaf((float)fixcnt[q] * pen, xv, y[c]);
     x0[q] = xv;
-    t[q] = (float)r[q] - y[c];
+    t[q] = fixcnt[q] ? 0.f : (float)r[q] - y[c];  // clamped residuals are not restricted
   }
 }
 

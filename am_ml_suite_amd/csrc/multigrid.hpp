@@ -46,6 +46,19 @@ __device__ __forceinline__ float pw1(int f, int c) {
 
 constexpr int kStencilBlocks = 14;  // stored 3x3 blocks per node: d = 13..26
 
+// Penalty rows on the coarse levels. A clamped fine node on an odd-sized upper boundary is
+// interpolated from two coarse nodes, one of them a layer past the grid, so the Galerkin
+// penalty term couples them with P/4 entries and pins only their average; the difference
+// keeps its O(K_ii) stiffness. In FP32 that stiffness is below the rounding of the P-sized
+// entries (P = 1e8 max K_ii), the coarse operator turns indefinite and the flexible CG
+// diverges (clamps on lower faces sit on coarse nodes and are unaffected). The coarse levels
+// therefore carry the penalty at 1e-5 P (1e3 max K_ii for the reference's factor), which
+// still pins the constrained combinations and is resolved in FP32; level 0 keeps the true
+// penalty, and the transfers between levels 0 and 1 skip the clamped fine DOFs (their
+// residual is not restricted, their value gets no coarse correction), so the smaller coarse
+// penalty never acts on them.
+constexpr double kMgCoarsePenalty = 1e-5;
+
 // Block (r, c) of A(n, n + d); the neighbour n + d must exist. sj, si: node strides along j, i.
 __device__ __forceinline__ float stencil_coef(const float* __restrict__ A, int64_t nn, int64_t n,
                                               int d, int r, int c, int64_t sj, int64_t si) {
@@ -97,8 +110,9 @@ __global__ __launch_bounds__(kBlock) void k_mg_galerkin1(const double* __restric
         acc[2] = fma(se, Gr[2], acc[2]);
       }
     }
-    // penalty rows: sum_m w_N(m) w_M(m) pen(m,i) on the diagonal component
-    const double pen = *P;
+    // penalty rows: sum_m w_N(m) w_M(m) pen(m,i) on the diagonal component, with the
+    // penalty scaled down to kMgCoarsePenalty * P (see the note at kMgCoarsePenalty)
+    const double pen = kMgCoarsePenalty * (*P);
     for (int mx = 2 * I - 1; mx <= 2 * I + 1; ++mx) {
       if (mx < 0 || mx > fg.nx || pw1(mx, MI) == 0.0f) continue;
       for (int my = 2 * J - 1; my <= 2 * J + 1; ++my) {
@@ -356,10 +370,11 @@ __device__ __forceinline__ void mg_restrict_node(const float* __restrict__ rf, c
   for (int c = 0; c < 3; ++c) rc[3 * N + c] = acc[c];
 }
 
-// Prolongation x_f += P x_c at fine node n.
+// Prolongation x_f += P x_c at fine node n; DOFs with fixcnt > 0 (level 0 only; null on the
+// coarse levels) get no correction.
 __device__ __forceinline__ void mg_prolong_node(const float* __restrict__ xc, const MgGrid& cg,
                                                 float* __restrict__ xf, const MgGrid& fg,
-                                                int64_t n) {
+                                                int64_t n, const uint8_t* __restrict__ fixcnt) {
   int i, j, k;
   mg_decode(n, fg, i, j, k);
   const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
@@ -382,7 +397,8 @@ __device__ __forceinline__ void mg_prolong_node(const float* __restrict__ xc, co
         for (int q = 0; q < 3; ++q) acc[q] = fmaf(w, xc[3 * N + q], acc[q]);
       }
 #pragma unroll
-  for (int q = 0; q < 3; ++q) xf[3 * n + q] += acc[q];
+  for (int q = 0; q < 3; ++q)
+    if (!fixcnt || !fixcnt[3 * n + q]) xf[3 * n + q] += acc[q];
 }
 
 // Dense rows x[row0 + u*rs] = Ainv[row, :] r, u < 8, for the calling wave: lanes split the
@@ -449,11 +465,12 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict_
 // Prolongation (fine node per thread).
 __global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__ xc, MgGrid cg,
                                                        float* __restrict__ xf, MgGrid fg,
+                                                       const uint8_t* __restrict__ fixcnt,
                                                        const int* done) {
   if (*done) return;
   const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (n >= fg.n_nodes) return;
-  mg_prolong_node(xc, cg, xf, fg, n);
+  mg_prolong_node(xc, cg, xf, fg, n, fixcnt);
 }
 
 // Coarsest level: x = Ainv r (dense, n <= 1024); eight rows per wave, four waves per block.
@@ -506,7 +523,7 @@ __global__ __launch_bounds__(kBlock) void k_mg0_pre(const double* __restrict__ r
     const int64_t q = 3 * n + c;
     y[c] = fmaf(fc[c] * pen, xo[c], y[c]);
     x0[q] = xo[c];
-    t[q] = rv[c] - y[c];
+    t[q] = fc[c] > 0.f ? 0.f : rv[c] - y[c];  // clamped residuals are not restricted
   }
 }
 
