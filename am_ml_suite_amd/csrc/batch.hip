@@ -15,9 +15,12 @@
 // convergence is judged on the true residual and the FP32 matvec error cannot let the two
 // drift apart.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -35,6 +38,7 @@ using namespace amt;
 namespace {
 
 constexpr int kBChunk = 16;        // PCG iterations per chunk graph (even: parity returns)
+constexpr int kBMgWorkers = 4;     // host threads of the multigrid batch (one HW queue each)
 constexpr int kBPersist = 2048;    // blocks of a persistent launch (8 per CU)
 constexpr int kBOcLevels = 6;
 constexpr int kBOcCand = (1 << kBOcLevels) - 1;
@@ -57,6 +61,10 @@ int bfail(int code, const std::string& msg) { return amt::set_global_error(code,
 
 struct amt_batch {
   int np = 0;
+  // multigrid mode: one single-problem handle (and HIP stream) per problem, driven by
+  // kBMgWorkers host threads, so the small problems' solver launches overlap on the device
+  std::vector<amt_problem*> mg;
+  std::vector<uint8_t> mg_begun;
   bool mixed = true, warm = true;
   std::vector<amt_config> cfg;
   std::vector<BProb> hp;
@@ -110,6 +118,7 @@ unsigned grid_of(int n_items) {
 
 void destroy(amt_batch* b) {
   if (!b) return;
+  for (amt_problem* h : b->mg) amt_destroy(h);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (hipGraphExec_t g : {b->g_pre, b->g_start0, b->g_start1, b->g_chunk, b->g_post})
     if (g) (void)hipGraphExecDestroy(g);
@@ -317,6 +326,53 @@ int run_impl(amt_batch* b, int iterations, amt_iteration_result* hist, double* e
 
 }  // namespace
 
+// Multigrid mode: kBMgWorkers host threads take whole problems from a shared counter and run
+// their `iterations` OC iterations on the problem's own handle and stream (single-problem
+// semantics, multigrid FCG). Problems are independent, so the order in which they run does
+// not change any result.
+int run_mg(amt_batch* b, int iterations, amt_iteration_result* hist, double* elapsed_s) {
+  for (int i = 0; i < b->np; ++i)
+    if (!b->loads_set[i])
+      return bfail(AMT_EINVAL, "problem " + std::to_string(i) + ": loads not set");
+  const auto t0 = std::chrono::steady_clock::now();
+  std::atomic<int> next{0}, failed{-1};
+  std::vector<int> rcs(b->np, AMT_OK);
+  auto work = [&]() {
+    for (;;) {
+      const int i = next.fetch_add(1);
+      if (i >= b->np || failed.load() >= 0) return;
+      amt_problem* h = b->mg[i];
+      int rc = AMT_OK;
+      if (!b->mg_begun[i]) {
+        rc = amt_begin_run(h);
+        b->mg_begun[i] = 1;
+      }
+      for (int it = 0; it < iterations && rc == AMT_OK; ++it) {
+        amt_iteration_result r;
+        rc = amt_run_iteration(h, &r);
+        if (rc == AMT_OK && hist) hist[(size_t)it * b->np + i] = r;
+      }
+      if (rc == AMT_OK) rc = amt_sync(h);
+      if (rc != AMT_OK) {
+        rcs[i] = rc;
+        int expect = -1;
+        failed.compare_exchange_strong(expect, i);
+      }
+    }
+  };
+  const int nw = std::min(kBMgWorkers, b->np);
+  std::vector<std::thread> pool;
+  for (int w = 1; w < nw; ++w) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  if (elapsed_s)
+    *elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const int f = failed.load();
+  if (f >= 0)
+    return bfail(rcs[f], "problem " + std::to_string(f) + ": " + amt_last_error(b->mg[f]));
+  return AMT_OK;
+}
+
 extern "C" {
 
 int amt_batch_create(const amt_config* cfgs, int n, amt_batch** out) {
@@ -330,14 +386,35 @@ int amt_batch_create(const amt_config* cfgs, int n, amt_batch** out) {
       return bfail(AMT_EINVAL, "problem " + std::to_string(i) + ": invalid element size or material");
     if (!(c.filter_radius >= 0) || c.filter_radius > 64)
       return bfail(AMT_EINVAL, "problem " + std::to_string(i) + ": filter radius must be in [0, 64]");
-    if (c.density_index != AMT_INDEX_CONSISTENT || c.precond != AMT_PRECOND_JACOBI)
-      return bfail(AMT_ENOTSUP, "batch: consistent density index and Jacobi-PCG only");
+    const bool mgc = cfgs[0].precond == AMT_PRECOND_MULTIGRID;
+    if (c.density_index != AMT_INDEX_CONSISTENT ||
+        c.precond != (mgc ? AMT_PRECOND_MULTIGRID : AMT_PRECOND_JACOBI))
+      return bfail(AMT_ENOTSUP, "batch: consistent density index and one solver for all "
+                                "problems (Jacobi-PCG or multigrid)");
     if (c.precision != cfgs[0].precision || c.warm_start != cfgs[0].warm_start)
       return bfail(AMT_EINVAL, "batch: precision and warm_start must agree across problems");
   }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return bfail(AMT_EHIP, "no HIP device available");
+  if (cfgs[0].precond == AMT_PRECOND_MULTIGRID) {
+    amt_batch* b = new amt_batch();
+    b->np = n;
+    b->cfg.assign(cfgs, cfgs + n);
+    b->loads_set.assign(n, 0);
+    b->mg_begun.assign(n, 0);
+    for (int i = 0; i < n; ++i) {
+      amt_problem* h = nullptr;
+      if (amt_create(&cfgs[i], &h) != AMT_OK) {
+        const std::string msg = amt_last_error(nullptr);
+        destroy(b);
+        return bfail(AMT_EHIP, "problem " + std::to_string(i) + ": " + msg);
+      }
+      b->mg.push_back(h);
+    }
+    *out = b;
+    return AMT_OK;
+  }
   amt_batch* b = new amt_batch();
   b->np = n;
   b->mixed = cfgs[0].precision == AMT_PREC_MIXED_FP32;
@@ -506,6 +583,12 @@ int amt_batch_destroy(amt_batch* b) {
 
 int amt_batch_size(const amt_batch* b, int i, int64_t* n_elements, int64_t* n_dofs) {
   if (!b || i < 0 || i >= b->np) return bfail(AMT_ERANGE, "problem index out of range");
+  if (!b->mg.empty()) {
+    const amt_config& c = b->cfg[i];
+    if (n_elements) *n_elements = (int64_t)c.nx * c.ny * c.nz;
+    if (n_dofs) *n_dofs = 3LL * (c.nx + 1) * (c.ny + 1) * (c.nz + 1);
+    return AMT_OK;
+  }
   if (n_elements) *n_elements = b->hp[i].g.n_elem;
   if (n_dofs) *n_dofs = 3 * b->hp[i].g.n_nodes;
   return AMT_OK;
@@ -514,6 +597,12 @@ int amt_batch_size(const amt_batch* b, int i, int64_t* n_elements, int64_t* n_do
 int amt_batch_set_loads(amt_batch* b, int i, const double* F, int64_t n_dof,
                         const int64_t* fixed, int64_t n_fixed) {
   if (!b || i < 0 || i >= b->np) return bfail(AMT_ERANGE, "problem index out of range");
+  if (!b->mg.empty()) {
+    const int rc = amt_set_loads(b->mg[i], F, n_dof, fixed, n_fixed);
+    if (rc != AMT_OK) return bfail(rc, amt_last_error(b->mg[i]));
+    b->loads_set[i] = 1;
+    return AMT_OK;
+  }
   const BProb& P = b->hp[i];
   const int64_t nd = 3 * P.g.n_nodes;
   if (!F || n_dof != nd)
@@ -540,6 +629,11 @@ int amt_batch_set_loads(amt_batch* b, int i, const double* F, int64_t n_dof,
 
 int amt_batch_set_density(amt_batch* b, int i, const double* rho, int64_t n_e) {
   if (!b || i < 0 || i >= b->np) return bfail(AMT_ERANGE, "problem index out of range");
+  if (!b->mg.empty()) {
+    const int rc = amt_set_density(b->mg[i], rho, n_e);
+    if (rc != AMT_OK) return bfail(rc, amt_last_error(b->mg[i]));
+    return AMT_OK;
+  }
   const BProb& P = b->hp[i];
   if (!rho || n_e != P.g.n_elem)
     return bfail(AMT_EINVAL, "Density vector size mismatch: expected " +
@@ -553,6 +647,11 @@ int amt_batch_set_density(amt_batch* b, int i, const double* rho, int64_t n_e) {
 
 int amt_batch_get_density(amt_batch* b, int i, double* rho, int64_t n_e) {
   if (!b || i < 0 || i >= b->np) return bfail(AMT_ERANGE, "problem index out of range");
+  if (!b->mg.empty()) {
+    const int rc = amt_get_density(b->mg[i], rho, n_e);
+    if (rc != AMT_OK) return bfail(rc, amt_last_error(b->mg[i]));
+    return AMT_OK;
+  }
   const BProb& P = b->hp[i];
   if (!rho || n_e != P.g.n_elem) return bfail(AMT_EINVAL, "density buffer size mismatch");
   B_HIP(hipStreamSynchronize(b->stream));
@@ -562,6 +661,11 @@ int amt_batch_get_density(amt_batch* b, int i, double* rho, int64_t n_e) {
 
 int amt_batch_get_displacement(amt_batch* b, int i, double* u, int64_t n_dof) {
   if (!b || i < 0 || i >= b->np) return bfail(AMT_ERANGE, "problem index out of range");
+  if (!b->mg.empty()) {
+    const int rc = amt_get_displacement(b->mg[i], u, n_dof);
+    if (rc != AMT_OK) return bfail(rc, amt_last_error(b->mg[i]));
+    return AMT_OK;
+  }
   const BProb& P = b->hp[i];
   if (!u || n_dof != 3 * P.g.n_nodes) return bfail(AMT_EINVAL, "displacement buffer size mismatch");
   B_HIP(hipStreamSynchronize(b->stream));
@@ -571,6 +675,7 @@ int amt_batch_get_displacement(amt_batch* b, int i, double* u, int64_t n_dof) {
 
 int amt_batch_run(amt_batch* b, int iterations, amt_iteration_result* hist, double* elapsed_s) {
   if (!b || iterations < 0) return bfail(AMT_EINVAL, "bad argument");
+  if (!b->mg.empty()) return run_mg(b, iterations, hist, elapsed_s);
   return b->mixed ? run_impl<float>(b, iterations, hist, elapsed_s)
                   : run_impl<double>(b, iterations, hist, elapsed_s);
 }

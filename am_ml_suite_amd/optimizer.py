@@ -442,10 +442,13 @@ class UNet:
 
 
 class BatchOptimizer:
-    """Many independent problems optimised in lockstep on one GPU (amt_batch_*): every phase
-    one persistent launch over all problems, the OC step replayed from hipGraphs.
-    problems: list of (nx, ny, nz, element_size, MaterialProperties, SIMPConfig); precision and
-    warm_start must agree; density index consistent, Jacobi-PCG."""
+    """Many independent problems optimised on one GPU (amt_batch_*). problems: list of
+    (nx, ny, nz, element_size, MaterialProperties, SIMPConfig); precision, warm_start and the
+    solver must agree; density index consistent.
+      - Jacobi-PCG: all problems in lockstep, every phase one persistent launch over all of
+        them, the OC step replayed from hipGraphs;
+      - preconditioner="multigrid": one single-problem handle and stream per problem, run by
+        four host threads so the problems' launches overlap on the device."""
 
     def __init__(self, problems):
         self._L = _ffi.load()

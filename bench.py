@@ -247,13 +247,14 @@ def run_learned(shape=(96, 48, 48), train_steps=400, samples=64):
     return out
 
 
-def run_batch(warmup, steps):
+def run_batch(warmup, steps, solver="jacobi"):
     import am_ml_suite_amd as A
     from am_ml_suite_amd.problems import make_config
     ps = A.batch_problems()
+    kw = (dict(precision="mixed_fp32", warm_start=True) if solver == "jacobi"
+          else dict(preconditioner="multigrid", warm_start=True))
     B = A.BatchOptimizer([(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3),
-                           make_config(p, precision="mixed_fp32", warm_start=True))
-                          for p in ps])
+                           make_config(p, **kw)) for p in ps])
     for i, p in enumerate(ps):
         B.set_loads(i, p.F, p.fixed)
         B.set_density(i, p.rho0)
@@ -264,7 +265,10 @@ def run_batch(warmup, steps):
     flat = [r for row in hist for r in row]
     log("  [batch] %d OC iterations of %d problems in %.3fs" % (steps, len(ps), el))
     return dict(workload="batch_problems_n64_seed3", n_problems=len(ps), n_dofs_total=n_dof,
-                n_elements_total=sum(p.n_elements for p in ps), precision="fp32 matvec + fp64",
+                n_elements_total=sum(p.n_elements for p in ps),
+                solver=("lockstep persistent launches, Jacobi-PCG, fp32 matvec + fp64"
+                        if solver == "jacobi" else
+                        "per-problem multigrid FCG (fp32 V-cycle, fp64 CG), 4 host threads"),
                 steps=steps, warmup=warmup, time_s=el, oc_iter_per_s=steps / el,
                 problem_oc_iter_per_s=steps * len(ps) / el, dof_oc_iter_per_s=n_dof * steps / el,
                 max_pcg_iters=max(r.solver_iterations for r in flat),
@@ -361,6 +365,8 @@ def main():
     if "batch" in paths:
         log("batch: 64 generated problems in lockstep")
         batch = run_batch(1, 3)
+        log("batch: the same 64 problems, multigrid per problem")
+        batch["multigrid"] = run_batch(1, 3, "multigrid")
 
     if not out:  # only the auxiliary workloads were asked for
         line = {"metric": "auxiliary workloads", "batch": batch}

@@ -107,3 +107,28 @@ def test_batch_errors():
     with pytest.raises(ValueError):  # precisions must agree
         A.BatchOptimizer([(p.nx, p.ny, p.nz, 1.0, MAT, make_config(p, precision=pr))
                           for p, pr in zip(probs, ("fp64", "mixed_fp32"))])
+
+
+def test_multigrid_batch_matches_single_problem_runs():
+    """Multigrid mode (one handle and stream per problem, four host threads): every problem's
+    compliance history and density are bitwise those of the same problem run alone, whatever
+    order the threads take the problems in."""
+    ps = A.batch_problems(n=6, seed=8, lo=10, hi=22)
+    cfgs = [make_config(p, preconditioner="multigrid", warm_start=True) for p in ps]
+    B = A.BatchOptimizer([(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3), c)
+                          for p, c in zip(ps, cfgs)])
+    for i, p in enumerate(ps):
+        B.set_loads(i, p.F, p.fixed)
+        B.set_density(i, p.rho0)
+    hist, _ = B.run(4)
+    for i in (0, 3, 5):
+        p = ps[i]
+        opt = A.TopologyOptimizer(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3), cfgs[i])
+        opt.set_loads(p.F, p.fixed)
+        opt.set_density(p.rho0)
+        opt.begin_run()
+        single = [opt.step() for _ in range(4)]
+        assert [r.compliance for r in single] == [hist[t][i].compliance for t in range(4)]
+        assert np.array_equal(opt.get_density(), B.get_density(i))
+        assert all(hist[t][i].solver_converged for t in range(4))
+        opt.close()
