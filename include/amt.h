@@ -207,7 +207,10 @@ int amt_unet_set_gain(amt_unet* net, double gain);
  * and reductions, replacing the recursive residual by the FP64 one every 16 iterations, until
  * |F_bc - K u| <= pcg_tol |F_bc|;
  * AMT_PREC_FP64 is plain FP64 Jacobi-PCG. precision and warm_start must agree across
- * problems. Errors of these calls are read with amt_last_error(NULL). */
+ * problems. With precond = AMT_PRECOND_MULTIGRID for every problem, each problem instead gets
+ * its own single-problem handle and HIP stream, and host threads run the problems' OC
+ * iterations concurrently (results bitwise those of each problem run alone). Errors of these
+ * calls are read with amt_last_error(NULL). */
 typedef struct amt_batch amt_batch;
 int amt_batch_create(const amt_config* configs, int n, amt_batch** out);
 int amt_batch_destroy(amt_batch* b);
