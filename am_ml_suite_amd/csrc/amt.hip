@@ -413,13 +413,17 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q) {
     hipLaunchKernelGGL(k_mg0_pre, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g, p->tg,
                        p->s32, p->Ke32, p->fixcnt, p->scal + S_P, mg[0].wd, mg[0].x, mg[0].t,
                        done);
+  // each restriction also writes the next level's pre-smoothing step x0 = wd * rhs (not for
+  // the dense coarsest level)
   hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[1].g.n_nodes)), dim3(kBlock), 0, p->stream,
-                     mg[0].t, mg[0].g, mg[1].rhs, mg[1].g, done);
+                     mg[0].t, mg[0].g, mg[1].rhs, mg[1].g, L > 2 ? mg[1].wd : nullptr, mg[1].x,
+                     done);
   for (size_t l = 1; l + 1 < L; ++l) {
-    hipLaunchKernelGGL(k_mg_pre, dim3(nblk(mg[l].g.n_nodes), 3), dim3(kBlock), 0, p->stream,
-                       mg[l].A, mg[l].g, mg[l].wd, mg[l].rhs, mg[l].x, mg[l].t, done);
+    hipLaunchKernelGGL(k_mg_pre, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
+                       mg[l].A, mg[l].g, mg[l].rhs, mg[l].x, mg[l].t, done);
     hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[l + 1].g.n_nodes)), dim3(kBlock), 0,
-                       p->stream, mg[l].t, mg[l].g, mg[l + 1].rhs, mg[l + 1].g, done);
+                       p->stream, mg[l].t, mg[l].g, mg[l + 1].rhs, mg[l + 1].g,
+                       l + 2 < L ? mg[l + 1].wd : nullptr, mg[l + 1].x, done);
   }
   hipLaunchKernelGGL(k_mg_dense, dim3((p->mg_nc + 8 * kWaves - 1) / (8 * kWaves)), dim3(kBlock),
                      0, p->stream, p->mg_ainv, p->mg_nc, mg[L - 1].rhs, mg[L - 1].x, done);
@@ -427,7 +431,7 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q) {
   for (size_t l = L - 2; l >= 1; --l) {
     hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
                        res, mg[l + 1].g, mg[l].x, mg[l].g, nullptr, done);
-    hipLaunchKernelGGL(k_mg_post, dim3(nblk(mg[l].g.n_nodes), 3), dim3(kBlock), 0, p->stream,
+    hipLaunchKernelGGL(k_mg_post, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
                        mg[l].A, mg[l].g, mg[l].wd, mg[l].rhs, mg[l].x, mg[l].t, done);
     res = mg[l].t;
   }

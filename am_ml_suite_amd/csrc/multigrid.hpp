@@ -312,19 +312,18 @@ __device__ __forceinline__ float stencil_row(const float* __restrict__ A, const 
   return y;
 }
 
-// Stencil level, row (node n, component r): x0 = wd*rhs (pre-smoothing from zero);
-// t = rhs - A x0.
+// Stencil level, row (node n, component r): t = rhs - A x0 with the pre-smoothing step from
+// zero, x0 = wd*rhs, already written by the restriction that produced rhs.
 __device__ __forceinline__ void mg_pre_row(const float* __restrict__ A, const MgGrid& g,
-                                           const float* __restrict__ wd,
-                                           const float* __restrict__ rhs, float* __restrict__ x0,
-                                           float* __restrict__ t, int64_t n, int r) {
+                                           const float* __restrict__ rhs,
+                                           const float* __restrict__ x0, float* __restrict__ t,
+                                           int64_t n, int r) {
   int i, j, k;
   mg_decode(n, g, i, j, k);
   float nb[27][3];
-  mg_gather(g, n, i, j, k, [&](int64_t q) { return wd[q] * rhs[q]; }, nb);
+  mg_gather(g, n, i, j, k, [&](int64_t q) { return x0[q]; }, nb);
   const float y = stencil_row(A, g, n, i, j, k, r, nb);
   const int64_t q = 3 * n + r;
-  x0[q] = wd[q] * rhs[q];  // == nb[13][r]; read directly to keep nb in registers
   t[q] = rhs[q] - y;
 }
 
@@ -343,10 +342,12 @@ __device__ __forceinline__ void mg_post_row(const float* __restrict__ A, const M
   out[q] = x[q] + wd[q] * (rhs[q] - y);
 }
 
-// Restriction r_c = P^T r_f at coarse node N.
+// Restriction r_c = P^T r_f at coarse node N; with wdc, also the coarse level's
+// pre-smoothing step from zero, x0c = wdc * r_c.
 __device__ __forceinline__ void mg_restrict_node(const float* __restrict__ rf, const MgGrid& fg,
                                                  float* __restrict__ rc, const MgGrid& cg,
-                                                 int64_t N) {
+                                                 int64_t N, const float* __restrict__ wdc,
+                                                 float* __restrict__ x0c) {
   int I, J, K;
   mg_decode(N, cg, I, J, K);
   float acc[3] = {0.f, 0.f, 0.f};
@@ -368,6 +369,10 @@ __device__ __forceinline__ void mg_restrict_node(const float* __restrict__ rf, c
       }
 #pragma unroll
   for (int c = 0; c < 3; ++c) rc[3 * N + c] = acc[c];
+  if (wdc) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x0c[3 * N + c] = wdc[3 * N + c] * acc[c];
+  }
 }
 
 // Prolongation x_f += P x_c at fine node n; DOFs with fixcnt > 0 (level 0 only; null on the
@@ -428,17 +433,36 @@ __device__ __forceinline__ void mg_dense_rows8(const float* __restrict__ Ainv, i
     if (lane == 0 && row0 + u * rs < n) x[row0 + u * rs] = acc[u];
 }
 
-// Stencil level, thread (node, row = blockIdx.y). Three threads per node keep the coarse
-// grids busy.
+// Stencil-level block mapping: a block handles kBlock consecutive nodes for one row component
+// r (uniform in the block, so the coefficient offsets stay scalar). Hardware block b runs on
+// XCD b % 8; the (node range, r) pairs are dealt out so that XCD x gets a contiguous run of
+// node ranges with all three rows of each, and the forward blocks, the neighbours' transposed
+// blocks and the three row passes over them come through that XCD's L2 (numbering the pairs
+// (range, r) = (b mod R, b div R) instead fetched each stencil block from HBM ~3 times).
+// Grid: 8 * per blocks with per = ceil(3 * ceil(n_nodes / kBlock) / 8).
+__device__ __forceinline__ bool mg_row_block(const MgGrid& g, int64_t& n, int& r) {
+  const int nr = (int)((g.n_nodes + kBlock - 1) / kBlock);
+  const int per = (3 * nr + 7) / 8;
+  const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  if (logical >= 3 * nr) return false;
+  r = logical % 3;
+  n = (int64_t)(logical / 3) * kBlock + threadIdx.x;
+  return n < g.n_nodes;
+}
+inline unsigned mg_row_grid(int64_t n_nodes) {
+  const int64_t nr = (n_nodes + kBlock - 1) / kBlock;
+  return 8u * (unsigned)((3 * nr + 7) / 8);
+}
+
 __global__ __launch_bounds__(kBlock) void k_mg_pre(const float* __restrict__ A, MgGrid g,
-                                                   const float* __restrict__ wd,
                                                    const float* __restrict__ rhs,
-                                                   float* __restrict__ x0,
+                                                   const float* __restrict__ x0,
                                                    float* __restrict__ t, const int* done) {
   if (*done) return;
-  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (n >= g.n_nodes) return;
-  mg_pre_row(A, g, wd, rhs, x0, t, n, blockIdx.y);
+  int64_t n;
+  int r;
+  if (!mg_row_block(g, n, r)) return;
+  mg_pre_row(A, g, rhs, x0, t, n, r);
 }
 
 __global__ __launch_bounds__(kBlock) void k_mg_post(const float* __restrict__ A, MgGrid g,
@@ -447,19 +471,22 @@ __global__ __launch_bounds__(kBlock) void k_mg_post(const float* __restrict__ A,
                                                     const float* __restrict__ x,
                                                     float* __restrict__ out, const int* done) {
   if (*done) return;
-  const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (n >= g.n_nodes) return;
-  mg_post_row(A, g, wd, rhs, x, out, n, blockIdx.y);
+  int64_t n;
+  int r;
+  if (!mg_row_block(g, n, r)) return;
+  mg_post_row(A, g, wd, rhs, x, out, n, r);
 }
 
 // Restriction (coarse node per thread).
 __global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict__ rf,
                                                         MgGrid fg, float* __restrict__ rc,
-                                                        MgGrid cg, const int* done) {
+                                                        MgGrid cg, const float* __restrict__ wdc,
+                                                        float* __restrict__ x0c,
+                                                        const int* done) {
   if (*done) return;
   const int64_t N = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (N >= cg.n_nodes) return;
-  mg_restrict_node(rf, fg, rc, cg, N);
+  mg_restrict_node(rf, fg, rc, cg, N, wdc, x0c);
 }
 
 // Prolongation (fine node per thread).
