@@ -314,6 +314,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--paths", default="plain,fast,batch,simp_py")
     ap.add_argument("--large-steps", type=int, default=100)
+    ap.add_argument("--fast-smoother", choices=("fp32", "mfma"), default="fp32",
+                    help="fine-level smoothing of the fast path's V-cycle: the FP32 vector "
+                         "operator (default) or the split-BF16 matrix-core operator")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a plain copy kernel of "
                          "known size, to scale the FETCH_SIZE / WRITE_SIZE counters)")
@@ -344,7 +347,9 @@ def main():
         out["plain"] = run_path(p, args.warmup, args.steps, "fp64", False, False)
     if "fast" in paths:
         log("fast: FP64 flexible CG + FP32 multigrid V-cycle + warm start")
-        out["fast"] = run_path(p, args.warmup, args.steps, "fp64", True, True, "multigrid")
+        out["fast"] = run_path(p, args.warmup, args.steps,
+                               "mixed_bf16" if args.fast_smoother == "mfma" else "fp64", True,
+                               True, "multigrid")
 
     large = None
     if "large" in paths:
@@ -434,10 +439,14 @@ def main():
         "value": ops, "unit": "OC iter/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * m["time"] / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64" if main_key == "fast" else "f64", "data": "synthetic",
+        "dtype": ("f64" if main_key != "fast" else
+                  "bf16+f32+f64" if args.fast_smoother == "mfma" else "f32+f64"),
+        "data": "synthetic",
         "config": {"workload": "bracket_like_120x60x60_seed2", "n_dofs": n_dof,
                    "n_elements": n_e, "pcg_tol": 1e-8, "path": main_key,
                    "solver": ("FP64 flexible CG + FP32 Galerkin multigrid V-cycle, warm start"
+                              + (", fine-level smoothing on MFMA (split BF16)"
+                                 if args.fast_smoother == "mfma" else "")
                               if main_key == "fast" else "FP64 Jacobi-PCG")},
         "roofline": roofline, "cpu_baseline": cpu, "batch": batch,
     }
