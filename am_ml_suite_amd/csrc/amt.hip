@@ -963,6 +963,11 @@ int oc_update(amt_problem* p, OcState* hs) {
   init.n_design = p->n_design > 0 ? p->n_design : (double)p->n_e;  // :222-225 fallback
   HIP_TRY(p, hipMemcpyAsync(p->oc, &init, sizeof(OcState), hipMemcpyHostToDevice, p->stream));
   const unsigned nb = std::min<unsigned>(nred(p->n_e), kOcBlocks);
+  hipLaunchKernelGGL(k_oc_bounds, dim3(nb), dim3(kBlock), 0, p->stream, p->rho, p->dcf,
+                     p->design, p->n_e, p->cfg.move_limit, p->cfg.rho_min, p->part(R_OC),
+                     p->stride_red);
+  hipLaunchKernelGGL(k_oc_bounds_final, dim3(1), dim3(kBlock), 0, p->stream, p->oc,
+                     (const double*)p->part(R_OC), (int)nb, p->stride_red);
   // ceil(100 / LEV) + 1 sweeps always suffice: each one advances LEV passes or finishes
   for (int sw = 0; sw < (100 + kOcLevels - 1) / kOcLevels + 1; ++sw) {
     hipLaunchKernelGGL((k_oc_sweep<kOcLevels>), dim3(nb), dim3(kBlock), 0, p->stream, p->rho,

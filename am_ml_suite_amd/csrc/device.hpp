@@ -46,6 +46,9 @@ struct PcgState {
 struct OcState {
   double l1, l2, lmid, vf, n_design, change, vol_sum, pad;
   int passes, done, pad1, pad2;
+  // passes whose outcome is known (k_oc_bounds): lambda + 1e-20 >= ff_hi -> volume vol_hi,
+  // lambda + 1e-20 <= ff_lo -> volume vol_lo (ff_hi = inf, ff_lo = -inf: none known)
+  double ff_hi, ff_lo, vol_hi, vol_lo;
 };
 
 // ---------------------------------------------------------------------------------------

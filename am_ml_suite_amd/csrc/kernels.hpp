@@ -448,8 +448,82 @@ __global__ __launch_bounds__(kBlock) void k_oc_sweep(const double* __restrict__ 
   store_partials_many<NC>(vol, partials, stride);
 }
 
+// Bisection passes with a known outcome. A design element with dc < 0 sits at its floor
+// max(rho_min, rho - move) once rho sqrt(-dc / (lambda + 1e-20)) <= that floor, i.e. for
+// lambda + 1e-20 >= rho^2 (-dc) / floor^2, and at its ceiling once the candidate reaches
+// rho + move, i.e. for lambda + 1e-20 <= rho^2 (-dc) / (rho + move)^2; elements with dc >= 0
+// take the same value at every lambda. Beyond the largest of the first bounds (below the
+// smallest of the second) the design volume is therefore a constant, computed once, and the
+// pass's l1/l2 decision is the one evaluating it would give. The bisection starts at
+// [1e-20, 1e20], so most of its ~80 passes only move l2 down to this range; they are applied
+// here in a few instructions instead of being evaluated sweep by sweep.
+__device__ __forceinline__ void oc_fast_forward(OcState& s) {
+  while (!s.done) {
+    if (s.passes >= 100 || (s.l2 - s.l1) / (s.l1 + s.l2) < 1e-4) {
+      s.done = 1;
+      break;
+    }
+    const double lm = 0.5 * (s.l1 + s.l2);
+    double vol;
+    if (lm + 1e-20 >= s.ff_hi)
+      vol = s.vol_hi;
+    else if (lm + 1e-20 <= s.ff_lo)
+      vol = s.vol_lo;
+    else
+      break;
+    s.lmid = lm;
+    ++s.passes;
+    if (vol / s.n_design > s.vf)
+      s.l1 = lm;
+    else
+      s.l2 = lm;
+  }
+}
+
+// Partials of the fast-forward bounds and volumes (grid and element order of k_oc_sweep):
+// [0] sum of candidates at lambda -> infinity, [1] at lambda = 0, [2] max floor bound,
+// [3] max of minus the ceiling bound.
+__global__ __launch_bounds__(kBlock) void k_oc_bounds(const double* __restrict__ rho,
+                                                      const double* __restrict__ dcf,
+                                                      const uint8_t* __restrict__ design,
+                                                      int64_t n, double move, double rho_min,
+                                                      double* partials, int stride) {
+  double v[2] = {0.0, 0.0}, b[2] = {0.0, -INFINITY};
+  AMT_GRID_STRIDE(e, n) {
+    if (!design[e]) continue;
+    const double r = rho[e], d = dcf[e];
+    v[0] += oc_candidate(r, d, 1e300, move, rho_min);
+    v[1] += oc_candidate(r, d, 0.0, move, rho_min);
+    if (d < 0.0) {
+      const double f = std_max(rho_min, r - move), c = r + move;
+      b[0] = fmax(b[0], r * r * (-d) / (f * f));
+      b[1] = fmax(b[1], -(r * r * (-d) / (c * c)));
+    }
+  }
+  store_partials<2>(v, partials, stride);
+  store_partials<2, true>(b, partials + 2 * (size_t)stride, stride);
+}
+
+// One block: the bounds into the state (with a 1e-9 relative margin over rounding), then the
+// first run of known passes.
+__global__ __launch_bounds__(kBlock) void k_oc_bounds_final(OcState* st, const double* partials,
+                                                            int nb, int stride) {
+  double v[2], b[2];
+  reduce_partials<2>(partials, nb, stride, v);
+  reduce_partials<2, true>(partials + 2 * (size_t)stride, nb, stride, b);
+  if (threadIdx.x) return;
+  OcState s = *st;
+  s.vol_hi = v[0];
+  s.vol_lo = v[1];
+  s.ff_hi = b[0] * (1.0 + 1e-9);
+  s.ff_lo = -b[1] * (1.0 - 1e-9);
+  oc_fast_forward(s);
+  *st = s;
+}
+
 // One block: reduce the candidate volumes and walk the bisection tree with the reference's
-// break test; updates l1, l2, the last evaluated lambda and the pass count.
+// break test; updates l1, l2, the last evaluated lambda and the pass count, then applies the
+// passes whose outcome is known (oc_fast_forward).
 template <int LEV>
 __global__ __launch_bounds__(kBlock) void k_oc_walk(OcState* st,
                                                     const double* __restrict__ partials,
@@ -481,10 +555,13 @@ __global__ __launch_bounds__(kBlock) void k_oc_walk(OcState* st,
     }
   }
   if (!done && (passes >= 100 || (l2 - l1) / (l1 + l2) < 1e-4)) done = 1;
-  st->l1 = l1;
-  st->l2 = l2;
-  st->passes = passes;
-  st->done = done;
+  OcState s = *st;
+  s.l1 = l1;
+  s.l2 = l2;
+  s.passes = passes;
+  s.done = done;
+  oc_fast_forward(s);
+  *st = s;
 }
 
 // rho <- candidate at the last evaluated lambda (design) / unchanged (non-design);

@@ -145,7 +145,13 @@ def test_filter_values_match_oracle(r):
     assert np.array_equal(out, ref)
 
 
-def test_oc_update_matches_oracle():
+@pytest.mark.parametrize("vf", [0.45, 0.35, 0.005, 0.999])
+def test_oc_update_matches_oracle(vf):
+    """Bisection pass count, density and change against the oracle. vf = 0.45 ends inside
+    the bracket (78 passes); 0.35 and 0.005 lie below the all-at-floor volume (every pass
+    raises l1 until the relative gap closes, 13 passes); 0.999 lies above the all-at-ceiling
+    volume (every pass lowers l2 until the 100-pass cap). The last three end inside the
+    passes the device applies from the floor/ceiling bounds."""
     nx, ny, nz = 12, 9, 7
     rng = np.random.default_rng(8)
     n = nx * ny * nz
@@ -154,13 +160,14 @@ def test_oc_update_matches_oracle():
     rho[40:60] = 0.01                                  # non-design (void)
     dcf = -rng.uniform(0.001, 4.0, n)
     dcf[60] = 0.5                                      # positive: NaN branch
-    opt = _opt(nx, ny, nz, volume_fraction=0.35, move_limit=0.15, rho_min=0.01)
+    opt = _opt(nx, ny, nz, volume_fraction=vf, move_limit=0.15, rho_min=0.01)
     opt.set_density(rho)
     opt.begin_run()
     ch, passes = opt.oc_update(dcf)
     got = opt.get_density()
     design = ((rho > 0.01 + 1e-6) & (rho < 0.99)).astype(np.uint8)
-    ref, chr_, pr, _ = O.oc_update(rho, dcf, design, 0.35, 0.15, 0.01)
+    ref, chr_, pr, _ = O.oc_update(rho, dcf, design, vf, 0.15, 0.01)
+    print("vf %g: passes gpu %d oracle %d" % (vf, passes, pr))
     assert passes == pr
     assert np.abs(got - ref).max() <= 1e-15
     assert ch == pytest.approx(chr_, abs=1e-15)
