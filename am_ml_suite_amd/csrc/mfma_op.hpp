@@ -1,10 +1,10 @@
 // mfma_op.hpp — the fine-grid operator y = sum_e s_e Ke0 x_e on the matrix cores (gfx950
 // v_mfma_f32_16x16x16_bf16), for the FP32 parts of the solvers.
 //
-// Element-batched formulation. A 256-thread block owns the same 4x8x8 node tile as the vector
-// operator (tile_op.hpp) and the 5x9x9 = 405 elements touching it. The element products
-// Y = Ke0 X (X: 24 x 405, the elements' nodal vectors) are one GEMM of 24 x 24 by 24 x 405:
-// with 16 x 16 x 16 tiles that is 2 (rows) x 2 (k) x 26 (element groups) MFMAs, about
+// Element-batched formulation. A 256-thread block owns the same 4x4x16 node tile as the vector
+// operator (tile_op.hpp) and the 5x5x17 = 425 elements touching it. The element products
+// Y = Ke0 X (X: 24 x 425, the elements' nodal vectors) are one GEMM of 24 x 24 by 24 x 425:
+// with 16 x 16 x 16 tiles that is 2 (rows) x 2 (k) x 27 (element groups) MFMAs, about
 // 1/16 of the 576 FMAs per node of the vector operator in issue slots. The halo is staged in
 // LDS, the products are scaled by s_e and written to LDS, and each thread then sums the rows of
 // its node from its (up to) eight elements in a fixed order.
@@ -37,7 +37,7 @@ __device__ __forceinline__ uint16_t mf_bf16(float f) {
 __device__ __forceinline__ float mf_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 constexpr int kYS = 25;  // LDS row stride (floats) of one element's 24 products (odd: banks)
-constexpr int kEGroups = (kETile + 15) / 16;  // 26 groups of 16 elements
+constexpr int kEGroups = (kETile + 15) / 16;  // 27 groups of 16 elements
 
 struct MfmaSmem {
   uint16_t xh[kHalo * 3], xl[kHalo * 3];

@@ -1,9 +1,9 @@
 // tile_op.hpp — LDS-tiled matrix-free hex8 operator for the fine voxel grid.
 //
-// A block of 256 threads owns a TI x TJ x TK = 4 x 8 x 8 tile of nodes (one node per thread,
-// k fastest as in the DOF numbering). It loads the tile plus a one-node halo (6 x 10 x 10
+// A block of 256 threads owns a TI x TJ x TK = 4 x 4 x 16 tile of nodes (one node per thread,
+// k fastest as in the DOF numbering). It loads the tile plus a one-node halo (6 x 6 x 18
 // nodes, 3 components each) into LDS with contiguous, coalesced loads along k, applying the
-// caller's transform (e.g. p = z + beta*p_old) exactly once per node, and the 5 x 9 x 9
+// caller's transform (e.g. p = z + beta*p_old) exactly once per node, and the 5 x 5 x 17
 // element scales around the tile. Every thread then reads its 27 neighbours from LDS at
 // compile-time offsets and applies Ke0 (uniform scalar loads). Compared with each thread
 // gathering its 27 neighbours from global memory, this removes the per-neighbour 64-bit
@@ -18,12 +18,26 @@
 
 namespace amt {
 
-constexpr int kTI = 4, kTJ = 8, kTK = 8;
+// LDS banking (32 banks of 4 bytes; a wave's 64-bit reads are served 16 lanes per clock, its
+// 32-bit reads 32 lanes per clock). The 16 lanes of one k-row read 16 consecutive nodes, which
+// at 3 values per node hit 16 distinct bank pairs in FP64, so 64-bit reads are conflict-free.
+// In FP32 a 32-lane group holds two k-rows: tile_thread puts rows j and j + 2 in one group and
+// the FP32 layout pads the LDS row to 24 nodes, so the two rows sit 3 * 48 = 144 = 16 (mod 32)
+// banks apart and the group's 32 reads hit the 32 banks once. (The 4 x 8 x 8 tile put two
+// rows of 8 in each 16-lane group and took two clocks per FP64 read.)
+constexpr int kTI = 4, kTJ = 4, kTK = 16;
 constexpr int kHI = kTI + 2, kHJ = kTJ + 2, kHK = kTK + 2;
-constexpr int kHalo = kHI * kHJ * kHK;               // 600 nodes
+constexpr int kHalo = kHI * kHJ * kHK;               // 648 nodes
 constexpr int kEI = kTI + 1, kEJ = kTJ + 1, kEK = kTK + 1;
-constexpr int kETile = kEI * kEJ * kEK;              // 405 elements
+constexpr int kETile = kEI * kEJ * kEK;              // 425 elements
 static_assert(kTI * kTJ * kTK == kBlock, "one node per thread");
+static_assert(kTK == 16 && kTJ == 4, "tile_thread's row pairing and the FP32 padding assume it");
+
+// LDS row strides (in nodes / elements) and sizes of the tile arrays
+template <typename T> constexpr int kRowN = sizeof(T) == 4 ? 24 : kHK;
+template <typename T> constexpr int kRowE = sizeof(T) == 4 ? 24 : kEK;
+template <typename T> constexpr int kHaloL = kHI * kHJ * kRowN<T>;
+template <typename T> constexpr int kETileL = kEI * kEJ * kRowE<T>;
 
 struct TileGrid {
   int ti, tj, tk;   // tiles per axis
@@ -44,8 +58,8 @@ inline TileGrid make_tile_grid(const GridDims& g) {
 
 template <typename T>
 struct TileSmem {
-  T x[kHalo * 3];
-  T s[kETile];
+  T x[kHaloL<T> * 3];  // node (hi, hj, hk) at (hi * kHJ + hj) * kRowN + hk
+  T s[kETileL<T>];     // element (ei, ej, ek) at (ei * kEJ + ej) * kRowE + ek
 };
 
 // Tile origin (node coordinates) of this block; returns false if the block has no tile.
@@ -106,21 +120,24 @@ __device__ __forceinline__ void tile_load(TileSmem<T>& sm, const GridDims& g, in
   for (int t = 0; t < kNPer; ++t) {
     const int node = threadIdx.x + t * kBlock;
     if (node < kHalo) {
+      const int l = (node / kHK) * kRowN<T> + node % kHK;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) sm.x[3 * node + c] = v[t][c];
+      for (int c = 0; c < 3; ++c) sm.x[3 * l + c] = v[t][c];
     }
   }
 #pragma unroll
   for (int t = 0; t < kSPer; ++t) {
     const int q = threadIdx.x + t * kBlock;
-    if (q < kETile) sm.s[q] = sv[t];
+    if (q < kETile) sm.s[(q / kEK) * kRowE<T> + q % kEK] = sv[t];
   }
 }
 
-// This thread's node inside the tile.
+// This thread's node inside the tile: a wave is one i-plane, its four 16-lane rows hold
+// j = 0, 2, 1, 3 (rows j and j + 2 share a 32-lane LDS group).
 __device__ __forceinline__ void tile_thread(int& li, int& lj, int& lk) {
   lk = threadIdx.x % kTK;
-  lj = (threadIdx.x / kTK) % kTJ;
+  const int r = (threadIdx.x / kTK) % kTJ;
+  lj = ((r & 1) << 1) | (r >> 1);
   li = threadIdx.x / (kTK * kTJ);
 }
 
@@ -128,7 +145,7 @@ __device__ __forceinline__ void tile_thread(int& li, int& lj, int& lk) {
 template <typename T>
 __device__ __forceinline__ void tile_fetch(const TileSmem<T>& sm, int li, int lj, int lk,
                                            Nbr<T>& nb, T (&s8)[8]) {
-  const int base = ((li + 1) * kHJ + (lj + 1)) * kHK + (lk + 1);
+  const int base = ((li + 1) * kHJ + (lj + 1)) * kRowN<T> + (lk + 1);
 #pragma unroll
   for (int di = -1; di <= 1; ++di)
 #pragma unroll
@@ -136,14 +153,14 @@ __device__ __forceinline__ void tile_fetch(const TileSmem<T>& sm, int li, int lj
 #pragma unroll
       for (int dk = -1; dk <= 1; ++dk) {
         const int m = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
-        const int o = 3 * (base + (di * kHJ + dj) * kHK + dk);
+        const int o = 3 * (base + (di * kHJ + dj) * kRowN<T> + dk);
 #pragma unroll
         for (int c = 0; c < 3; ++c) nb.v[m][c] = sm.x[o + c];
       }
   // element (i - ox[a], ...) sits at tile element index (li+1-ox, lj+1-oy, lk+1-oz)
-  const int eb = ((li + 1) * kEJ + (lj + 1)) * kEK + (lk + 1);
+  const int eb = ((li + 1) * kEJ + (lj + 1)) * kRowE<T> + (lk + 1);
 #pragma unroll
-  for (int a = 0; a < 8; ++a) s8[a] = sm.s[eb - (kOX[a] * kEJ + kOY[a]) * kEK - kOZ[a]];
+  for (int a = 0; a < 8; ++a) s8[a] = sm.s[eb - (kOX[a] * kEJ + kOY[a]) * kRowE<T> - kOZ[a]];
 }
 
 // Element-loop form of tile_fetch + apply_node: y = sum_a s_a Ke0[3a.., :] x_e(a), reading
@@ -154,17 +171,18 @@ __device__ __forceinline__ void tile_fetch(const TileSmem<T>& sm, int li, int lj
 template <typename T, typename TK>
 __device__ __forceinline__ void tile_apply(const TileSmem<T>& sm, int li, int lj, int lk,
                                            const TK* __restrict__ Ke, T (&y)[3], T (&own)[3]) {
-  const int nb0 = ((li + 1) * kHJ + (lj + 1)) * kHK + (lk + 1);
-  const int eb = ((li + 1) * kEJ + (lj + 1)) * kEK + (lk + 1);
+  const int nb0 = ((li + 1) * kHJ + (lj + 1)) * kRowN<T> + (lk + 1);
+  const int eb = ((li + 1) * kEJ + (lj + 1)) * kRowE<T> + (lk + 1);
   y[0] = y[1] = y[2] = T(0);
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
     asm volatile("" ::: "memory");
-    const T se = sm.s[eb - (kOX[a] * kEJ + kOY[a]) * kEK - kOZ[a]];
+    const T se = sm.s[eb - (kOX[a] * kEJ + kOY[a]) * kRowE<T> - kOZ[a]];
     T t0 = T(0), t1 = T(0), t2 = T(0);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-      const int dn = ((kOX[b] - kOX[a]) * kHJ + (kOY[b] - kOY[a])) * kHK + (kOZ[b] - kOZ[a]);
+      const int dn =
+          ((kOX[b] - kOX[a]) * kHJ + (kOY[b] - kOY[a])) * kRowN<T> + (kOZ[b] - kOZ[a]);
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const T xv = sm.x[3 * (nb0 + dn) + d];
