@@ -56,7 +56,9 @@ constexpr int kNnPatience = 16;  // learned FCG: chunks allowed without a 10% |r
 constexpr double kNnGainMomentum = 0.05;  // running means behind the correction's weight
 constexpr float kMgOmega = 0.6f;  // damped-Jacobi weight of the V-cycle smoother...
 constexpr float kMgTheta = 1.9f;  // ...capped per row at theta / sum_j |A_ij| (spectrum < 2)
-constexpr int64_t kMgCoarseDofs = 300;  // coarsen until a level has at most this many DOFs
+// coarsen until a level has at most this many DOFs; the coarsest level's dense inverse runs in
+// one block with up to 6 x 6 register-resident entries per thread (n <= 192)
+constexpr int64_t kMgCoarseDofs = 192;
 
 // Partials regions (doubles) inside one buffer; see amt_problem::part()
 enum Region { R_APPLY = 0, R_UPD, R_INIT, R_OC, R_MISC, R_COUNT };
@@ -115,7 +117,6 @@ struct amt_problem {
   float* mg_ainv = nullptr;   // coarsest dense inverse
   int mg_nc = 0;              // coarsest DOFs
   FcgState* fcg = nullptr;
-  std::vector<float> mg_host_stencil;
   // one captured chunk of kMgChunk flexible-CG iterations, replayed per chunk
   hipGraph_t fcg_graph = nullptr;
   hipGraphExec_t fcg_exec = nullptr;
@@ -301,77 +302,13 @@ int mg_build_hierarchy(amt_problem* p, const double* Ke) {
     p->mg.push_back(L);
   }
   p->mg_nc = (int)(3 * p->mg.back().g.n_nodes);
-  if (p->mg_nc > 1024) return fail(p, AMT_EINVAL, "multigrid coarsest level too large");
+  if (p->mg_nc > kMgCoarseDofs) return fail(p, AMT_EINVAL, "multigrid coarsest level too large");
   AMT_CHECK(dalloc(p, &p->mg_ainv, (size_t)p->mg_nc * p->mg_nc));
   std::vector<double> G(8 * 576);
   mg_child_tables(Ke, G.data());
   AMT_CHECK(dalloc(p, &p->mg_G, G.size()));
   AMT_CHECK(copy_in(p, p->mg_G, G.data(), G.size() * sizeof(double)));
   return AMT_OK;
-}
-
-// Dense inverse of the coarsest Galerkin operator (host, FP64). Rows that no element reaches
-// are decoupled (unit diagonal); a pivot that rounding drives to <= 1e-13 of its diagonal is
-// decoupled too, so the result stays symmetric positive definite.
-void mg_dense_inverse(const float* st, const MgGrid& g, std::vector<float>& inv) {
-  const int n = (int)(3 * g.n_nodes);
-  std::vector<double> A((size_t)n * n, 0.0);
-  for (int64_t N = 0; N < g.n_nodes; ++N) {
-    const int I = (int)(N / ((int64_t)g.nny * g.nnz)), J = (int)((N / g.nnz) % g.nny),
-              K = (int)(N % g.nnz);
-    for (int d = 0; d < 27; ++d) {
-      const int MI = I + d / 9 - 1, MJ = J + (d / 3) % 3 - 1, MK = K + d % 3 - 1;
-      if (MI < 0 || MI >= g.nnx || MJ < 0 || MJ >= g.nny || MK < 0 || MK >= g.nnz) continue;
-      const int64_t M = ((int64_t)MI * g.nny + MJ) * g.nnz + MK;
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j)
-          A[(size_t)(3 * N + i) * n + 3 * M + j] =
-              d >= 13 ? st[(int64_t)((d - 13) * 9 + 3 * i + j) * g.n_nodes + N]
-                      : st[(int64_t)((13 - d) * 9 + 3 * j + i) * g.n_nodes + M];
-    }
-  }
-  std::vector<double> diag(n);
-  for (int i = 0; i < n; ++i) {
-    diag[i] = A[(size_t)i * n + i];
-    if (!(diag[i] > 0.0)) {
-      for (int j = 0; j < n; ++j) A[(size_t)i * n + j] = A[(size_t)j * n + i] = 0.0;
-      A[(size_t)i * n + i] = diag[i] = 1.0;
-    }
-  }
-  // Cholesky A = L L^T (lower triangle in place)
-  for (int j = 0; j < n; ++j) {
-    double d = A[(size_t)j * n + j];
-    for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
-    if (!(d > 1e-13 * diag[j])) {
-      for (int k = 0; k < j; ++k) A[(size_t)j * n + k] = 0.0;
-      for (int i = j + 1; i < n; ++i) A[(size_t)i * n + j] = 0.0;
-      A[(size_t)j * n + j] = 1.0 / std::sqrt(diag[j]);
-      continue;
-    }
-    const double l = std::sqrt(d);
-    A[(size_t)j * n + j] = l;
-    for (int i = j + 1; i < n; ++i) {
-      double v = A[(size_t)i * n + j];
-      for (int k = 0; k < j; ++k) v -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
-      A[(size_t)i * n + j] = v / l;
-    }
-  }
-  // inverse = L^{-T} L^{-1}: solve for each unit vector
-  inv.assign((size_t)n * n, 0.f);
-  std::vector<double> y(n);
-  for (int c = 0; c < n; ++c) {
-    for (int i = 0; i < n; ++i) {
-      double v = (i == c) ? 1.0 : 0.0;
-      for (int k = 0; k < i; ++k) v -= A[(size_t)i * n + k] * y[k];
-      y[i] = v / A[(size_t)i * n + i];
-    }
-    for (int i = n - 1; i >= 0; --i) {
-      double v = y[i];
-      for (int k = i + 1; k < n; ++k) v -= A[(size_t)k * n + i] * y[k];
-      y[i] = v / A[(size_t)i * n + i];
-    }
-    for (int i = 0; i < n; ++i) inv[(size_t)i * n + c] = (float)y[i];
-  }
 }
 
 // Coarse operators for the current element scales and penalty (after k_dinv).
@@ -389,14 +326,27 @@ int mg_setup(amt_problem* p) {
     hipLaunchKernelGGL(k_mg_stencil_wdinv, dim3(nblk(3 * mg[l].g.n_nodes)), dim3(kBlock), 0,
                        p->stream, mg[l].A, mg[l].g, kMgOmega, kMgTheta, mg[l].wd);
   }
-  AMT_CHECK(check_launch(p, "mg_setup"));
+  // coarsest level: dense inverse on the device (no host round trip, the OC iteration stays
+  // asynchronous)
   const auto& C = mg.back();
-  p->mg_host_stencil.resize((size_t)9 * kStencilBlocks * C.g.n_nodes);
-  AMT_CHECK(copy_out(p, p->mg_host_stencil.data(), C.A,
-                     sizeof(float) * 9 * kStencilBlocks * C.g.n_nodes));
-  std::vector<float> inv;
-  mg_dense_inverse(p->mg_host_stencil.data(), C.g, inv);
-  return copy_in(p, p->mg_ainv, inv.data(), inv.size() * sizeof(float));
+  const int n = p->mg_nc;
+#define AMT_DENSE_INV(R)                                                                        \
+  case R:                                                                                       \
+    hipLaunchKernelGGL(k_mg_dense_inverse<R>, dim3(1), dim3(1024), 0, p->stream, C.A, C.g, n,   \
+                       p->mg_ainv);                                                             \
+    break;
+  switch ((n + 31) / 32) {
+    AMT_DENSE_INV(1)
+    AMT_DENSE_INV(2)
+    AMT_DENSE_INV(3)
+    AMT_DENSE_INV(4)
+    AMT_DENSE_INV(5)
+    AMT_DENSE_INV(6)
+    default:
+      return fail(p, AMT_EINVAL, "multigrid coarsest level too large");
+  }
+#undef AMT_DENSE_INV
+  return check_launch(p, "mg_setup");
 }
 
 // z = V-cycle(r) (FP32 inside); writes partials of z.r and z.q (q may be null) to R_INIT.

@@ -500,7 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__
   mg_prolong_node(xc, cg, xf, fg, n, fixcnt);
 }
 
-// Coarsest level: x = Ainv r (dense, n <= 1024); eight rows per wave, four waves per block.
+// Coarsest level: x = Ainv r (dense, n <= 192); eight rows per wave, four waves per block.
 __global__ __launch_bounds__(kBlock) void k_mg_dense(const float* __restrict__ Ainv, int n,
                                                      const float* __restrict__ r,
                                                      float* __restrict__ x, const int* done) {
@@ -508,6 +508,97 @@ __global__ __launch_bounds__(kBlock) void k_mg_dense(const float* __restrict__ A
   const int row0 = blockIdx.x * 8 * kWaves + (threadIdx.x >> 6);
   if (row0 >= n) return;
   mg_dense_rows8(Ainv, n, r, x, row0, kWaves);
+}
+
+// Dense inverse of the coarsest Galerkin operator, on the device in one 1024-thread block
+// (FP64), written as FP32 row-major n x n. The matrix is assembled from the level's symmetric
+// half stencil (neighbour d >= 13 from the node's own blocks, d < 13 as the transpose of the
+// neighbour's block 13 - d) and inverted by in-place Gauss-Jordan elimination (the sweep
+// operator): for pivot k with p = a_kk, a_ij -= a_ik a_kj / p, a_kj = a_kj / p,
+// a_ik = -a_ik / p, a_kk = 1 / p; after all n pivots the array holds the inverse. No pivoting:
+// the operator is symmetric positive semidefinite, and its pivots are the Schur-complement
+// diagonals of the Cholesky factorisation. A pivot that rounding drives to <= 1e-13 of its
+// original diagonal (a DOF no element stiffens, or a floating piece) decouples its DOF: row and
+// column k are zeroed and the DOF gets the Jacobi value 1 / a_kk (1 if a_kk <= 0), so the
+// result stays symmetric positive definite.
+//
+// Thread (tx, ty) of the 32 x 32 grid holds entries (ty + 32 a, tx + 32 b), a, b < R, in
+// registers; per pivot it reads R values of the pivot column and R of the pivot row from LDS,
+// so a step costs 2R LDS reads and R^2 FMAs per thread and two barriers. n <= 32 R; the padded
+// rows and columns hold the identity and never couple.
+__device__ __forceinline__ double mg_dense_entry(const float* __restrict__ A, const MgGrid& g,
+                                                 int r, int c) {
+  const int64_t N = r / 3, M = c / 3;
+  int I, J, K, MI, MJ, MK;
+  mg_decode(N, g, I, J, K);
+  mg_decode(M, g, MI, MJ, MK);
+  const int di = MI - I, dj = MJ - J, dk = MK - K;
+  if (di < -1 || di > 1 || dj < -1 || dj > 1 || dk < -1 || dk > 1) return 0.0;
+  const int d = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
+  const int ci = r % 3, cj = c % 3;
+  return d >= 13 ? (double)A[((int64_t)(d - 13) * 9 + 3 * ci + cj) * g.n_nodes + N]
+                 : (double)A[((int64_t)(13 - d) * 9 + 3 * cj + ci) * g.n_nodes + M];
+}
+
+template <int R>
+__global__ __launch_bounds__(1024) void k_mg_dense_inverse(const float* __restrict__ A, MgGrid g,
+                                                           int n, float* __restrict__ inv) {
+  __shared__ double rowv[32 * R], colv[32 * R], dg[32 * R];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  double m[R][R];
+#pragma unroll
+  for (int a = 0; a < R; ++a)
+#pragma unroll
+    for (int b = 0; b < R; ++b) {
+      const int i = ty + 32 * a, j = tx + 32 * b;
+      m[a][b] = (i < n && j < n) ? mg_dense_entry(A, g, i, j) : (i == j ? 1.0 : 0.0);
+      if (i == j) dg[i] = m[a][b];
+    }
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    const int ka = k >> 5, kr = k & 31;
+    // pivot row and column (read before any entry of this step changes); the register array
+    // is indexed with compile-time indices only (ka is uniform, so the tests are scalar)
+#pragma unroll
+    for (int a = 0; a < R; ++a)
+      if (a == ka && ty == kr) {
+#pragma unroll
+        for (int b = 0; b < R; ++b) rowv[tx + 32 * b] = m[a][b];
+      }
+#pragma unroll
+    for (int b = 0; b < R; ++b)
+      if (b == ka && tx == kr) {
+#pragma unroll
+        for (int a = 0; a < R; ++a) colv[ty + 32 * a] = m[a][b];
+      }
+    __syncthreads();
+    const double p = colv[k], dk = dg[k];
+    const bool dec = !(p > 1e-13 * dk);
+    const double ip = dec ? 0.0 : 1.0 / p;
+    const double dval = dec ? (dk > 0.0 ? 1.0 / dk : 1.0) : ip;
+#pragma unroll
+    for (int a = 0; a < R; ++a) {
+      const int i = ty + 32 * a;
+      const double c = colv[i] * ip;
+#pragma unroll
+      for (int b = 0; b < R; ++b) {
+        const int j = tx + 32 * b;
+        const double rj = rowv[j];
+        double v = fma(-c, rj, m[a][b]);
+        if (i == k) v = (j == k) ? dval : rj * ip;
+        else if (j == k) v = -c;
+        m[a][b] = v;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < R; ++a)
+#pragma unroll
+    for (int b = 0; b < R; ++b) {
+      const int i = ty + 32 * a, j = tx + 32 * b;
+      if (i < n && j < n) inv[(int64_t)i * n + j] = (float)m[a][b];
+    }
 }
 
 // Level 0 (matrix-free, FP32 s and Ke0, LDS tile): x0 = wd*r, t = r - K x0; r is the FP64
