@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per k_fcg_apply launch on this workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
 # scaled by a plain 256 MiB copy (profiles/fcg_apply_bracket120x60x60_hbm_traffic.md)
-FCG_APPLY_TRAFFIC_BYTES = 49.59e6
+FCG_APPLY_TRAFFIC_BYTES = 49.39e6
 
 
 def log(msg):
