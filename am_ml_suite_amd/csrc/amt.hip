@@ -366,27 +366,26 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q) {
   // each restriction also writes the next level's pre-smoothing step x0 = wd * rhs (not for
   // the dense coarsest level)
   hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[1].g.n_nodes)), dim3(kBlock), 0, p->stream,
-                     mg[0].t, mg[0].g, mg[1].rhs, mg[1].g, L > 2 ? mg[1].wd : nullptr, mg[1].x,
-                     done);
+                     mg[0].t, mg[0].g, mg[1].rhs, mg[1].g, L > 2 ? mg[1].wd : nullptr, mg[1].x);
   for (size_t l = 1; l + 1 < L; ++l) {
     hipLaunchKernelGGL(k_mg_pre, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
-                       mg[l].A, mg[l].g, mg[l].rhs, mg[l].x, mg[l].t, done);
+                       mg[l].A, mg[l].g, mg[l].rhs, mg[l].x, mg[l].t);
     hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[l + 1].g.n_nodes)), dim3(kBlock), 0,
                        p->stream, mg[l].t, mg[l].g, mg[l + 1].rhs, mg[l + 1].g,
-                       l + 2 < L ? mg[l + 1].wd : nullptr, mg[l + 1].x, done);
+                       l + 2 < L ? mg[l + 1].wd : nullptr, mg[l + 1].x);
   }
   hipLaunchKernelGGL(k_mg_dense, dim3((p->mg_nc + 8 * kWaves - 1) / (8 * kWaves)), dim3(kBlock),
-                     0, p->stream, p->mg_ainv, p->mg_nc, mg[L - 1].rhs, mg[L - 1].x, done);
+                     0, p->stream, p->mg_ainv, p->mg_nc, mg[L - 1].rhs, mg[L - 1].x);
   const float* res = mg[L - 1].x;  // correction of the level below the current one
   for (size_t l = L - 2; l >= 1; --l) {
     hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
-                       res, mg[l + 1].g, mg[l].x, mg[l].g, nullptr, done);
+                       res, mg[l + 1].g, mg[l].x, mg[l].g, nullptr);
     hipLaunchKernelGGL(k_mg_post, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
-                       mg[l].A, mg[l].g, mg[l].wd, mg[l].rhs, mg[l].x, mg[l].t, done);
+                       mg[l].A, mg[l].g, mg[l].wd, mg[l].rhs, mg[l].x, mg[l].t);
     res = mg[l].t;
   }
   hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, res,
-                     mg[1].g, mg[0].x, mg[0].g, p->fixcnt, done);
+                     mg[1].g, mg[0].x, mg[0].g, p->fixcnt);
   if (mf)
     hipLaunchKernelGGL(k_mg0_post_mfma, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g,
                        p->tg, p->s32, p->Kh16, p->Kl16, p->fixcnt, p->scal + S_P, mg[0].wd,
