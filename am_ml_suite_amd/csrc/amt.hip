@@ -117,6 +117,7 @@ struct amt_problem {
   float* mg_ainv = nullptr;   // coarsest dense inverse
   int mg_nc = 0;              // coarsest DOFs
   FcgState* fcg = nullptr;
+  double* rr[2] = {nullptr, nullptr};  // flexible-CG residual, alternating (rr[0] is r)
   // one captured chunk of kMgChunk flexible-CG iterations, replayed per chunk
   hipGraph_t fcg_graph = nullptr;
   hipGraphExec_t fcg_exec = nullptr;
@@ -301,6 +302,8 @@ int mg_build_hierarchy(amt_problem* p, const double* Ke) {
     AMT_CHECK(dalloc(p, &L.t, nd));
     p->mg.push_back(L);
   }
+  p->rr[0] = p->r;
+  AMT_CHECK(dalloc(p, &p->rr[1], p->n_dof));
   p->mg_nc = (int)(3 * p->mg.back().g.n_nodes);
   if (p->mg_nc > kMgCoarseDofs) return fail(p, AMT_EINVAL, "multigrid coarsest level too large");
   AMT_CHECK(dalloc(p, &p->mg_ainv, (size_t)p->mg_nc * p->mg_nc));
@@ -350,12 +353,21 @@ int mg_setup(amt_problem* p) {
 }
 
 // z = V-cycle(r) (FP32 inside); writes partials of z.r and z.q (q may be null) to R_INIT.
-void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q) {
+// With r_prev, the flexible-CG update is folded into the level-0 pre-smoothing
+// (k_mg0_pre_upd): r = r_prev - alpha q is formed there, x += alpha p_new, and the |r|^2
+// partials go to R_UPD, one per tile block.
+void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q,
+               const double* r_prev = nullptr, const double* p_new = nullptr) {
   auto& mg = p->mg;
   const int* done = (const int*)((const char*)p->fcg + offsetof(FcgState, done));
   const size_t L = mg.size();
   const bool mf = p->cfg.precision == AMT_PREC_MIXED_BF16;
-  if (mf)
+  if (r_prev)
+    hipLaunchKernelGGL(k_mg0_pre_upd, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r_prev,
+                       (double*)r, q, p->u, p_new, p->g, p->tg, p->s32, p->Ke32, p->fixcnt,
+                       p->scal + S_P, mg[0].wd, mg[0].x, mg[0].t, p->fcg,
+                       (const double*)p->part(R_APPLY), (int)p->nb_tile, p->part(R_UPD));
+  else if (mf)
     hipLaunchKernelGGL(k_mg0_pre_mfma, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g,
                        p->tg, p->s32, p->Kh16, p->Kl16, p->fixcnt, p->scal + S_P, mg[0].wd,
                        mg[0].x, mg[0].t, done);
@@ -401,6 +413,7 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q) {
 void fcg_enqueue_chunk(amt_problem* p, bool eager_prof) {
   const int nba = (int)p->nb_tile, nbu = (int)nred(p->n_dof);
   int* done_flag = (int*)(p->scal + S_FLAG);
+  const bool fused = p->cfg.precision != AMT_PREC_MIXED_BF16;  // FP32 level-0 smoothing
   for (int t = 0; t < kMgChunk; ++t) {
     hipEvent_t ev = nullptr;
     if (eager_prof && p->prof_on && p->ev_used + 2 <= (int)p->ev_pool.size()) {
@@ -411,13 +424,19 @@ void fcg_enqueue_chunk(amt_problem* p, bool eager_prof) {
                        p->p[(t + 1) & 1], p->q, p->s64, p->Ke64, p->fixcnt, p->scal + S_P,
                        p->g, p->tg, p->fcg, p->part(R_APPLY));
     if (ev) prof_end(p, ev);
-    hipLaunchKernelGGL(k_fcg_update, dim3(nbu), dim3(kBlock), 0, p->stream, p->u, p->r,
-                       p->p[(t + 1) & 1], p->q, p->n_dof, p->fcg,
-                       (const double*)p->part(R_APPLY), nba, p->part(R_UPD));
-    mg_vcycle(p, p->r, p->z, p->q);
+    if (fused) {
+      // the update runs inside the level-0 pre-smoothing; r alternates between two buffers
+      mg_vcycle(p, p->rr[(t + 1) & 1], p->z, p->q, p->rr[t & 1], p->p[(t + 1) & 1]);
+    } else {
+      hipLaunchKernelGGL(k_fcg_update, dim3(nbu), dim3(kBlock), 0, p->stream, p->u, p->r,
+                         p->p[(t + 1) & 1], p->q, p->n_dof, p->fcg,
+                         (const double*)p->part(R_APPLY), nba, p->part(R_UPD));
+      mg_vcycle(p, p->r, p->z, p->q);
+    }
     hipLaunchKernelGGL(k_fcg_scalar, dim3(1), dim3(kBlock), 0, p->stream, p->fcg,
-                       (const double*)p->part(R_UPD), nbu, (const double*)p->part(R_APPLY),
-                       nba, (const double*)p->part(R_INIT), nba, p->stride_red, 0, done_flag);
+                       (const double*)p->part(R_UPD), fused ? nba : nbu,
+                       (const double*)p->part(R_APPLY), nba, (const double*)p->part(R_INIT), nba,
+                       p->stride_red, 0, done_flag);
   }
 }
 

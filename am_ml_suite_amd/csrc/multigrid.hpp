@@ -840,6 +840,70 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(double* __restrict__ x,
   store_partials<1>(acc, upart, gridDim.x);
 }
 
+// Level-0 pre-smoothing with the flexible-CG update folded in (one launch for k_fcg_update +
+// k_mg0_pre): alpha = rz / p.q from the operator launch's partials; the tile loads r and q and
+// forms r' = r - alpha q (k_fcg_update's fma) at every halo node, so that x0 = wd r' and
+// t = r' - K x0 are k_mg0_pre's. The tile's own nodes write r' to rnew, a second buffer, because
+// other blocks still read r in their halos. They also write x += alpha p and block partials of
+// |r'|^2, one per block. The halo loads follow the partials behind alpha (their values are
+// transformed as they arrive) and the own-node loads follow the element loop, so few
+// registers stay live and 8 waves fit per SIMD (no spill at the 64-VGPR limit).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_mg0_pre_upd(
+    const double* __restrict__ r, double* __restrict__ rnew, const double* __restrict__ qv,
+    double* __restrict__ x, const double* __restrict__ pv, GridDims g, TileGrid tg,
+    const float* __restrict__ s, const float* __restrict__ Ke,
+    const uint8_t* __restrict__ fixcnt, const double* __restrict__ P,
+    const float* __restrict__ wd, float* __restrict__ x0, float* __restrict__ t,
+    const FcgState* st, const double* __restrict__ apart, int nba, double* __restrict__ upart) {
+  __shared__ TileSmem<float> sm;
+  int i0 = 0, j0 = 0, k0 = 0;
+  const bool has = tile_origin(tg, i0, j0, k0);
+  int li, lj, lk;
+  tile_thread(li, lj, lk);
+  const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+  const bool own = has && i <= g.nx && j <= g.ny && k <= g.nz;
+  const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
+  float fc[3] = {0.f, 0.f, 0.f};
+  if (own) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fc[c] = (float)fixcnt[3 * n + c];
+  }
+  double pq[1];
+  reduce_partials<1>(apart, nba, nba, pq);
+  if (st->done) return;  // uniform over the grid, before any barrier
+  const double a = st->rz / pq[0];
+  if (has)
+    tile_load(sm, g, i0, j0, k0,
+              [&](int64_t q) { return wd[q] * (float)fma(-a, qv[q], r[q]); }, s);
+  __syncthreads();
+  double acc[1] = {0.0};
+  if (own) {
+    float y[3], xs[3];
+    tile_apply(sm, li, lj, lk, Ke, y, xs);
+    double ro[3], qo[3], xo[3], po[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      ro[c] = r[3 * n + c];
+      qo[c] = qv[3 * n + c];
+      xo[c] = x[3 * n + c];
+      po[c] = pv[3 * n + c];
+    }
+    const float pen = (float)(*P);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t q = 3 * n + c;
+      const double rn = fma(-a, qo[c], ro[c]);
+      rnew[q] = rn;
+      x[q] = fma(a, po[c], xo[c]);
+      acc[0] = fma(rn, rn, acc[0]);
+      y[c] = fmaf(fc[c] * pen, xs[c], y[c]);
+      x0[q] = xs[c];
+      t[q] = fc[c] > 0.f ? 0.f : (float)rn - y[c];  // clamped residuals are not restricted
+    }
+  }
+  store_partials<1>(acc, upart, gridDim.x);
+}
+
 // One block after the V-cycle: convergence test on |r|^2, then the flexible (Polak-Ribiere /
 // Notay) beta = -(z.q)/(p.q) and rz = z.r. first=1: only rz from the first V-cycle.
 __global__ __launch_bounds__(kBlock) void k_fcg_scalar(FcgState* st,

@@ -84,38 +84,48 @@ constexpr int kSPer = (kETile + kBlock - 1) / kBlock;  // element scales per thr
 // Thread slots are whole halo nodes (three values each): the slot's tile coordinates are
 // decoded once per node, not once per value, and its global offset is a 32-bit offset from
 // the tile's (uniform, 64-bit) base, so the per-slot address arithmetic stays small.
+// Halo slot t of this thread: its first DOF q and whether the node lies in the grid.
+__device__ __forceinline__ bool tile_slot(const GridDims& g, int i0, int j0, int k0, int t,
+                                          int64_t& q) {
+  const int64_t nbase = ((int64_t)(i0 - 1) * g.nny + (j0 - 1)) * g.nnz + (k0 - 1);
+  const int node = threadIdx.x + t * kBlock;
+  const int hk = node % kHK, hj = (node / kHK) % kHJ, hi = node / (kHK * kHJ);
+  const int i = i0 + hi - 1, j = j0 + hj - 1, k = k0 + hk - 1;
+  q = 3 * (nbase + (hi * g.nny + hj) * g.nnz + hk);
+  return node < kHalo && i >= 0 && i <= g.nx && j >= 0 && j <= g.ny && k >= 0 && k <= g.nz;
+}
+
+// Element-scale slot t of this thread (0 outside the grid).
+template <typename T>
+__device__ __forceinline__ T tile_scale(const GridDims& g, int i0, int j0, int k0, int t,
+                                        const T* __restrict__ s) {
+  const int64_t ebase = ((int64_t)(i0 - 1) * g.ny + (j0 - 1)) * g.nz + (k0 - 1);
+  const int q = threadIdx.x + t * kBlock;
+  const int ek = q % kEK, ej = (q / kEK) % kEJ, ei = q / (kEK * kEJ);
+  const int i = i0 + ei - 1, j = j0 + ej - 1, k = k0 + ek - 1;
+  const bool ok = q < kETile && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
+  return ok ? s[ebase + (ei * g.ny + ej) * g.nz + ek] : T(0);
+}
+
 template <typename T, typename F>
 __device__ __forceinline__ void tile_gather(T (&v)[kNPer][3], T (&sv)[kSPer], const GridDims& g,
                                             int i0, int j0, int k0, F f,
                                             const T* __restrict__ s) {
-  const int64_t nbase = ((int64_t)(i0 - 1) * g.nny + (j0 - 1)) * g.nnz + (k0 - 1);
 #pragma unroll
   for (int t = 0; t < kNPer; ++t) {
-    const int node = threadIdx.x + t * kBlock;
-    const int hk = node % kHK, hj = (node / kHK) % kHJ, hi = node / (kHK * kHJ);
-    const int i = i0 + hi - 1, j = j0 + hj - 1, k = k0 + hk - 1;
-    const bool ok = node < kHalo && i >= 0 && i <= g.nx && j >= 0 && j <= g.ny && k >= 0 &&
-                    k <= g.nz;
-    const int64_t q = 3 * (nbase + (hi * g.nny + hj) * g.nnz + hk);
+    int64_t q;
+    const bool ok = tile_slot(g, i0, j0, k0, t, q);
 #pragma unroll
     for (int c = 0; c < 3; ++c) v[t][c] = ok ? f(q + c) : T(0);
   }
-  const int64_t ebase = ((int64_t)(i0 - 1) * g.ny + (j0 - 1)) * g.nz + (k0 - 1);
 #pragma unroll
-  for (int t = 0; t < kSPer; ++t) {
-    const int q = threadIdx.x + t * kBlock;
-    const int ek = q % kEK, ej = (q / kEK) % kEJ, ei = q / (kEK * kEJ);
-    const int i = i0 + ei - 1, j = j0 + ej - 1, k = k0 + ek - 1;
-    const bool ok = q < kETile && i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
-    sv[t] = ok ? s[ebase + (ei * g.ny + ej) * g.nz + ek] : T(0);
-  }
+  for (int t = 0; t < kSPer; ++t) sv[t] = tile_scale(g, i0, j0, k0, t, s);
 }
 
-template <typename T, typename F>
-__device__ __forceinline__ void tile_load(TileSmem<T>& sm, const GridDims& g, int i0, int j0,
-                                          int k0, F f, const T* __restrict__ s) {
-  T v[kNPer][3], sv[kSPer];
-  tile_gather(v, sv, g, i0, j0, k0, f, s);
+// LDS stores of gathered halo values and element scales (the layout of TileSmem).
+template <typename T>
+__device__ __forceinline__ void tile_store(TileSmem<T>& sm, const T (&v)[kNPer][3],
+                                           const T (&sv)[kSPer]) {
 #pragma unroll
   for (int t = 0; t < kNPer; ++t) {
     const int node = threadIdx.x + t * kBlock;
@@ -130,6 +140,14 @@ __device__ __forceinline__ void tile_load(TileSmem<T>& sm, const GridDims& g, in
     const int q = threadIdx.x + t * kBlock;
     if (q < kETile) sm.s[(q / kEK) * kRowE<T> + q % kEK] = sv[t];
   }
+}
+
+template <typename T, typename F>
+__device__ __forceinline__ void tile_load(TileSmem<T>& sm, const GridDims& g, int i0, int j0,
+                                          int k0, F f, const T* __restrict__ s) {
+  T v[kNPer][3], sv[kSPer];
+  tile_gather(v, sv, g, i0, j0, k0, f, s);
+  tile_store(sm, v, sv);
 }
 
 // This thread's node inside the tile: a wave is one i-plane, its four 16-lane rows hold
