@@ -51,6 +51,10 @@ constexpr int kOcBlocks = 512;  // OC sweep grid (partials per candidate)
 constexpr int kOcCand = (1 << kOcLevels) - 1;
 constexpr int kMaxReduceBlocks = 1024;
 constexpr int kChunk = 32;  // PCG iterations launched between two host polls
+// Largest tile grid for which the flexible-CG update runs inside the level-0 pre-smoothing
+// (k_mg0_pre_upd): 2048 blocks each reading 2048 partials is 33 MB of L2 reads per launch; at
+// 256x128x128 (19,305 tiles) it would be 3 GB, so larger grids keep k_fcg_update.
+constexpr unsigned kFusedUpdateTiles = 2048;
 constexpr int kMgChunk = 4;  // flexible-CG iterations per captured graph (even)
 constexpr int kNnPatience = 16;  // learned FCG: chunks allowed without a 10% |r| drop
 constexpr double kNnGainMomentum = 0.05;  // running means behind the correction's weight
@@ -413,7 +417,9 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q,
 void fcg_enqueue_chunk(amt_problem* p, bool eager_prof) {
   const int nba = (int)p->nb_tile, nbu = (int)nred(p->n_dof);
   int* done_flag = (int*)(p->scal + S_FLAG);
-  const bool fused = p->cfg.precision != AMT_PREC_MIXED_BF16;  // FP32 level-0 smoothing
+  // FP32 level-0 smoothing, and few enough tile blocks that every block of the fused launch can
+  // reduce all the operator partials itself (nb_tile^2 doubles of L2 reads per launch)
+  const bool fused = p->cfg.precision != AMT_PREC_MIXED_BF16 && p->nb_tile <= kFusedUpdateTiles;
   for (int t = 0; t < kMgChunk; ++t) {
     hipEvent_t ev = nullptr;
     if (eager_prof && p->prof_on && p->ev_used + 2 <= (int)p->ev_pool.size()) {
