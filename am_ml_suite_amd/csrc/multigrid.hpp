@@ -779,7 +779,9 @@ __global__ void k_fcg_start(FcgState* st, const double* rr, const double* bb, do
 }
 
 // p = z + beta p_old (formed once per node in the LDS tile); q = K p (FP64); partials p.q.
-__global__ __launch_bounds__(kBlock) void k_fcg_apply(
+// Held to 64 VGPRs (8 waves per SIMD, no spill): with its 19 KB of LDS, 8 blocks fit per CU, so
+// the C2 grid (1,984 tiles) is resident in one round instead of 7 blocks per CU and a tail.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_fcg_apply(
     const double* __restrict__ z, const double* __restrict__ pold, double* __restrict__ pnew,
     double* __restrict__ q, const double* __restrict__ s, const double* __restrict__ Ke,
     const uint8_t* __restrict__ fixcnt, const double* __restrict__ P, GridDims g, TileGrid tg,
