@@ -780,7 +780,8 @@ __global__ void k_fcg_start(FcgState* st, const double* rr, const double* bb, do
 
 // p = z + beta p_old (formed once per node in the LDS tile); q = K p (FP64); partials p.q.
 // Held to 64 VGPRs (8 waves per SIMD, no spill): with its 19 KB of LDS, 8 blocks fit per CU, so
-// the C2 grid (1,984 tiles) is resident in one round instead of 7 blocks per CU and a tail.
+// the 120x60x60 grid (1,984 tiles) is resident in one round instead of 7 blocks per CU and a
+// tail.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_fcg_apply(
     const double* __restrict__ z, const double* __restrict__ pold, double* __restrict__ pnew,
     double* __restrict__ q, const double* __restrict__ s, const double* __restrict__ Ke,
