@@ -328,7 +328,10 @@ int mg_setup(amt_problem* p) {
                      mg[1].A);
   for (size_t l = 1; l < mg.size(); ++l) {
     if (l >= 2)
-      hipLaunchKernelGGL(k_mg_coarsen, dim3(nblk(mg[l].g.n_nodes), kStencilBlocks, 3), dim3(kBlock), 0,
+      hipLaunchKernelGGL(k_mg_coarsen,
+                         dim3((unsigned)((mg[l].g.n_nodes + kCoarsenNodes - 1) / kCoarsenNodes),
+                              kStencilBlocks, 3),
+                         dim3(9 * kCoarsenNodes), 0,
                          p->stream, mg[l - 1].A, mg[l - 1].g, mg[l].g, mg[l].A);
     hipLaunchKernelGGL(k_mg_stencil_wdinv, dim3(nblk(3 * mg[l].g.n_nodes)), dim3(kBlock), 0,
                        p->stream, mg[l].A, mg[l].g, kMgOmega, kMgTheta, mg[l].wd);

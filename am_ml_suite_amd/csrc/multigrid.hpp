@@ -141,17 +141,15 @@ __global__ __launch_bounds__(kBlock) void k_mg_galerkin1(const double* __restric
 // compile time; fine nodes outside the grid (and a neighbour M outside the coarse grid) are read
 // at a clamped position with weight 0, so all loads issue without a branch. The terms are
 // summed in the order m (x, y, z), then m' (x, y, z).
-template <int DI, int DJ, int DK>
+template <int DI, int DJ, int DK, int AX, int AY>
 __device__ __forceinline__ void coarsen_block(const float* __restrict__ Af, const MgGrid& fg,
                                               const MgGrid& cg, int I, int J, int K, int i,
                                               float (&acc)[3]) {
   const int64_t nn = fg.n_nodes;
   const int MI = I + DI, MJ = J + DJ, MK = K + DK;
   const bool okM = MI >= 0 && MI < cg.nnx && MJ >= 0 && MJ < cg.nny && MK >= 0 && MK < cg.nnz;
-#pragma unroll
-  for (int ax = -1; ax <= 1; ++ax)
-#pragma unroll
-    for (int ay = -1; ay <= 1; ++ay)
+  {
+    constexpr int ax = AX, ay = AY;
 #pragma unroll
       for (int az = -1; az <= 1; ++az) {
         const int mx = 2 * I + ax, my = 2 * J + ay, mz = 2 * K + az;
@@ -194,37 +192,78 @@ __device__ __forceinline__ void coarsen_block(const float* __restrict__ Af, cons
           }
         }
       }
+  }
 }
 
-// Level l+1 from level l: thread (coarse N, offset D = 13 + blockIdx.y, row i = blockIdx.z);
-// the offset is uniform in the block and selects the unrolled term list.
-__global__ __launch_bounds__(kBlock) void k_mg_coarsen(const float* __restrict__ Af, MgGrid fg,
-                                                       MgGrid cg, float* __restrict__ Ac) {
-  const int64_t N = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (N >= cg.n_nodes) return;
-  const int i = blockIdx.z;
-  int I, J, K;
-  mg_decode(N, cg, I, J, K);
-  float acc[3] = {0.f, 0.f, 0.f};
-  switch (blockIdx.y) {
-    case 0: coarsen_block<0, 0, 0>(Af, fg, cg, I, J, K, i, acc); break;
-    case 1: coarsen_block<0, 0, 1>(Af, fg, cg, I, J, K, i, acc); break;
-    case 2: coarsen_block<0, 1, -1>(Af, fg, cg, I, J, K, i, acc); break;
-    case 3: coarsen_block<0, 1, 0>(Af, fg, cg, I, J, K, i, acc); break;
-    case 4: coarsen_block<0, 1, 1>(Af, fg, cg, I, J, K, i, acc); break;
-    case 5: coarsen_block<1, -1, -1>(Af, fg, cg, I, J, K, i, acc); break;
-    case 6: coarsen_block<1, -1, 0>(Af, fg, cg, I, J, K, i, acc); break;
-    case 7: coarsen_block<1, -1, 1>(Af, fg, cg, I, J, K, i, acc); break;
-    case 8: coarsen_block<1, 0, -1>(Af, fg, cg, I, J, K, i, acc); break;
-    case 9: coarsen_block<1, 0, 0>(Af, fg, cg, I, J, K, i, acc); break;
-    case 10: coarsen_block<1, 0, 1>(Af, fg, cg, I, J, K, i, acc); break;
-    case 11: coarsen_block<1, 1, -1>(Af, fg, cg, I, J, K, i, acc); break;
-    case 12: coarsen_block<1, 1, 0>(Af, fg, cg, I, J, K, i, acc); break;
-    default: coarsen_block<1, 1, 1>(Af, fg, cg, I, J, K, i, acc); break;
+// Level l+1 from level l: block (64 coarse nodes, offset D = 13 + blockIdx.y, row i =
+// blockIdx.z), wave w taking the fine nodes 2N + a with (a_x, a_y) = (w / 3 - 1, w % 3 - 1)
+// (the offset and the wave's share are uniform in the wave and select the unrolled term list).
+// Nine waves per output cut the serial load chain of each thread to a ninth; their partial
+// sums are added in LDS in fixed order.
+constexpr int kCoarsenNodes = 64;
+template <int DI, int DJ, int DK>
+__device__ __forceinline__ void coarsen_part(int w, const float* __restrict__ Af,
+                                             const MgGrid& fg, const MgGrid& cg, int I, int J,
+                                             int K, int i, float (&acc)[3]) {
+  switch (w) {
+    case 0: coarsen_block<DI, DJ, DK, -1, -1>(Af, fg, cg, I, J, K, i, acc); break;
+    case 1: coarsen_block<DI, DJ, DK, -1, 0>(Af, fg, cg, I, J, K, i, acc); break;
+    case 2: coarsen_block<DI, DJ, DK, -1, 1>(Af, fg, cg, I, J, K, i, acc); break;
+    case 3: coarsen_block<DI, DJ, DK, 0, -1>(Af, fg, cg, I, J, K, i, acc); break;
+    case 4: coarsen_block<DI, DJ, DK, 0, 0>(Af, fg, cg, I, J, K, i, acc); break;
+    case 5: coarsen_block<DI, DJ, DK, 0, 1>(Af, fg, cg, I, J, K, i, acc); break;
+    case 6: coarsen_block<DI, DJ, DK, 1, -1>(Af, fg, cg, I, J, K, i, acc); break;
+    case 7: coarsen_block<DI, DJ, DK, 1, 0>(Af, fg, cg, I, J, K, i, acc); break;
+    default: coarsen_block<DI, DJ, DK, 1, 1>(Af, fg, cg, I, J, K, i, acc); break;
   }
+}
+__device__ __forceinline__ void coarsen_wave(int D, int w, const float* __restrict__ Af,
+                                             const MgGrid& fg, const MgGrid& cg, int I, int J,
+                                             int K, int i, float (&acc)[3]) {
+  switch (D) {
+    case 0: coarsen_part<0, 0, 0>(w, Af, fg, cg, I, J, K, i, acc); break;
+    case 1: coarsen_part<0, 0, 1>(w, Af, fg, cg, I, J, K, i, acc); break;
+    case 2: coarsen_part<0, 1, -1>(w, Af, fg, cg, I, J, K, i, acc); break;
+    case 3: coarsen_part<0, 1, 0>(w, Af, fg, cg, I, J, K, i, acc); break;
+    case 4: coarsen_part<0, 1, 1>(w, Af, fg, cg, I, J, K, i, acc); break;
+    case 5: coarsen_part<1, -1, -1>(w, Af, fg, cg, I, J, K, i, acc); break;
+    case 6: coarsen_part<1, -1, 0>(w, Af, fg, cg, I, J, K, i, acc); break;
+    case 7: coarsen_part<1, -1, 1>(w, Af, fg, cg, I, J, K, i, acc); break;
+    case 8: coarsen_part<1, 0, -1>(w, Af, fg, cg, I, J, K, i, acc); break;
+    case 9: coarsen_part<1, 0, 0>(w, Af, fg, cg, I, J, K, i, acc); break;
+    case 10: coarsen_part<1, 0, 1>(w, Af, fg, cg, I, J, K, i, acc); break;
+    case 11: coarsen_part<1, 1, -1>(w, Af, fg, cg, I, J, K, i, acc); break;
+    case 12: coarsen_part<1, 1, 0>(w, Af, fg, cg, I, J, K, i, acc); break;
+    default: coarsen_part<1, 1, 1>(w, Af, fg, cg, I, J, K, i, acc); break;
+  }
+}
+
+__global__ __launch_bounds__(9 * kCoarsenNodes) void k_mg_coarsen(const float* __restrict__ Af,
+                                                                   MgGrid fg, MgGrid cg,
+                                                                   float* __restrict__ Ac) {
+  __shared__ float part[9][kCoarsenNodes][3];
+  const int w = threadIdx.x / kCoarsenNodes, l = threadIdx.x % kCoarsenNodes;
+  const int64_t N = (int64_t)blockIdx.x * kCoarsenNodes + l;
+  const bool ok = N < cg.n_nodes;
+  const int i = blockIdx.z;
+  float acc[3] = {0.f, 0.f, 0.f};
+  if (ok) {
+    int I, J, K;
+    mg_decode(N, cg, I, J, K);
+    coarsen_wave((int)blockIdx.y, w, Af, fg, cg, I, J, K, i, acc);
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) part[w][l][j] = acc[j];
+  __syncthreads();
+  if (w || !ok) return;
   const int d = blockIdx.y;  // stored block index D - 13
 #pragma unroll
-  for (int j = 0; j < 3; ++j) Ac[(int64_t)(d * 9 + 3 * i + j) * cg.n_nodes + N] = acc[j];
+  for (int j = 0; j < 3; ++j) {
+    float v = part[0][l][j];
+#pragma unroll
+    for (int u = 1; u < 9; ++u) v += part[u][l][j];
+    Ac[(int64_t)(d * 9 + 3 * i + j) * cg.n_nodes + N] = v;
+  }
 }
 
 // Smoother weights. Damped Jacobi w_i = omega / A_ii diverges once omega * lambda_max(D^-1 A)
