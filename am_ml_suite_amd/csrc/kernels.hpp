@@ -448,6 +448,19 @@ __global__ __launch_bounds__(kBlock) void k_oc_sweep(const double* __restrict__ 
   store_partials_many<NC>(vol, partials, stride);
 }
 
+// End-of-iteration scalars gathered for one host read: OC state, compliance, |b - K u|^2.
+struct IterScalars {
+  OcState oc;
+  double compl_, rr;
+};
+__global__ void k_iter_scalars(const OcState* __restrict__ oc, const double* __restrict__ compl_,
+                               const double* __restrict__ rr, IterScalars* __restrict__ out) {
+  if (threadIdx.x) return;
+  out->oc = *oc;
+  out->compl_ = *compl_;
+  out->rr = *rr;
+}
+
 // Bisection passes with a known outcome. A design element with dc < 0 sits at its floor
 // max(rho_min, rho - move) once rho sqrt(-dc / (lambda + 1e-20)) <= that floor, i.e. for
 // lambda + 1e-20 >= rho^2 (-dc) / floor^2, and at its ceiling once the candidate reaches
