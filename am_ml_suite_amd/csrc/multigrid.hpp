@@ -295,8 +295,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_wd0(const double* __restrict__ s,
   node_decode(n, g, i, j, k);
   double s8[8];
   load_element_scales(s, g, i, j, k, s8);
-  // neighbour block d = (di, dj, dk) gathers the element pairs (a, b) with b - a = d
+  // neighbour block d = (di, dj, dk) gathers the element pairs (a, b) with b - a = d; unrolled,
+  // so the pair tests and the Ke0 offsets are compile-time (64 pairs of 3 FMAs)
   double rowabs = 0.0, diag = 0.0;
+#pragma unroll
   for (int d = 0; d < 27; ++d) {
     const int di = d / 9 - 1, dj = (d / 3) % 3 - 1, dk = d % 3 - 1;
     double blk[3] = {0.0, 0.0, 0.0};
@@ -601,7 +603,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_dense(const float* __restrict__ A
 //
 // Thread (tx, ty) of the 32 x 32 grid holds entries (ty + 32 a, tx + 32 b), a, b < R, in
 // registers; per pivot it reads R values of the pivot column and R of the pivot row from LDS,
-// so a step costs 2R LDS reads and R^2 FMAs per thread and two barriers. n <= 32 R; the padded
+// so a step costs 2R LDS reads and R^2 FMAs per thread and one barrier. n <= 32 R; the padded
 // rows and columns hold the identity and never couple.
 __device__ __forceinline__ double mg_dense_entry(const float* __restrict__ A, const MgGrid& g,
                                                  int r, int c) {
@@ -620,7 +622,9 @@ __device__ __forceinline__ double mg_dense_entry(const float* __restrict__ A, co
 template <int R>
 __global__ __launch_bounds__(1024) void k_mg_dense_inverse(const float* __restrict__ A, MgGrid g,
                                                            int n, float* __restrict__ inv) {
-  __shared__ double rowv[32 * R], colv[32 * R], dg[32 * R];
+  // pivot row / column, double-buffered by pivot parity: one barrier per pivot suffices, since
+  // the buffer written for pivot k + 2 was last read before every thread passed pivot k + 1's
+  __shared__ double rowb[2][32 * R], colb[2][32 * R], dg[32 * R];
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   double m[R][R];
 #pragma unroll
@@ -634,6 +638,8 @@ __global__ __launch_bounds__(1024) void k_mg_dense_inverse(const float* __restri
   __syncthreads();
   for (int k = 0; k < n; ++k) {
     const int ka = k >> 5, kr = k & 31;
+    double* rowv = rowb[k & 1];
+    double* colv = colb[k & 1];
     // pivot row and column (read before any entry of this step changes); the register array
     // is indexed with compile-time indices only (ka is uniform, so the tests are scalar)
 #pragma unroll
@@ -678,7 +684,6 @@ __global__ __launch_bounds__(1024) void k_mg_dense_inverse(const float* __restri
 #pragma unroll
       for (int b = 0; b < R; ++b)
         if (a == ka && b == ka && ty == kr && tx == kr) m[a][b] = dval;
-    __syncthreads();
   }
 #pragma unroll
   for (int a = 0; a < R; ++a)
