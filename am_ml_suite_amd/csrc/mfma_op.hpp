@@ -4,7 +4,7 @@
 // Element-batched formulation. A 256-thread block owns the same 4x4x16 node tile as the vector
 // operator (tile_op.hpp) and the 5x5x17 = 425 elements touching it. The element products
 // Y = Ke0 X (X: 24 x 425, the elements' nodal vectors) are one GEMM of 24 x 24 by 24 x 425:
-// with 16 x 16 x 16 tiles that is 2 (rows) x 2 (k) x 27 (element groups) MFMAs, about
+// with 16 x 16 x 16 tiles that is 2 (row halves) x 2 (k) x 27 (element groups) MFMAs, about
 // 1/16 of the 576 FMAs per node of the vector operator in issue slots. The halo is staged in
 // LDS, the products are scaled by s_e and written to LDS, and each thread then sums the rows of
 // its node from its (up to) eight elements in a fixed order.
@@ -36,7 +36,11 @@ __device__ __forceinline__ uint16_t mf_bf16(float f) {
 }
 __device__ __forceinline__ float mf_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-constexpr int kYS = 25;  // LDS row stride (floats) of one element's 24 products (odd: banks)
+// The element products are formed in two halves of 12 rows (corners 0-3, then 4-7), each from
+// one 16-row MFMA tile starting at row 12h: the same 2 x 2 x 3 MFMAs per element group as two
+// fixed row tiles, but the LDS product buffer holds 12 rows per element instead of 24 (22 KB
+// instead of 43 KB per block: 5 blocks per CU instead of 3).
+constexpr int kYS = 13;  // LDS row stride (floats) of one element's 12 products (odd: banks)
 constexpr int kEGroups = (kETile + 15) / 16;  // 27 groups of 16 elements
 
 struct MfmaSmem {
@@ -45,7 +49,8 @@ struct MfmaSmem {
   float y[kETile * kYS];
 };
 
-// The lane's Ke0 fragments: [row tile][k step], high and low BF16 parts.
+// The lane's Ke0 fragments: [half][k step], high and low BF16 parts; half h holds rows
+// 12h .. 12h + 15 (rows past 23 are zero).
 struct KeFrag {
   mf_bf16x4 h[2][2], l[2][2];
 };
@@ -60,7 +65,7 @@ __device__ __forceinline__ void load_kefrag(KeFrag& f, const uint16_t* __restric
     for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int m = mt * 16 + (lane & 15), k = ks * 16 + 4 * (lane >> 4) + e;
+        const int m = mt * 12 + (lane & 15), k = ks * 16 + 4 * (lane >> 4) + e;
         const bool ok = m < 24 && k < 24;
         f.h[mt][ks][e] = ok ? (short)Kh[m * 24 + k] : (short)0;
         f.l[mt][ks][e] = ok ? (short)Kl[m * 24 + k] : (short)0;
@@ -93,9 +98,9 @@ __device__ __forceinline__ void mfma_tile_load(MfmaSmem& sm, const GridDims& g, 
   }
 }
 
-// Element products s_e * Ke0 x_e of the tile's 405 elements into sm.y (after mfma_tile_load
-// and a barrier; the caller syncs again before reading sm.y).
-__device__ __forceinline__ void mfma_tile_products(MfmaSmem& sm, const KeFrag& kf) {
+// Rows 12h .. 12h + 11 of the element products s_e * Ke0 x_e of the tile's 425 elements into
+// sm.y (after mfma_tile_load and a barrier; the caller syncs again before reading sm.y).
+__device__ __forceinline__ void mfma_tile_products(MfmaSmem& sm, const KeFrag& kf, int half) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col = lane & 15, kq = lane >> 4;
   // LDS offsets (relative to the element's first node) of the lane's 8 k values
@@ -126,35 +131,50 @@ __device__ __forceinline__ void mfma_tile_products(MfmaSmem& sm, const KeFrag& k
         bl[ks][e] = ok ? (short)sm.xl[base + koff[ks][e]] : (short)0;
       }
     const float se = ev ? sm.s[en] : 0.f;
+    mf_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
-      mf_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < 2; ++ks) {
+      const mf_bf16x4 kh = half ? kf.h[1][ks] : kf.h[0][ks];
+      const mf_bf16x4 kl = half ? kf.l[1][ks] : kf.l[0][ks];
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(kh, bh[ks], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(kh, bl[ks], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(kl, bh[ks], acc, 0, 0, 0);
+    }
 #pragma unroll
-      for (int ks = 0; ks < 2; ++ks) {
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(kf.h[mt][ks], bh[ks], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(kf.h[mt][ks], bl[ks], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(kf.l[mt][ks], bh[ks], acc, 0, 0, 0);
-      }
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int m = mt * 16 + 4 * kq + e;
-        if (ev && m < 24) sm.y[en * kYS + m] = se * acc[e];
-      }
+    for (int e = 0; e < 4; ++e) {
+      const int m = 4 * kq + e;  // row 12 half + m of Ke0
+      if (ev && m < 12) sm.y[en * kYS + m] = se * acc[e];
     }
   }
 }
 
-// The thread's node (tile position li, lj, lk): y[c] = sum over its 8 elements (fixed order).
+// The thread's node (tile position li, lj, lk): y[c] += its rows from the elements where it is
+// corner 4 half .. 4 half + 3 (over both halves: all 8 elements, in corner order).
 __device__ __forceinline__ void mfma_node_sum(const MfmaSmem& sm, int li, int lj, int lk,
-                                              float (&y)[3]) {
+                                              int half, float (&y)[3]) {
   const int eb = ((li + 1) * kEJ + (lj + 1)) * kEK + (lk + 1);
-  y[0] = y[1] = y[2] = 0.f;
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    const float* row = sm.y + (eb - (kOX[a] * kEJ + kOY[a]) * kEK - kOZ[a]) * kYS + 3 * a;
+  for (int u = 0; u < 4; ++u) {
+    const int a = 4 * half + u;
+    const float* row = sm.y + (eb - (kOX[a] * kEJ + kOY[a]) * kEK - kOZ[a]) * kYS + 3 * u;
     y[0] += row[0];
     y[1] += row[1];
     y[2] += row[2];
+  }
+}
+
+// y = sum_e s_e Ke0 x_e at the thread's node, after mfma_tile_load and a barrier: both halves
+// of the products, each summed into y (every thread of the block calls it; has: the block
+// has a tile).
+__device__ __forceinline__ void mfma_tile_operator(MfmaSmem& sm, const KeFrag& kf, bool has,
+                                                   int li, int lj, int lk, float (&y)[3]) {
+  y[0] = y[1] = y[2] = 0.f;
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    if (half) __syncthreads();  // every node sum of the first half is done
+    if (has) mfma_tile_products(sm, kf, half);
+    __syncthreads();
+    if (has) mfma_node_sum(sm, li, lj, lk, half, y);
   }
 }
 
@@ -174,15 +194,13 @@ __global__ __launch_bounds__(kBlock) void k_apply_mfma(const float* __restrict__
   load_kefrag(kf, Kh, Kl);
   mfma_tile_load(sm, g, i0, j0, k0, [&](int64_t q) { return x[q]; }, s);
   __syncthreads();
-  mfma_tile_products(sm, kf);
-  __syncthreads();
   int li, lj, lk;
   tile_thread(li, lj, lk);
+  float yv[3];
+  mfma_tile_operator(sm, kf, true, li, lj, lk, yv);
   const int i = i0 + li, j = j0 + lj, k = k0 + lk;
   if (i > g.nx || j > g.ny || k > g.nz) return;
   const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
-  float yv[3];
-  mfma_node_sum(sm, li, lj, lk, yv);
   const float pen = P ? (float)(*P) : 0.f;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -216,16 +234,14 @@ __global__ __launch_bounds__(kBlock) void k_pcg_apply_mfma(
   if (has)
     mfma_tile_load(sm, g, i0, j0, k0, [&](int64_t qq) { return fmaf(beta, pold[qq], z[qq]); }, s);
   __syncthreads();
-  if (has) mfma_tile_products(sm, kf);
-  __syncthreads();
   int li, lj, lk;
   tile_thread(li, lj, lk);
+  float yv[3];
+  mfma_tile_operator(sm, kf, has, li, lj, lk, yv);
   const int i = i0 + li, j = j0 + lj, k = k0 + lk;
   double acc[1] = {0.0};
   if (has && i <= g.nx && j <= g.ny && k <= g.nz) {
     const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
-    float yv[3];
-    mfma_node_sum(sm, li, lj, lk, yv);
     const float pen = (float)(*P);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -255,15 +271,13 @@ __global__ __launch_bounds__(kBlock) void k_mg0_pre_mfma(
   load_kefrag(kf, Kh, Kl);
   mfma_tile_load(sm, g, i0, j0, k0, [&](int64_t q) { return wd[q] * (float)r[q]; }, s);
   __syncthreads();
-  mfma_tile_products(sm, kf);
-  __syncthreads();
   int li, lj, lk;
   tile_thread(li, lj, lk);
+  float y[3];
+  mfma_tile_operator(sm, kf, true, li, lj, lk, y);
   const int i = i0 + li, j = j0 + lj, k = k0 + lk;
   if (i > g.nx || j > g.ny || k > g.nz) return;
   const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
-  float y[3];
-  mfma_node_sum(sm, li, lj, lk, y);
   const float pen = (float)(*P);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -290,16 +304,14 @@ __global__ __launch_bounds__(kBlock) void k_mg0_post_mfma(
   load_kefrag(kf, Kh, Kl);
   if (has) mfma_tile_load(sm, g, i0, j0, k0, [&](int64_t qq) { return x[qq]; }, s);
   __syncthreads();
-  if (has) mfma_tile_products(sm, kf);
-  __syncthreads();
   int li, lj, lk;
   tile_thread(li, lj, lk);
+  float y[3];
+  mfma_tile_operator(sm, kf, has, li, lj, lk, y);
   const int i = i0 + li, j = j0 + lj, k = k0 + lk;
   double acc[2] = {0.0, 0.0};
   if (has && i <= g.nx && j <= g.ny && k <= g.nz) {
     const int64_t n = ((int64_t)i * g.nny + j) * g.nnz + k;
-    float y[3];
-    mfma_node_sum(sm, li, lj, lk, y);
     const float pen = (float)(*P);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
