@@ -31,6 +31,7 @@ EXPORTS = [
     "amt_batch_run", "amt_unet_collect", "amt_unet_train",
     "amt_simp_create", "amt_simp_destroy", "amt_simp_add_load_case", "amt_simp_set_density",
     "amt_simp_get_density", "amt_simp_step", "amt_simp_filter", "amt_simp_get_displacement",
+    "amt_oc_update_noalloc",
 ]
 
 
@@ -167,6 +168,8 @@ def load():
     L.amt_simp_step.argtypes = [P, C.POINTER(AmtSimpIteration)]
     L.amt_simp_filter.argtypes = [P, _dp, _dp, C.c_int64]
     L.amt_simp_get_displacement.argtypes = [P, C.c_int, _dp, C.c_int64]
+    L.amt_oc_update_noalloc.argtypes = [P, P, P, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                        P, P]
     for name in EXPORTS:
         if name not in ("amt_config_default", "amt_last_error"):
             getattr(L, name).restype = C.c_int

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <type_traits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1602,3 +1603,78 @@ int amt_profile_read(amt_problem* p, int64_t* launches, double* total_ms) {
 }
 
 }  // extern "C"
+
+// ---- OC update on caller buffers, without allocation ----------------------------------------
+// The kernels' scratch (sweep and bound partials, final partials, bisection state) lives in one
+// file-scope device array. Its host-side name is only a stand-in for the symbol: kernels get the
+// device address that hipGetSymbolAddress returns, looked up once per device and kept.
+namespace {
+constexpr int kNaStride = kOcBlocks;                           // partials per value
+constexpr size_t kNaSweep = (size_t)kOcCand * kNaStride;       // 63 candidate volumes (>= 4
+                                                               // bound values)
+constexpr size_t kNaFinal = 2 * (size_t)kNaStride;             // max |drho|, volume (count)
+constexpr size_t kNaState = kNaSweep + kNaFinal;               // OcState offset, in doubles
+constexpr size_t kNaDoubles = kNaState + (sizeof(OcState) + 7) / 8;
+static_assert(kOcCand >= 4, "the bound partials share the sweep region");
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mutex;
+double* g_na_ptr[kNaMaxDevices] = {nullptr};
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNaDoubles];
+
+extern "C" int amt_oc_update_noalloc(double* rho, const double* dcf, const uint8_t* design,
+                                     int64_t n_e, double volume_fraction, double move_limit,
+                                     double rho_min, double* result, void* stream) {
+  if (!rho || !dcf || !design || !result)
+    return set_global_error(AMT_EINVAL, "amt_oc_update_noalloc: null buffer");
+  if (n_e <= 0 || n_e > AMT_OC_NOALLOC_MAX_ELEMENTS)
+    return set_global_error(AMT_EINVAL,
+                            "amt_oc_update_noalloc: n_e = " + std::to_string(n_e) +
+                                " outside [1, 2^31] (the limit of the static scratch)");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNaMaxDevices)
+    return set_global_error(AMT_EHIP, "amt_oc_update_noalloc: no usable HIP device");
+  double* S = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_na_mutex);
+    if (!g_na_ptr[dev]) {
+      void* ptr = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !ptr)
+        return set_global_error(AMT_EHIP, std::string("amt_oc_update_noalloc: "
+                                                      "hipGetSymbolAddress: ") +
+                                              hipGetErrorString(e));
+      g_na_ptr[dev] = (double*)ptr;
+    }
+    S = g_na_ptr[dev];
+  }
+  hipStream_t s = (hipStream_t)stream;
+  double* part = S;                      // bound partials, then the sweep's
+  double* fin = S + kNaSweep;            // design count, then the final partials
+  OcState* st = (OcState*)(S + kNaState);
+  const unsigned nb = std::min<unsigned>(nred(n_e), kOcBlocks);
+  hipLaunchKernelGGL(k_design_count, dim3(nb), dim3(kBlock), 0, s, design, n_e, fin, kNaStride);
+  hipLaunchKernelGGL(k_oc_state_init, dim3(1), dim3(kBlock), 0, s, st, (const double*)fin,
+                     (int)nb, kNaStride, n_e, volume_fraction);
+  hipLaunchKernelGGL(k_oc_bounds, dim3(nb), dim3(kBlock), 0, s, rho, dcf, design, n_e,
+                     move_limit, rho_min, part, kNaStride);
+  hipLaunchKernelGGL(k_oc_bounds_final, dim3(1), dim3(kBlock), 0, s, st, (const double*)part,
+                     (int)nb, kNaStride);
+  for (int sw = 0; sw < (100 + kOcLevels - 1) / kOcLevels + 1; ++sw) {
+    hipLaunchKernelGGL((k_oc_sweep<kOcLevels>), dim3(nb), dim3(kBlock), 0, s, rho, dcf, design,
+                       n_e, move_limit, rho_min, st, part, kNaStride);
+    hipLaunchKernelGGL((k_oc_walk<kOcLevels>), dim3(1), dim3(kBlock), 0, s, st,
+                       (const double*)part, (int)nb, kNaStride);
+  }
+  hipLaunchKernelGGL(k_oc_final, dim3(nb), dim3(kBlock), 0, s, rho, dcf, design, n_e,
+                     move_limit, rho_min, st, fin, kNaStride);
+  hipLaunchKernelGGL(k_oc_final_reduce, dim3(1), dim3(kBlock), 0, s, st, (const double*)fin,
+                     (int)nb, kNaStride);
+  hipLaunchKernelGGL(k_oc_state_out, dim3(1), dim3(64), 0, s, (const OcState*)st, result);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return set_global_error(AMT_EHIP, std::string("amt_oc_update_noalloc: launch: ") +
+                                          hipGetErrorString(e));
+  return AMT_OK;
+}

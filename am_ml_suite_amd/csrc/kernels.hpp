@@ -614,6 +614,52 @@ __global__ __launch_bounds__(kBlock) void k_oc_final_reduce(OcState* st,
   }
 }
 
+// Partial counts of a caller-supplied design mask (the OC update on caller buffers).
+__global__ __launch_bounds__(kBlock) void k_design_count(const uint8_t* __restrict__ design,
+                                                         int64_t n, double* partials,
+                                                         int stride) {
+  double acc[1] = {0.0};
+  AMT_GRID_STRIDE(e, n) acc[0] += design[e] ? 1.0 : 0.0;
+  store_partials<1>(acc, partials, stride);
+}
+
+// One block: the bisection's starting state [1e-20, 1e20] with the design count reduced from
+// k_design_count's partials (an empty mask counts every element, topology_optimizer.cpp:222-225).
+// Every field is written, so nothing of a previous call survives in the state.
+__global__ __launch_bounds__(kBlock) void k_oc_state_init(OcState* st, const double* partials,
+                                                          int nb, int stride, int64_t n,
+                                                          double vf) {
+  double c[1];
+  reduce_partials<1>(partials, nb, stride, c);
+  if (threadIdx.x) return;
+  OcState s;
+  s.l1 = 1e-20;
+  s.l2 = 1e20;
+  s.lmid = 0.0;
+  s.vf = vf;
+  s.n_design = c[0] > 0.0 ? c[0] : (double)n;
+  s.change = 0.0;
+  s.vol_sum = 0.0;
+  s.pad = 0.0;
+  s.passes = 0;
+  s.done = 0;
+  s.pad1 = s.pad2 = 0;
+  s.ff_hi = INFINITY;
+  s.ff_lo = -INFINITY;
+  s.vol_hi = s.vol_lo = 0.0;
+  *st = s;
+}
+
+// out[0] = max |drho|, out[1] = design volume sum, out[2] = bisection passes, out[3] = design
+// element count.
+__global__ void k_oc_state_out(const OcState* __restrict__ st, double* __restrict__ out) {
+  if (threadIdx.x || blockIdx.x) return;
+  out[0] = st->change;
+  out[1] = st->vol_sum;
+  out[2] = (double)st->passes;
+  out[3] = st->n_design;
+}
+
 // Design mask from the initial density (topology_optimizer.cpp:419-428); partial counts.
 __global__ __launch_bounds__(kBlock) void k_design_mask(const double* __restrict__ rho,
                                                         uint8_t* __restrict__ design,

@@ -283,6 +283,26 @@ int amt_simp_filter(amt_simp* h, const double* x, double* y, int64_t n_e);
 /* Displacement of load case lc from the last step. */
 int amt_simp_get_displacement(amt_simp* h, int lc, double* u, int64_t n_dof);
 
+/* ---- OC update without allocation ----
+ * No counterpart in the reference. For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. The same update and
+ * kernels as amt_oc_update (updateDensitiesOC, topology_optimizer.cpp:205-287), on caller
+ * device buffers and a caller stream (NULL: the null stream), enqueued without a host sync:
+ *   rho[n_e]    device, in/out: densities (slot order), updated in place;
+ *   dcf[n_e]    device: filtered sensitivities;
+ *   design[n_e] device: 1 for design elements (the mask amt_begin_run builds: rho_min + 1e-6 <
+ *               rho < 0.99 at the run's start); an all-zero mask counts every element;
+ *   result[4]   device, out: max |drho|, design volume sum, bisection passes, design count.
+ * Scratch (partials and bisection state, ~266 KB) is one static device array per GPU, held
+ * until the process ends; nothing is allocated. n_e is limited to AMT_OC_NOALLOC_MAX_ELEMENTS
+ * (larger: AMT_EINVAL). Every call on one GPU shares that array, so calls that overlap on one
+ * GPU (other streams or threads) race; calls on one stream are ordered. Errors: amt_last_error
+ * (NULL). */
+#define AMT_OC_NOALLOC_MAX_ELEMENTS ((int64_t)1 << 31)
+int amt_oc_update_noalloc(double* rho, const double* dcf, const uint8_t* design, int64_t n_e,
+                          double volume_fraction, double move_limit, double rho_min,
+                          double* result, void* stream);
+
 /* Wait for all work of this handle. */
 int amt_sync(amt_problem* p);
 /* Device name / CU count of the handle's GPU. */
