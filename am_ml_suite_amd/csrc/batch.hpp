@@ -588,7 +588,8 @@ __global__ void k_b_oc_init(const BProb* __restrict__ D, int np,
   oc[pi] = s;
 }
 
-// Candidate volumes for the next LEV bisection levels (as k_oc_sweep), element-item partials.
+// Candidate volumes for the next LEV bisection levels (as k_oc_sweep: the reference's own
+// per-element expression), element-item partials.
 template <int LEV>
 __global__ __launch_bounds__(kBlock) void k_b_oc_sweep(const BProb* __restrict__ D,
                                                        const int* __restrict__ ip, int n,
@@ -598,15 +599,14 @@ __global__ __launch_bounds__(kBlock) void k_b_oc_sweep(const BProb* __restrict__
                                                        const OcState* __restrict__ oc,
                                                        double* __restrict__ partials) {
   constexpr int NC = (1 << LEV) - 1;
-  __shared__ double gfac[NC];
+  __shared__ double lmid[NC];
   AMT_ITEMS(w, n) {
     const int pi = ip[w];
     const OcState st = oc[pi];
     if (st.done) continue;
     const BProb P = D[pi];
-    __syncthreads();  // the previous item's readers of gfac are done
-    if (threadIdx.x < NC)
-      gfac[threadIdx.x] = 1.0 / sqrt(oc_tree_mid(st.l1, st.l2, threadIdx.x + 1) + 1e-20);
+    __syncthreads();  // the previous item's readers of lmid are done
+    if (threadIdx.x < NC) lmid[threadIdx.x] = oc_tree_mid(st.l1, st.l2, threadIdx.x + 1);
     __syncthreads();
     double vol[NC];
 #pragma unroll
@@ -616,12 +616,9 @@ __global__ __launch_bounds__(kBlock) void k_b_oc_sweep(const BProb* __restrict__
       const int64_t e = base + l * kBlock + threadIdx.x;
       if (e >= P.g.n_elem) break;
       if (!design[P.e_off + e]) continue;
-      const double r = rho[P.e_off + e];
-      const double rs = r * sqrt(-dcf[P.e_off + e]);
-      const double lo = r - P.move, hi = r + P.move;
+      const double r = rho[P.e_off + e], d = dcf[P.e_off + e];
 #pragma unroll
-      for (int c = 0; c < NC; ++c)
-        vol[c] += std_max(P.rho_min, std_max(lo, std_min(1.0, std_min(hi, rs * gfac[c]))));
+      for (int c = 0; c < NC; ++c) vol[c] += oc_candidate(r, d, lmid[c], P.move, P.rho_min);
     }
     store_partial_at<NC>(vol, partials, n, w);
   }

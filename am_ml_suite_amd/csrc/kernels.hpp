@@ -416,10 +416,11 @@ __global__ void k_filter_list(int64_t e, const StencilEntry* __restrict__ st,
 // updates are the reference's, in ~LEV times fewer grid-wide reductions.
 // ---------------------------------------------------------------------------------------
 
-// Candidate volumes for the 2^LEV - 1 midpoints of the next LEV bisection levels. For the
-// volume sums the candidate uses rho * sqrt(-dc) / sqrt(lambda + 1e-20) (one sqrt per element,
-// one multiply per candidate); k_oc_final recomputes the density at the chosen lambda with
-// the reference's own expression.
+// Candidate volumes for the 2^LEV - 1 midpoints of the next LEV bisection levels. Every
+// candidate is the reference's own expression, rho * sqrt(-dc / (lambda + 1e-20)) clamped
+// (topology_optimizer.cpp:249-256), with IEEE division and square root (correctly rounded on
+// the device as on the host), so each element's candidate is bitwise the reference's; only the
+// order of the volume sum differs (a fixed tree here, an OpenMP static reduction there).
 template <int LEV>
 __global__ __launch_bounds__(kBlock) void k_oc_sweep(const double* __restrict__ rho,
                                                      const double* __restrict__ dcf,
@@ -429,21 +430,17 @@ __global__ __launch_bounds__(kBlock) void k_oc_sweep(const double* __restrict__ 
                                                      int stride) {
   if (st->done) return;
   constexpr int NC = (1 << LEV) - 1;
-  __shared__ double gfac[NC];
-  if (threadIdx.x < NC)
-    gfac[threadIdx.x] = 1.0 / sqrt(oc_tree_mid(st->l1, st->l2, threadIdx.x + 1) + 1e-20);
+  __shared__ double lmid[NC];
+  if (threadIdx.x < NC) lmid[threadIdx.x] = oc_tree_mid(st->l1, st->l2, threadIdx.x + 1);
   __syncthreads();
   double vol[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) vol[c] = 0.0;
   AMT_GRID_STRIDE(e, n) {
     if (!design[e]) continue;
-    const double r = rho[e];
-    const double rs = r * sqrt(-dcf[e]);
-    const double lo = r - move, hi = r + move;
+    const double r = rho[e], d = dcf[e];
 #pragma unroll
-    for (int c = 0; c < NC; ++c)
-      vol[c] += std_max(rho_min, std_max(lo, std_min(1.0, std_min(hi, rs * gfac[c]))));
+    for (int c = 0; c < NC; ++c) vol[c] += oc_candidate(r, d, lmid[c], move, rho_min);
   }
   store_partials_many<NC>(vol, partials, stride);
 }

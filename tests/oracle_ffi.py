@@ -227,3 +227,21 @@ def bracket_fixed(nx, ny, nz):
     f = np.zeros(n, dtype=np.int64)
     lib().orc_bracket_fixed(nx, ny, nz, f.ctypes.data)
     return f
+
+
+def oc_tie_case():
+    """One design element whose first bisection pass is an exact tie: lambda_1 = (1e-20 + 1e20)/2,
+    dc = -4 (lambda_1 + 1e-20), so rho sqrt(-dc / (lambda_1 + 1e-20)) = 0.375 * 2 = 0.75 exactly
+    (a power-of-two quotient and an exact square root) and vf = 0.75. The reference's strict test
+    current_vf > vf_target (topology_optimizer.cpp:262-266) is false, so l2 = lambda_1 and every
+    later pass evaluates a smaller lambda: the final density is above 0.75. Computing the
+    candidate as rho sqrt(-dc) / sqrt(lambda + 1e-20) gives 0.7500000000000001 there, flips the
+    first decision and ends below 0.75."""
+    n = 16
+    lam1 = 0.5 * (1e-20 + 1e20)
+    rho = np.ones(n)                 # non-design (>= 0.99)
+    dcf = -np.linspace(1.0, 2.0, n)
+    rho[5] = 0.375
+    dcf[5] = -4.0 * (lam1 * 1.0 + 1e-20)
+    design = ((rho > 0.01 + 1e-6) & (rho < 0.99)).astype(np.uint8)
+    return rho, dcf, design, 0.75, 0.5, 0.01  # rho, dcf, design, vf, move, rho_min

@@ -173,6 +173,26 @@ def test_oc_update_matches_oracle(vf):
     assert ch == pytest.approx(chr_, abs=1e-15)
 
 
+def test_oc_update_exact_tie_matches_oracle():
+    """A bisection pass that lands exactly on vf (oracle_ffi.oc_tie_case): each element's
+    candidate is the reference's rho * sqrt(-dc / (lambda + 1e-20)), bitwise, so the device takes
+    the tie to l2 as the reference does and ends with the oracle's pass count and density,
+    above 0.75. (rho sqrt(-dc) / sqrt(lambda + 1e-20) gives 0.7500000000000001 at that pass,
+    flips it and ends ~4e-5 lower.)"""
+    rho, dcf, design, vf, move, rho_min = O.oc_tie_case()
+    opt = _opt(4, 2, 2, volume_fraction=vf, move_limit=move, rho_min=rho_min)
+    opt.set_density(rho)
+    opt.begin_run()
+    ch, passes = opt.oc_update(dcf)
+    got = opt.get_density()
+    ref, chr_, pr, _ = O.oc_update(rho, dcf, design, vf, move, rho_min)
+    print("tie: passes gpu %d oracle %d, rho %.17g / %.17g" % (passes, pr, got[5], ref[5]))
+    assert passes == pr
+    assert got[5] > 0.75
+    assert np.array_equal(got, ref)
+    assert ch == chr_
+
+
 # ---- full OC loop ---------------------------------------------------------------------------
 def test_full_run_cantilever_50_iterations():
     """30x10x10 cantilever, 50 OC iterations, Jacobi-PCG tol 1e-8, consistent density index:

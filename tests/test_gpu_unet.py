@@ -132,3 +132,36 @@ def test_learned_preconditioner_gain():
     assert (u @ ph.F) == pytest.approx(uref @ ph.F, rel=1e-6)
     print("held-out solve: learned FCG %d iterations (fallback %s) vs Jacobi-PCG %d"
           % (st["iterations"], st["fallback"], sref["iterations"]))
+
+
+@pytest.mark.parametrize("gain", [-1e6, 1e9])
+def test_learned_preconditioner_forced_fallback(gain):
+    """A corrupt correction weight makes M(r) = D^-1 r + gain * rms * net(.) indefinite (or
+    dominated by an untrained network): the learned flexible CG stalls or loses positive
+    curvature, and plain FP64 Jacobi-PCG takes over from the current iterate
+    (linear_solver.cpp:148-152,175-180 report non-convergence and continue; here the host
+    decides the hand-over). The fallback must be reported, reach the 1e-8 FP64 true residual and
+    agree with plain Jacobi-PCG on the compliance within 1e-6; the reported iteration count
+    includes the flexible-CG passes before the hand-over."""
+    p = A.bracket_like_problem(40, 20, 20, seed=4)
+    # a bounded iteration budget: a stall that the host failed to detect ends the test quickly
+    opt = A.TopologyOptimizer(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3),
+                              make_config(p, preconditioner="unet", solver_max_iterations=4000))
+    opt.set_loads(p.F, p.fixed)
+    opt.set_density(p.rho0)
+    net = A.UNet(optimizer=opt, seed=7)
+    net.gain = gain
+    opt.attach_unet(net)
+    u, st = opt.solve()
+    ref, _ = _bracket_opt(4, "jacobi")
+    uref, sref = ref.solve()
+    print("gain %g: fallback %d, %d iterations (Jacobi-PCG alone %d), true residual %.2e"
+          % (gain, st["fallback"], st["iterations"], sref["iterations"],
+             st["true_rel_residual"]))
+    assert st["fallback"] == 1
+    assert st["converged"] and st["true_rel_residual"] <= 1.5e-8, st
+    assert st["iterations"] > 0 and st["outer_iterations"] == 2
+    assert (u @ p.F) == pytest.approx(uref @ p.F, rel=1e-6)
+    opt.close()
+    ref.close()
+    net.close()

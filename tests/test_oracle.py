@@ -300,3 +300,16 @@ def test_notebook_iteration0_compliance():
                         max_iterations=1, pcg_tol=1e-12, pcg_maxit=50000)
     _, hist, _, _ = O.run(cfg, rho0, F, fixed)
     assert hist[0]["compliance"] == pytest.approx(G["compliance_iter0"], rel=1e-9)
+
+
+def test_oc_exact_tie_goes_to_l2():
+    """The bisection's first pass lands exactly on vf (oracle_ffi.oc_tie_case): the strict
+    comparison sends the tie to l2, so the final lambda lies below the first midpoint and the
+    element ends above 0.75; the move limit (0.375 + 0.5) and the bisection tolerance bound it."""
+    rho, dcf, design, vf, move, rho_min = O.oc_tie_case()
+    new, ch, npass, lam = O.oc_update(rho, dcf, design, vf, move, rho_min)
+    lam1 = 0.5 * (1e-20 + 1e20)
+    assert lam < lam1
+    assert 0.75 < new[5] < 0.75 * (1 + 1e-4)
+    assert np.array_equal(np.delete(new, 5), np.delete(rho, 5))  # non-design unchanged
+    assert 10 <= npass <= 20
