@@ -31,7 +31,7 @@ EXPORTS = [
     "amt_batch_run", "amt_unet_collect", "amt_unet_train",
     "amt_simp_create", "amt_simp_destroy", "amt_simp_add_load_case", "amt_simp_set_density",
     "amt_simp_get_density", "amt_simp_step", "amt_simp_filter", "amt_simp_get_displacement",
-    "amt_oc_update_noalloc",
+    "amt_oc_update_noalloc", "amt_operator_info", "amt_operator_timings",
 ]
 
 
@@ -46,7 +46,7 @@ class AmtConfig(C.Structure):
         ("pcg_tol", C.c_double), ("pcg_maxit", C.c_int), ("penalty_factor", C.c_double),
         ("density_index", C.c_int), ("precision", C.c_int), ("precond", C.c_int),
         ("warm_start", C.c_int), ("inner_tol", C.c_double), ("inner_maxit", C.c_int),
-        ("verbose", C.c_int), ("use_graphs", C.c_int),
+        ("verbose", C.c_int), ("use_graphs", C.c_int), ("operator_tile", C.c_int),
     ]
 
 
@@ -170,6 +170,10 @@ def load():
     L.amt_simp_get_displacement.argtypes = [P, C.c_int, _dp, C.c_int64]
     L.amt_oc_update_noalloc.argtypes = [P, P, P, C.c_int64, C.c_double, C.c_double, C.c_double,
                                         P, P]
+    L.amt_operator_info.argtypes = [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_double)]
+    L.amt_operator_timings.argtypes = [P, C.c_int, _dp, C.c_int]
     for name in EXPORTS:
         if name not in ("amt_config_default", "amt_last_error"):
             getattr(L, name).restype = C.c_int

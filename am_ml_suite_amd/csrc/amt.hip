@@ -17,7 +17,9 @@
 #include <cstdio>
 #include <cstring>
 #include <type_traits>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -26,6 +28,7 @@
 #include "kernels.hpp"
 #include "multigrid.hpp"
 #include "mfma_op.hpp"
+#include "stream_kernels.hpp"
 #include "unet_api.hpp"
 #include "unet_glue.hpp"
 #include <cstddef>
@@ -65,6 +68,13 @@ constexpr float kMgTheta = 1.9f;  // ...capped per row at theta / sum_j |A_ij| (
 // one block with up to 6 x 6 register-resident entries per thread (n <= 192)
 constexpr int64_t kMgCoarseDofs = 192;
 
+// Level-0 operator candidates: 0 = the 4 x 4 x 16 node gather tile (tile_op.hpp); 1 + 3 s + l =
+// the element-column streaming tile (stream_op.hpp) of shape kStreamShape[s] (threads along j,
+// k) with kStreamPlanes[l] node planes per block.
+constexpr int kStreamShape[3][2] = {{16, 16}, {8, 32}, {4, 64}};
+constexpr int kStreamPlanes[3] = {4, 8, 16};
+constexpr int kOpCandidates = 1 + 3 * 3;
+
 // Partials regions (doubles) inside one buffer; see amt_problem::part()
 enum Region { R_APPLY = 0, R_UPD, R_INIT, R_OC, R_MISC, R_COUNT };
 
@@ -80,6 +90,20 @@ struct amt_problem {
   GridDims g;
   TileGrid tg;
   unsigned nb_tile = 0;  // blocks of the LDS-tiled operator kernels
+  // level-0 operator launch chosen per kernel (autotuned on the first multigrid solve)
+  struct OpSel {
+    int cand = 0;    // index into the candidates above
+    int shape = -1;  // stream shape, -1 for the gather tile
+    StreamGrid sg;
+    unsigned nb = 0;
+    double us = 0.0;  // measured launch time (0 if forced or untuned)
+  };
+  OpSel op_apply, op_upd, op_post;  // FP64 CG operator / FP32 pre-smoothing / FP32 post
+  bool ops_tuned = false;
+  double tune_us[3][kOpCandidates] = {{0.0}};  // each candidate's measured time (0: not run)
+  HadK<double> had64;
+  HadK<float> had32;
+  FcgState* tune_st = nullptr;
   int64_t n_e = 0, n_n = 0, n_dof = 0;
   hipStream_t stream = nullptr;
   std::string err;
@@ -365,6 +389,241 @@ int mg_setup(amt_problem* p) {
   return check_launch(p, "mg_setup");
 }
 
+// ---- level-0 operator launches: gather tile or streaming tile, chosen per kernel ----
+
+amt_problem::OpSel make_op_sel(const amt_problem* p, int cand) {
+  amt_problem::OpSel s;
+  s.cand = cand;
+  if (cand <= 0) {
+    s.cand = 0;
+    s.shape = -1;
+    s.nb = p->nb_tile;
+    return s;
+  }
+  s.shape = (cand - 1) / 3;
+  s.sg = make_stream_grid(p->g, kStreamShape[s.shape][0], kStreamShape[s.shape][1],
+                          kStreamPlanes[(cand - 1) % 3]);
+  s.nb = stream_blocks(s.sg);
+  return s;
+}
+
+// KER<TJ, TK> for the selection's streaming shape, on the handle's stream.
+#define AMT_STREAM_LAUNCH(SEL, KER, ...)                                                       \
+  do {                                                                                         \
+    switch ((SEL).shape) {                                                                     \
+      case 0:                                                                                  \
+        hipLaunchKernelGGL((KER<16, 16>), dim3((SEL).nb), dim3(kBlock), 0, p->stream,          \
+                           __VA_ARGS__);                                                       \
+        break;                                                                                 \
+      case 1:                                                                                  \
+        hipLaunchKernelGGL((KER<8, 32>), dim3((SEL).nb), dim3(kBlock), 0, p->stream,           \
+                           __VA_ARGS__);                                                       \
+        break;                                                                                 \
+      default:                                                                                 \
+        hipLaunchKernelGGL((KER<4, 64>), dim3((SEL).nb), dim3(kBlock), 0, p->stream,           \
+                           __VA_ARGS__);                                                       \
+        break;                                                                                 \
+    }                                                                                          \
+  } while (0)
+
+// y = K x (+ penalty with P) in FP64 / FP32 with the selection's launch.
+void launch_apply64(amt_problem* p, const amt_problem::OpSel& s, const double* x, double* y,
+                    const double* P) {
+  if (s.shape < 0)
+    hipLaunchKernelGGL((k_apply<double, double>), dim3(s.nb), dim3(kBlock), 0, p->stream, x, y,
+                       p->s64, p->Ke64, p->fixcnt, P, p->g, p->tg);
+  else if (s.shape == 0)
+    hipLaunchKernelGGL((k_apply_s<double, 16, 16>), dim3(s.nb), dim3(kBlock), 0, p->stream, x,
+                       y, p->s64, p->had64, p->fixcnt, P, p->g, s.sg);
+  else if (s.shape == 1)
+    hipLaunchKernelGGL((k_apply_s<double, 8, 32>), dim3(s.nb), dim3(kBlock), 0, p->stream, x,
+                       y, p->s64, p->had64, p->fixcnt, P, p->g, s.sg);
+  else
+    hipLaunchKernelGGL((k_apply_s<double, 4, 64>), dim3(s.nb), dim3(kBlock), 0, p->stream, x,
+                       y, p->s64, p->had64, p->fixcnt, P, p->g, s.sg);
+}
+void launch_apply32(amt_problem* p, const amt_problem::OpSel& s, const float* x, float* y,
+                    const double* P) {
+  if (s.shape < 0)
+    hipLaunchKernelGGL((k_apply<float, float>), dim3(s.nb), dim3(kBlock), 0, p->stream, x, y,
+                       p->s32, p->Ke32, p->fixcnt, P, p->g, p->tg);
+  else if (s.shape == 0)
+    hipLaunchKernelGGL((k_apply_s<float, 16, 16>), dim3(s.nb), dim3(kBlock), 0, p->stream, x, y,
+                       p->s32, p->had32, p->fixcnt, P, p->g, s.sg);
+  else if (s.shape == 1)
+    hipLaunchKernelGGL((k_apply_s<float, 8, 32>), dim3(s.nb), dim3(kBlock), 0, p->stream, x, y,
+                       p->s32, p->had32, p->fixcnt, P, p->g, s.sg);
+  else
+    hipLaunchKernelGGL((k_apply_s<float, 4, 64>), dim3(s.nb), dim3(kBlock), 0, p->stream, x, y,
+                       p->s32, p->had32, p->fixcnt, P, p->g, s.sg);
+}
+
+// r = Fb - K x with block partials of |r|^2 into R_APPLY; returns the block count.
+unsigned launch_residual(amt_problem* p, const double* x, double* rout) {
+  const auto& s = p->op_apply;
+  if (s.shape < 0)
+    hipLaunchKernelGGL(k_residual, dim3(s.nb), dim3(kBlock), 0, p->stream, x, p->Fb, rout,
+                       p->s64, p->Ke64, p->fixcnt, p->scal + S_P, p->g, p->tg, p->part(R_APPLY));
+  else
+    AMT_STREAM_LAUNCH(s, k_residual_s, x, p->Fb, rout, p->s64, p->had64, p->fixcnt,
+                      p->scal + S_P, p->g, s.sg, p->part(R_APPLY));
+  return s.nb;
+}
+
+// p_new = z + beta p_old, q = K p_new, partials p.q into R_APPLY (flexible CG).
+void launch_fcg_apply(amt_problem* p, const amt_problem::OpSel& s, const double* z,
+                      const double* pold, double* pnew, const FcgState* st) {
+  if (s.shape < 0)
+    hipLaunchKernelGGL(k_fcg_apply, dim3(s.nb), dim3(kBlock), 0, p->stream, z, pold, pnew, p->q,
+                       p->s64, p->Ke64, p->fixcnt, p->scal + S_P, p->g, p->tg, st,
+                       p->part(R_APPLY));
+  else
+    AMT_STREAM_LAUNCH(s, k_fcg_apply_s, z, pold, pnew, p->q, p->s64, p->had64, p->fixcnt,
+                      p->scal + S_P, p->g, s.sg, st, p->part(R_APPLY));
+}
+
+void launch_mg0_pre_upd(amt_problem* p, const amt_problem::OpSel& s, const double* r_prev,
+                        double* rnew, const double* q, double* x, const double* pnew,
+                        const FcgState* st, int nba) {
+  auto& L0 = p->mg[0];
+  if (s.shape < 0)
+    hipLaunchKernelGGL(k_mg0_pre_upd, dim3(s.nb), dim3(kBlock), 0, p->stream, r_prev, rnew, q, x,
+                       pnew, p->g, p->tg, p->s32, p->Ke32, p->fixcnt, p->scal + S_P, L0.wd,
+                       L0.x, L0.t, st, (const double*)p->part(R_APPLY), nba, p->part(R_UPD));
+  else
+    AMT_STREAM_LAUNCH(s, k_mg0_pre_upd_s, r_prev, rnew, q, x, pnew, p->g, s.sg, p->s32,
+                      p->had32, p->fixcnt, p->scal + S_P, L0.wd, L0.x, L0.t, st,
+                      (const double*)p->part(R_APPLY), nba, p->part(R_UPD));
+}
+
+void launch_mg0_pre(amt_problem* p, const amt_problem::OpSel& s, const double* r,
+                    const int* done) {
+  auto& L0 = p->mg[0];
+  if (s.shape < 0)
+    hipLaunchKernelGGL(k_mg0_pre, dim3(s.nb), dim3(kBlock), 0, p->stream, r, p->g, p->tg,
+                       p->s32, p->Ke32, p->fixcnt, p->scal + S_P, L0.wd, L0.x, L0.t, done);
+  else
+    AMT_STREAM_LAUNCH(s, k_mg0_pre_s, r, p->g, s.sg, p->s32, p->had32, p->fixcnt,
+                      p->scal + S_P, L0.wd, L0.x, L0.t, done);
+}
+
+void launch_mg0_post(amt_problem* p, const amt_problem::OpSel& s, const double* r, double* z,
+                     const double* q, double* partials, int stride, const int* done) {
+  auto& L0 = p->mg[0];
+  if (s.shape < 0)
+    hipLaunchKernelGGL(k_mg0_post, dim3(s.nb), dim3(kBlock), 0, p->stream, r, p->g, p->tg,
+                       p->s32, p->Ke32, p->fixcnt, p->scal + S_P, L0.wd, L0.x, z, q, partials,
+                       stride, done);
+  else
+    AMT_STREAM_LAUNCH(s, k_mg0_post_s, r, p->g, s.sg, p->s32, p->had32, p->fixcnt,
+                      p->scal + S_P, L0.wd, L0.x, z, q, partials, stride, done);
+}
+
+// Blocks (partials) of the level-0 post-smoothing launch.
+unsigned mg0_post_blocks(const amt_problem* p) {
+  return p->cfg.precision == AMT_PREC_MIXED_BF16 ? p->nb_tile : p->op_post.nb;
+}
+
+// Tuned choices per grid (candidate and measured microseconds for the three kernels), shared by
+// every handle of the process.
+struct TuneEntry {
+  int cand[3];
+  double us[3];
+  double all_us[3][kOpCandidates];
+};
+std::mutex g_tune_mutex;
+std::map<std::tuple<int, int, int>, TuneEntry> g_tune_cache;
+
+// Autotuner of the level-0 operator launches: every candidate of each of the three kernels of
+// the multigrid flexible CG is launched 2 + 5 times on the handle's buffers before the solve
+// starts (outputs go to vectors the solve overwrites; the displacement is not touched) and the
+// fastest mean time wins. All streaming candidates compute bitwise identical operator values;
+// the choice changes the speed, and the rounding of the block-wise inner products (and, against
+// the gather tile, of the operator sum).
+int tune_ops(amt_problem* p) {
+  if (p->ops_tuned) return AMT_OK;
+  const auto key = std::make_tuple(p->g.nx, p->g.ny, p->g.nz);
+  // held through the tuning: handles of one grid (the batch engine's threads) share one choice
+  std::lock_guard<std::mutex> lk(g_tune_mutex);
+  {
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      amt_problem::OpSel* sel[3] = {&p->op_apply, &p->op_upd, &p->op_post};
+      for (int k = 0; k < 3; ++k) {
+        *sel[k] = make_op_sel(p, it->second.cand[k]);
+        sel[k]->us = it->second.us[k];
+      }
+      std::memcpy(p->tune_us, it->second.all_us, sizeof(p->tune_us));
+      p->ops_tuned = true;
+      return AMT_OK;
+    }
+  }
+  if (!p->tune_st) AMT_CHECK(dalloc(p, &p->tune_st, 1));
+  FcgState ts;
+  std::memset(&ts, 0, sizeof(ts));
+  ts.rz = 1.0;
+  ts.beta = 0.5;
+  AMT_CHECK(copy_in(p, p->tune_st, &ts, sizeof(ts)));
+  const int* tdone = (const int*)((const char*)p->tune_st + offsetof(FcgState, done));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(p, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(p, AMT_EHIP, "hipEventCreate failed");
+  }
+  TuneEntry te;
+  int rc = AMT_OK;
+  for (int kind = 0; kind < 3 && rc == AMT_OK; ++kind) {
+    double best = 1e300;
+    int best_c = 0;
+    for (int c = 0; c < kOpCandidates && rc == AMT_OK; ++c) {
+      const amt_problem::OpSel s = make_op_sel(p, c);
+      auto launch = [&]() {
+        if (kind == 0)
+          launch_fcg_apply(p, s, p->z, p->p[0], p->p[1], p->tune_st);
+        else if (kind == 1)
+          launch_mg0_pre_upd(p, s, p->r, p->rr[1], p->q, p->z, p->p[1], p->tune_st,
+                             (int)p->op_apply.nb);
+        else
+          launch_mg0_post(p, s, p->r, p->z, p->q, p->part(R_INIT), p->stride_red, tdone);
+      };
+      for (int w = 0; w < 2; ++w) launch();
+      (void)hipEventRecord(e0, p->stream);
+      for (int w = 0; w < 5; ++w) launch();
+      (void)hipEventRecord(e1, p->stream);
+      float ms = 0.f;
+      if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(p, AMT_EHIP, "operator autotuning launch failed");
+        break;
+      }
+      const double us = 1e3 * ms / 5.0;
+      te.all_us[kind][c] = us;
+      if (us < best) {
+        best = us;
+        best_c = c;
+      }
+    }
+    te.cand[kind] = best_c;
+    te.us[kind] = best;
+    if (kind == 0) {  // the later kinds reduce this launch's partials
+      p->op_apply = make_op_sel(p, best_c);
+      p->op_apply.us = best;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  AMT_CHECK(rc);
+  p->op_upd = make_op_sel(p, te.cand[1]);
+  p->op_upd.us = te.us[1];
+  p->op_post = make_op_sel(p, te.cand[2]);
+  p->op_post.us = te.us[2];
+  std::memcpy(p->tune_us, te.all_us, sizeof(p->tune_us));
+  p->ops_tuned = true;
+  g_tune_cache[key] = te;
+  return AMT_OK;
+}
+
 // z = V-cycle(r) (FP32 inside); writes partials of z.r and z.q (q may be null) to R_INIT.
 // With r_prev, the flexible-CG update is folded into the level-0 pre-smoothing
 // (k_mg0_pre_upd): r = r_prev - alpha q is formed there, x += alpha p_new, and the |r|^2
@@ -376,18 +635,14 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q,
   const size_t L = mg.size();
   const bool mf = p->cfg.precision == AMT_PREC_MIXED_BF16;
   if (r_prev)
-    hipLaunchKernelGGL(k_mg0_pre_upd, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r_prev,
-                       (double*)r, q, p->u, p_new, p->g, p->tg, p->s32, p->Ke32, p->fixcnt,
-                       p->scal + S_P, mg[0].wd, mg[0].x, mg[0].t, p->fcg,
-                       (const double*)p->part(R_APPLY), (int)p->nb_tile, p->part(R_UPD));
+    launch_mg0_pre_upd(p, p->op_upd, r_prev, (double*)r, q, p->u, p_new, p->fcg,
+                       (int)p->op_apply.nb);
   else if (mf)
     hipLaunchKernelGGL(k_mg0_pre_mfma, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g,
                        p->tg, p->s32, p->Kh16, p->Kl16, p->fixcnt, p->scal + S_P, mg[0].wd,
                        mg[0].x, mg[0].t, done);
   else
-    hipLaunchKernelGGL(k_mg0_pre, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g, p->tg,
-                       p->s32, p->Ke32, p->fixcnt, p->scal + S_P, mg[0].wd, mg[0].x, mg[0].t,
-                       done);
+    launch_mg0_pre(p, p->op_post, r, done);
   // each restriction also writes the next level's pre-smoothing step x0 = wd * rhs (not for
   // the dense coarsest level)
   hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[1].g.n_nodes)), dim3(kBlock), 0, p->stream,
@@ -416,28 +671,24 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q,
                        p->tg, p->s32, p->Kh16, p->Kl16, p->fixcnt, p->scal + S_P, mg[0].wd,
                        mg[0].x, z, q, p->part(R_INIT), p->stride_red, done);
   else
-    hipLaunchKernelGGL(k_mg0_post, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g, p->tg,
-                       p->s32, p->Ke32, p->fixcnt, p->scal + S_P, mg[0].wd, mg[0].x, z, q,
-                       p->part(R_INIT), p->stride_red, done);
+    launch_mg0_post(p, p->op_post, r, z, q, p->part(R_INIT), p->stride_red, done);
 }
 
 // kMgChunk flexible-CG iterations (apply, update, V-cycle, scalar). kMgChunk is even, so
 // every chunk starts on the same p-buffer parity and one captured graph serves all chunks.
 void fcg_enqueue_chunk(amt_problem* p, bool eager_prof) {
-  const int nba = (int)p->nb_tile, nbu = (int)nred(p->n_dof);
+  const int nba = (int)p->op_apply.nb, nbu = (int)nred(p->n_dof), nbz = mg0_post_blocks(p);
   int* done_flag = (int*)(p->scal + S_FLAG);
-  // FP32 level-0 smoothing, and few enough tile blocks that every block of the fused launch can
-  // reduce all the operator partials itself (nb_tile^2 doubles of L2 reads per launch)
-  const bool fused = p->cfg.precision != AMT_PREC_MIXED_BF16 && p->nb_tile <= kFusedUpdateTiles;
+  // FP32 level-0 smoothing, and few enough operator blocks that every block of the fused launch
+  // can reduce all the operator partials itself (nba^2 doubles of L2 reads per launch)
+  const bool fused = p->cfg.precision != AMT_PREC_MIXED_BF16 && p->op_apply.nb <= kFusedUpdateTiles;
   for (int t = 0; t < kMgChunk; ++t) {
     hipEvent_t ev = nullptr;
     if (eager_prof && p->prof_on && p->ev_used + 2 <= (int)p->ev_pool.size()) {
       ev = p->ev_pool[p->ev_used];
       (void)hipEventRecord(ev, p->stream);
     }
-    hipLaunchKernelGGL(k_fcg_apply, dim3(nba), dim3(kBlock), 0, p->stream, p->z, p->p[t & 1],
-                       p->p[(t + 1) & 1], p->q, p->s64, p->Ke64, p->fixcnt, p->scal + S_P,
-                       p->g, p->tg, p->fcg, p->part(R_APPLY));
+    launch_fcg_apply(p, p->op_apply, p->z, p->p[t & 1], p->p[(t + 1) & 1], p->fcg);
     if (ev) prof_end(p, ev);
     if (fused) {
       // the update runs inside the level-0 pre-smoothing; r alternates between two buffers
@@ -449,8 +700,8 @@ void fcg_enqueue_chunk(amt_problem* p, bool eager_prof) {
       mg_vcycle(p, p->r, p->z, p->q);
     }
     hipLaunchKernelGGL(k_fcg_scalar, dim3(1), dim3(kBlock), 0, p->stream, p->fcg,
-                       (const double*)p->part(R_UPD), fused ? nba : nbu,
-                       (const double*)p->part(R_APPLY), nba, (const double*)p->part(R_INIT), nba,
+                       (const double*)p->part(R_UPD), fused ? (int)p->op_upd.nb : nbu,
+                       (const double*)p->part(R_APPLY), nba, (const double*)p->part(R_INIT), nbz,
                        p->stride_red, 0, done_flag);
   }
 }
@@ -482,6 +733,7 @@ bool fcg_graph_ready(amt_problem* p) {
 
 // Flexible CG (FP64) preconditioned by the V-cycle, from x = 0 or the previous displacement.
 int solve_mg(amt_problem* p, amt_solve_stats* st) {
+  AMT_CHECK(tune_ops(p));
   const bool warm = p->cfg.warm_start && p->have_u;
   if (!warm)
     hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream,
@@ -492,12 +744,13 @@ int solve_mg(amt_problem* p, amt_solve_stats* st) {
                      p->scal + S_BB, p->cfg.pcg_tol, p->cfg.pcg_maxit);
   hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->p[0],
                      0.0, p->n_dof);
-  const int nba = (int)p->nb_tile, nbu = (int)nred(p->n_dof);
+  const int nba = (int)p->op_apply.nb, nbu = (int)nred(p->n_dof);
   int* done_flag = (int*)(p->scal + S_FLAG);
   mg_vcycle(p, p->r, p->z, nullptr);  // z0 = M r0
   hipLaunchKernelGGL(k_fcg_scalar, dim3(1), dim3(kBlock), 0, p->stream, p->fcg,
                      (const double*)p->part(R_UPD), nbu, (const double*)p->part(R_APPLY), nba,
-                     (const double*)p->part(R_INIT), nba, p->stride_red, 1, done_flag);
+                     (const double*)p->part(R_INIT), mg0_post_blocks(p), p->stride_red, 1,
+                     done_flag);
   AMT_CHECK(check_launch(p, "fcg init"));
   // no host read here: the first batch of chunks is enqueued behind the set-up at once (its
   // kernels return at once if the start is already converged)
@@ -635,10 +888,9 @@ int pcg_run(amt_problem* p, const double* b, TX* x, T* r, T* z, T** pp, T* q, co
 
 // FP64 residual r_out = Fb - K x and |r|^2 into scal[S_RR] (blocking read).
 int residual64(amt_problem* p, const double* x, double* rout, double* rr_host) {
-  hipLaunchKernelGGL(k_residual, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, x, p->Fb, rout,
-                     p->s64, p->Ke64, p->fixcnt, p->scal + S_P, p->g, p->tg, p->part(R_APPLY));
-  hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kBlock), 0, p->stream, p->part(R_APPLY),
-                     (int)p->nb_tile, (int)p->nb_tile, p->scal + S_RR);
+  const int nb = (int)launch_residual(p, x, rout);
+  hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kBlock), 0, p->stream, p->part(R_APPLY), nb,
+                     nb, p->scal + S_RR);
   AMT_CHECK(check_launch(p, "k_residual"));
   if (rr_host) {
     HIP_TRY(p, hipMemcpyAsync(rr_host, p->scal + S_RR, sizeof(double), hipMemcpyDeviceToHost,
@@ -796,9 +1048,7 @@ int solve_unet(amt_problem* p, amt_solve_stats* st) {
   if (!warm) {
     // learned warm start: u0 = gamma M(F_bc), gamma the energy-optimal multiple
     nn_apply<double>(p, net, p->Fb, p->z, nullptr, nullptr);
-    hipLaunchKernelGGL((k_apply<double, double>), dim3(p->nb_tile), dim3(kBlock), 0, p->stream,
-                       p->z, p->q, p->s64, p->Ke64, p->fixcnt, (const double*)(p->scal + S_P),
-                       p->g, p->tg);
+    launch_apply64(p, p->op_apply, p->z, p->q, (const double*)(p->scal + S_P));
     dot_dev(p, p->z, p->Fb, p->scal + S_NNUAU);
     dot_dev(p, p->z, p->q, p->scal + S_NNUR);
     hipLaunchKernelGGL(k_nn_scale, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->u, p->z,
@@ -810,7 +1060,7 @@ int solve_unet(amt_problem* p, amt_solve_stats* st) {
   hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->p[0],
                      0.0, p->n_dof);
   // k_nn_unpack runs nred(n_dof) blocks: its partials fit the R_INIT region
-  const int nba = (int)p->nb_tile, nbu = (int)nred(p->n_dof), nbz = nbu;
+  const int nba = (int)p->op_apply.nb, nbu = (int)nred(p->n_dof), nbz = nbu;
   int* done_flag = (int*)(p->scal + S_FLAG);
   nn_apply<double>(p, net, p->r, p->z, nullptr, nullptr);
   hipLaunchKernelGGL(k_fcg_scalar, dim3(1), dim3(kBlock), 0, p->stream, p->fcg,
@@ -824,9 +1074,7 @@ int solve_unet(amt_problem* p, amt_solve_stats* st) {
   int t = 0, fell_back = 0, chunk = 0, mark_chunk = 0;
   while (!hs.done) {
     for (int c = 0; c < kMgChunk; ++c, ++t) {
-      hipLaunchKernelGGL(k_fcg_apply, dim3(nba), dim3(kBlock), 0, p->stream, p->z, p->p[t & 1],
-                         p->p[(t + 1) & 1], p->q, p->s64, p->Ke64, p->fixcnt, p->scal + S_P,
-                         p->g, p->tg, p->fcg, p->part(R_APPLY));
+      launch_fcg_apply(p, p->op_apply, p->z, p->p[t & 1], p->p[(t + 1) & 1], p->fcg);
       hipLaunchKernelGGL(k_fcg_update, dim3(nbu), dim3(kBlock), 0, p->stream, p->u, p->r,
                          p->p[(t + 1) & 1], p->q, p->n_dof, p->fcg,
                          (const double*)p->part(R_APPLY), nba, p->part(R_UPD));
@@ -1065,6 +1313,7 @@ void amt_config_default(amt_config* c) {
   c->inner_maxit = 20000;
   c->verbose = 0;
   c->use_graphs = 1;
+  c->operator_tile = -1;
 }
 
 int amt_element_stiffness(double E, double nu, double h, double* Ke) {
@@ -1094,6 +1343,9 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
   if (cfg->precision != AMT_PREC_FP64 && cfg->precision != AMT_PREC_MIXED_FP32 &&
       cfg->precision != AMT_PREC_MIXED_BF16)
     return fail(nullptr, AMT_EINVAL, "unknown precision");
+  if (cfg->operator_tile < -1 || cfg->operator_tile >= kOpCandidates)
+    return fail(nullptr, AMT_EINVAL, "operator_tile must be -1 (autotune) or a candidate 0.." +
+                                         std::to_string(kOpCandidates - 1));
   const int64_t n_n = (int64_t)(cfg->nx + 1) * (cfg->ny + 1) * (cfg->nz + 1);
   if (3 * n_n > (int64_t)INT32_MAX) return fail(nullptr, AMT_EINVAL, "grid too large");
   int ndev = 0;
@@ -1161,6 +1413,14 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
   p->nb_tile = 8u * (unsigned)p->tg.per_xcd;
   p->stride_apply = (int)std::max<int64_t>({(int64_t)nblk(p->n_n), (int64_t)nblk(ne),
                                             (int64_t)p->nb_tile});
+  // every operator candidate's partials fit the R_APPLY region
+  for (int c = 1; c < kOpCandidates; ++c)
+    p->stride_apply = std::max<int>(p->stride_apply, (int)make_op_sel(p, c).nb);
+  {
+    const int forced = cfg->operator_tile;
+    p->op_apply = p->op_upd = p->op_post = make_op_sel(p, forced < 0 ? 0 : forced);
+    p->ops_tuned = forced >= 0;
+  }
   p->stride_red = std::max<int>(kMaxReduceBlocks, p->stride_apply);
   {
     // R_APPLY: 1 value per operator block; R_UPD: 2; R_INIT: 3; R_OC: kOcCand; R_MISC: 2
@@ -1194,6 +1454,8 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
   float Kf[576];
   hex8_stiffness(cfg->E, cfg->nu, cfg->element_size, Ke);
   for (int i = 0; i < 576; ++i) Kf[i] = (float)Ke[i];
+  if (!hadamard_blocks<double>(Ke, p->had64) || !hadamard_blocks<float>(Ke, p->had32))
+    return bail(fail(p, AMT_EINVAL, "element stiffness lacks the cube's symmetry"));
   CREATE_TRY(copy_in(p, p->Ke64, Ke, sizeof(Ke)));
   CREATE_TRY(copy_in(p, p->Ke32, Kf, sizeof(Kf)));
   {
@@ -1347,9 +1609,7 @@ int amt_apply_operator(amt_problem* p, const double* x, double* y, int with_bc) 
   if (with_bc && !p->loads_set) return fail(p, AMT_EINVAL, "loads not set (amt_set_loads)");
   if (!p->coeffs_ready) AMT_CHECK(prepare_coeffs(p));
   AMT_CHECK(copy_in(p, p->z, x, sizeof(double) * p->n_dof));
-  hipLaunchKernelGGL((k_apply<double, double>), dim3(p->nb_tile), dim3(kBlock), 0, p->stream,
-                     p->z, p->q, p->s64, p->Ke64, p->fixcnt,
-                     with_bc ? (const double*)(p->scal + S_P) : nullptr, p->g, p->tg);
+  launch_apply64(p, p->op_apply, p->z, p->q, with_bc ? (const double*)(p->scal + S_P) : nullptr);
   AMT_CHECK(check_launch(p, "k_apply"));
   return copy_out(p, y, p->q, sizeof(double) * p->n_dof);
 }
@@ -1377,8 +1637,7 @@ int amt_apply_operator_prec(amt_problem* p, const double* x, double* y, int with
       hipLaunchKernelGGL(k_apply_mfma, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, xf, yf,
                          p->s32, p->Kh16, p->Kl16, p->fixcnt, P, p->g, p->tg);
     else
-      hipLaunchKernelGGL((k_apply<float, float>), dim3(p->nb_tile), dim3(kBlock), 0, p->stream,
-                         xf, yf, p->s32, p->Ke32, p->fixcnt, P, p->g, p->tg);
+      launch_apply32(p, p->op_post, xf, yf, P);
     hipLaunchKernelGGL(k_f2d, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, yf, p->q,
                        p->n_dof);
     rc = check_launch(p, "operator");
@@ -1578,6 +1837,24 @@ int amt_device_info(char* name, int cap, int* n_cu) {
     std::snprintf(name, cap, "%s (%s)", prop.name, prop.gcnArchName);
   }
   if (n_cu) *n_cu = prop.multiProcessorCount;
+  return AMT_OK;
+}
+
+int amt_operator_info(amt_problem* p, int kind, int* candidate, int* tj, int* tk, int* planes,
+                      double* us) {
+  if (!p || kind < 0 || kind > 2) return AMT_EINVAL;
+  const amt_problem::OpSel& s = kind == 0 ? p->op_apply : kind == 1 ? p->op_upd : p->op_post;
+  if (candidate) *candidate = s.cand;
+  if (tj) *tj = s.shape < 0 ? kTJ : kStreamShape[s.shape][0];
+  if (tk) *tk = s.shape < 0 ? kTK : kStreamShape[s.shape][1];
+  if (planes) *planes = s.shape < 0 ? kTI : s.sg.L;
+  if (us) *us = s.us;
+  return AMT_OK;
+}
+
+int amt_operator_timings(amt_problem* p, int kind, double* us, int n) {
+  if (!p || kind < 0 || kind > 2 || n < 0 || (n > 0 && !us)) return AMT_EINVAL;
+  for (int c = 0; c < std::min(n, kOpCandidates); ++c) us[c] = p->tune_us[kind][c];
   return AMT_OK;
 }
 

@@ -66,6 +66,7 @@ class SIMPConfig:
     inner_tolerance: float = 1e-4
     inner_max_iterations: int = 20000
     use_graphs: bool = True
+    operator_tile: int = -1           # level-0 operator launch: -1 autotune, else a candidate
 
 
 @dataclass
@@ -139,6 +140,7 @@ def make_amt_config(nx, ny, nz, element_size, material, config):
     c.inner_maxit = int(config.inner_max_iterations)
     c.verbose = 0
     c.use_graphs = int(bool(config.use_graphs))
+    c.operator_tile = int(config.operator_tile)
     return c
 
 
@@ -348,6 +350,22 @@ class TopologyOptimizer:
         n, ms = C.c_int64(), C.c_double()
         self._check(self._L.amt_profile_read(self._h, C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    def operator_info(self, kind=0):
+        """Level-0 operator launch of kind 0 (FP64 CG operator), 1 (FP32 pre-smoothing with the
+        flexible-CG update) or 2 (FP32 post-smoothing): dict(candidate, tj, tk, planes, us)."""
+        v = [C.c_int() for _ in range(4)]
+        us = C.c_double()
+        self._check(self._L.amt_operator_info(self._h, int(kind), *[C.byref(x) for x in v],
+                                              C.byref(us)))
+        return dict(candidate=v[0].value, tj=v[1].value, tk=v[2].value, planes=v[3].value,
+                    us=us.value)
+
+    def operator_timings(self, kind=0):
+        """The autotuner's launch time (microseconds) of every candidate for that kind."""
+        out = np.zeros(10)
+        self._check(self._L.amt_operator_timings(self._h, int(kind), out, out.size))
+        return out
 
 
 def element_stiffness(E=1.0, nu=0.3, h=1.0):

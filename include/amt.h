@@ -69,6 +69,11 @@ typedef struct amt_config {
   int inner_maxit;
   int verbose;
   int use_graphs;        /* 1: replay the multigrid CG iteration chunk as a HIP graph */
+  int operator_tile;     /* level-0 operator launch of the multigrid solver: -1 autotune on the
+                            first solve (default); 0 the 4x4x16 node gather tile; 1 + 3 s + l the
+                            element-column streaming tile of shape s (0: 16x16, 1: 8x32, 2: 4x64
+                            threads along j x k) with 4, 8 or 16 (l = 0, 1, 2) node planes per
+                            block. All streaming shapes give the same operator values. */
 } amt_config;
 
 /* IterationResult (topology_optimizer.hpp:65-74) plus solver detail. */
@@ -307,6 +312,18 @@ int amt_oc_update_noalloc(double* rho, const double* dcf, const uint8_t* design,
 int amt_sync(amt_problem* p);
 /* Device name / CU count of the handle's GPU. */
 int amt_device_info(char* name, int cap, int* n_cu);
+
+/* The level-0 operator launch chosen for kind 0 (FP64 CG operator and residual), 1 (FP32
+ * pre-smoothing with the flexible-CG update) or 2 (FP32 post-smoothing): *candidate as in
+ * amt_config.operator_tile, the tile shape (*tj x *tk threads, *planes node planes; 4 x 16 x 4
+ * node tile for candidate 0) and the launch time the autotuner measured (*us, 0 if it did not
+ * run). Valid after the first multigrid solve (the tuner runs there); no counterpart in the
+ * reference. */
+int amt_operator_info(amt_problem* p, int kind, int* candidate, int* tj, int* tk, int* planes,
+                      double* us);
+/* The autotuner's measured launch time of every candidate for that kind, us[c] for candidate c
+ * (as amt_config.operator_tile), up to n entries; 0 where it did not run. */
+int amt_operator_timings(amt_problem* p, int kind, double* us, int n);
 
 /* Kernel timing of the operator (K x) launches, measured with HIP events on the handle's
  * stream: enable, then read the number of timed launches and their summed time. */

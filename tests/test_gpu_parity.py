@@ -193,6 +193,71 @@ def test_oc_update_exact_tie_matches_oracle():
     assert ch == chr_
 
 
+# ---- level-0 operator launches (gather tile and element-column streaming tiles) ------------
+@pytest.mark.parametrize("shape", [(7, 5, 4), (33, 18, 70), (2, 1, 1)])
+def test_operator_tiles_match_assembled_csr(shape):
+    """Every level-0 operator candidate (operator_tile 0: the node gather tile; 1..9: the
+    Hadamard-form element-column streaming tile in three shapes x three plane counts) against
+    the oracle's assembled CSR with penalty rows, 1e-13 relative; all streaming candidates give
+    bitwise identical results (the same per-element products summed in the same order); the
+    FP32 streaming operator within 2e-6 of the FP64 one. 33x18x70 spans several tiles along
+    every axis for each shape."""
+    nx, ny, nz = shape
+    rng = np.random.default_rng(nx * 100 + nz)
+    rho = rng.uniform(0.0, 1.0, nx * ny * nz)
+    n = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+    F = rng.standard_normal(n)
+    fixed = np.unique(rng.integers(0, n, size=max(3, n // 50))).astype(np.int64)
+    rp, col, val, Fb, P = _oracle_K(nx, ny, nz, rho, F, fixed, 0)
+    x = rng.standard_normal(n)
+    ref = O.spmv(rp, col, val, x)
+    ys = {}
+    for tile in range(10):
+        opt = _opt(nx, ny, nz, E_min=1e-9, rho_min=0.01, density_index="consistent",
+                   operator_tile=tile)
+        opt.set_loads(F, fixed)
+        opt.set_density(rho)
+        y = opt.apply_operator(x, with_bc=True)
+        assert _rel(y, ref) <= 1e-13, (tile, _rel(y, ref))
+        info = opt.operator_info(0)
+        assert info["candidate"] == tile
+        ys[tile] = y
+        if tile >= 1:
+            y32 = opt.apply_operator(x, with_bc=True, precision="mixed_fp32")
+            assert _rel(y32, y) <= 2e-6, (tile, _rel(y32, y))
+        opt.close()
+    for tile in range(2, 10):
+        assert np.array_equal(ys[tile], ys[1]), tile
+
+
+def test_multigrid_solve_operator_tiles():
+    """The multigrid flexible CG with the gather tile and with forced streaming candidates, and
+    with the autotuner's pick: every solve at the 1e-8 FP64 residual, compliance within 1e-9 of
+    each other, iteration counts within 2. (The operator values are the same for every
+    streaming candidate, but the solver's inner products are summed per block, and the block
+    count follows the tile, so the iterates differ in rounding.)"""
+    p = A.bracket_like_problem(40, 20, 20)
+    out = {}
+    for tile in (0, 1, 5, 9, -1):
+        opt = A.TopologyOptimizer(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3),
+                                  make_config(p, preconditioner="multigrid", operator_tile=tile))
+        opt.set_loads(p.F, p.fixed)
+        opt.set_density(p.rho0)
+        u, st = opt.solve()
+        info = [opt.operator_info(k) for k in range(3)]
+        print("operator_tile %d: %d iterations, residual %.2e, launches %s" % (
+            tile, st["iterations"], st["true_rel_residual"], info))
+        assert st["converged"] and st["true_rel_residual"] <= 1.5e-8, (tile, st)
+        out[tile] = (u, st["iterations"], info)
+        opt.close()
+    c0 = out[0][0] @ p.F
+    for tile, (u, its, info) in out.items():
+        assert (u @ p.F) == pytest.approx(c0, rel=1e-9)
+    its = [v[1] for v in out.values()]
+    assert max(its) - min(its) <= 2, its
+    assert all(i["us"] > 0 for i in out[-1][2])  # the autotuner ran and timed its picks
+
+
 # ---- full OC loop ---------------------------------------------------------------------------
 def test_full_run_cantilever_50_iterations():
     """30x10x10 cantilever, 50 OC iterations, Jacobi-PCG tol 1e-8, consistent density index:
