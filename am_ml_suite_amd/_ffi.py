@@ -32,6 +32,7 @@ EXPORTS = [
     "amt_simp_create", "amt_simp_destroy", "amt_simp_add_load_case", "amt_simp_set_density",
     "amt_simp_get_density", "amt_simp_step", "amt_simp_filter", "amt_simp_get_displacement",
     "amt_oc_update_noalloc", "amt_operator_info", "amt_operator_timings",
+    "amt_unet_benchmark", "amt_unet_infer", "amt_unet_collect_warm",
 ]
 
 
@@ -144,6 +145,8 @@ def load():
     L.amt_unet_set_params.argtypes = [P, _fp, C.c_int64]
     L.amt_unet_forward.argtypes = [P, _fp, _fp]
     L.amt_unet_backward.argtypes = [P, _fp, _fp]
+    L.amt_unet_infer.argtypes = [P, _fp, _fp]
+    L.amt_unet_collect_warm.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
     L.amt_unet_activation.argtypes = [P, C.c_int, _fp, C.c_int64]
     L.amt_attach_unet.argtypes = [P, P]
     L.amt_unet_collect.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
@@ -174,6 +177,7 @@ def load():
                                     C.POINTER(C.c_int), C.POINTER(C.c_int),
                                     C.POINTER(C.c_double)]
     L.amt_operator_timings.argtypes = [P, C.c_int, _dp, C.c_int]
+    L.amt_unet_benchmark.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     for name in EXPORTS:
         if name not in ("amt_config_default", "amt_last_error"):
             getattr(L, name).restype = C.c_int

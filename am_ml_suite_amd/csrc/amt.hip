@@ -163,7 +163,14 @@ struct amt_problem {
   amt_unet* net = nullptr;  // attached, not owned
   float* nn_feat = nullptr;
   float *nn_U = nullptr, *nn_AU = nullptr, *nn_J = nullptr, *nn_KJ = nullptr;
-  std::vector<float*> bank;  // Jacobi-PCG residual snapshots (training inputs)
+  // training samples: a residual (FP32) and the density it belongs to (the sample's operator
+  // and density feature are rebuilt from it when the network trains on it)
+  struct Sample {
+    float* r;
+    double* rho;
+  };
+  std::vector<Sample> bank;
+  int warm_collect = 0;  // multigrid warm solves record their start residual while bank < this
 
   bool loads_set = false, design_ready = false, coeffs_ready = false, have_u = false;
   double n_design = 0;
@@ -238,6 +245,8 @@ int copy_out(amt_problem* p, void* dst, const void* src, size_t bytes) {
 
 int mg_setup(amt_problem* p);
 void dot_dev(amt_problem* p, const double* a, const double* b, double* out);
+int bank_push(amt_problem* p, const double* r);
+int learned_warm_start(amt_problem* p);
 int residual64(amt_problem* p, const double* x, double* rout, double* rr_host);
 void prof_begin(amt_problem* p, hipEvent_t* a);
 void prof_end(amt_problem* p, hipEvent_t a);
@@ -740,6 +749,12 @@ int solve_mg(amt_problem* p, amt_solve_stats* st) {
                        p->u, 0.0, p->n_dof);
   dot_dev(p, p->Fb, p->Fb, p->scal + S_BB);
   AMT_CHECK(residual64(p, p->u, p->r, nullptr));
+  if (warm) {
+    // the start residual of a warm solve is a training sample while collection is armed; with a
+    // network attached it also takes the learned step u += gamma M(r)
+    if ((int)p->bank.size() < p->warm_collect) AMT_CHECK(bank_push(p, p->r));
+    if (p->net) AMT_CHECK(learned_warm_start(p));
+  }
   hipLaunchKernelGGL(k_fcg_start, dim3(1), dim3(64), 0, p->stream, p->fcg, p->scal + S_RR,
                      p->scal + S_BB, p->cfg.pcg_tol, p->cfg.pcg_maxit);
   hipLaunchKernelGGL(k_fill<double>, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->p[0],
@@ -1012,8 +1027,9 @@ int nn_alloc(amt_problem* p) {
 }
 
 // Network forward on residual r: X0 from the density feature and D^-1 r / rms.
+// infer: the forward pass without the training intermediates (no backward follows).
 template <typename T>
-void nn_forward(amt_problem* p, amt_unet* net, const T* r) {
+void nn_forward(amt_problem* p, amt_unet* net, const T* r, bool infer) {
   net->stream = p->stream;
   const unsigned nbz = nred(p->n_dof);
   hipLaunchKernelGGL(k_nn_zz<T>, dim3(nbz), dim3(kBlock), 0, p->stream, r, p->dinv, p->n_dof,
@@ -1023,7 +1039,7 @@ void nn_forward(amt_problem* p, amt_unet* net, const T* r) {
   const VoxDims& d = net->d[0];
   hipLaunchKernelGGL(k_nn_pack<T>, dim3(nblk(d.nv * 16)), dim3(kBlock), 0, p->stream, r, p->dinv,
                      p->nn_feat, p->g, d, p->scal + S_NNZZ, net->act[amt_unet::X0]);
-  amt_unet_impl::forward(net);
+  amt_unet_impl::forward(net, infer);
 }
 
 // z = M(r) = D^-1 r + gain * rms * net(feat, D^-1 r / rms); partials of z.r, z.q into R_INIT
@@ -1031,7 +1047,7 @@ void nn_forward(amt_problem* p, amt_unet* net, const T* r) {
 template <typename T>
 void nn_apply(amt_problem* p, amt_unet* net, const T* r, double* z64, float* z32,
               const double* q) {
-  nn_forward<T>(p, net, r);
+  nn_forward<T>(p, net, r, true);
   hipLaunchKernelGGL(k_nn_unpack<T>, dim3(nred(p->n_dof)), dim3(kBlock), 0, p->stream,
                      net->act[amt_unet::OUT], net->d[0], p->g, r, p->dinv, p->diag, p->fixcnt,
                      p->scal + S_NNZZ, net->gain, z64, z32, q, p->part(R_INIT));
@@ -1125,6 +1141,61 @@ int solve_unet(amt_problem* p, amt_solve_stats* st) {
   return AMT_OK;
 }
 
+// Learned warm start of a multigrid solve: with r = F_bc - K u (from residual64), z = M(r) (the
+// network's correction on top of Jacobi), u += gamma z with gamma = z.r / z.K z (the step that
+// minimises the energy error along z; 0 if z.K z is not positive), then r and |r|^2 afresh in
+// FP64. Two operator applications and one network pass per solve.
+int learned_warm_start(amt_problem* p) {
+  amt_unet* net = p->net;
+  AMT_CHECK(nn_check(p, net));
+  if (!p->nn_feat) {
+    AMT_CHECK(nn_alloc(p));
+    hipLaunchKernelGGL(k_nn_feature, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, p->s64, p->g,
+                       p->nn_feat);
+  }
+  net->stream = p->stream;
+  nn_apply<double>(p, net, p->r, p->z, nullptr, nullptr);
+  launch_apply64(p, p->op_apply, p->z, p->q, (const double*)(p->scal + S_P));
+  dot_dev(p, p->z, p->r, p->scal + S_NNUAU);
+  dot_dev(p, p->z, p->q, p->scal + S_NNUR);
+  hipLaunchKernelGGL(k_nn_step, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->u, p->z,
+                     (const double*)(p->scal + S_NNUAU), p->n_dof);
+  AMT_CHECK(check_launch(p, "learned warm start"));
+  return residual64(p, p->u, p->r, nullptr);
+}
+
+// Appends a training sample: r (FP64, converted) with a copy of the current density.
+int bank_push(amt_problem* p, const double* r) {
+  amt_problem::Sample s{nullptr, nullptr};
+  AMT_CHECK(dalloc(p, &s.r, p->n_dof));
+  AMT_CHECK(dalloc(p, &s.rho, p->n_e));
+  hipLaunchKernelGGL(k_d2f, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, r, s.r, p->n_dof);
+  HIP_TRY(p, hipMemcpyAsync(s.rho, p->rho, sizeof(double) * p->n_e, hipMemcpyDeviceToDevice,
+                            p->stream));
+  p->bank.push_back(s);
+  return AMT_OK;
+}
+
+// The coefficients a training step reads, for a sample's density: element scales, diagonal,
+// penalty, Jacobi inverse and the network's density feature (no multigrid set-up).
+void sample_coeffs(amt_problem* p, const double* rho) {
+  (void)hipMemcpyAsync(p->rho, rho, sizeof(double) * p->n_e, hipMemcpyDeviceToDevice,
+                       p->stream);
+  const GridDims g = p->g;
+  hipLaunchKernelGGL(k_simp, dim3(nblk(p->n_e)), dim3(kBlock), 0, p->stream, p->rho, p->s64,
+                     p->s32, g, p->cfg.density_index == AMT_INDEX_REF_COMPAT ? 1 : 0,
+                     p->cfg.penalty, p->cfg.E_min, p->cfg.rho_min);
+  hipLaunchKernelGGL(k_diag, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, p->diag, p->s64,
+                     p->Ke64, g, p->part(R_APPLY), p->stride_apply);
+  hipLaunchKernelGGL((k_finalize<1, true>), dim3(1), dim3(kBlock), 0, p->stream,
+                     p->part(R_APPLY), (int)nblk(p->n_n), p->stride_apply, p->scal + S_MAXDIAG);
+  hipLaunchKernelGGL(k_dinv, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, p->diag,
+                     p->fixcnt, p->scal + S_MAXDIAG, p->cfg.penalty_factor, p->scal + S_P,
+                     p->dinv, p->dinv32, p->n_dof);
+  hipLaunchKernelGGL(k_nn_feature, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, p->s64, p->g,
+                     p->nn_feat);
+}
+
 // One training step on residual sample r (FP32): energy of U = J + theta N (J = D^-1 r, N the
 // network's correction, theta from the running means), two FP32 operator applications;
 // terms (optional) gets {U.KU, U.r, r.D^-1 r, J.KJ, J.r}. lr <= 0 only evaluates.
@@ -1134,7 +1205,7 @@ void nn_train_step(amt_problem* p, amt_unet* net, const float* r, float lr, doub
                      p->part(R_UPD));
   hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kBlock), 0, p->stream, p->part(R_UPD),
                      (int)nbz, (int)nbz, p->scal + S_NNRDR);
-  nn_forward<float>(p, net, r);
+  nn_forward<float>(p, net, r, false);
   const VoxDims& d = net->d[0];
   hipLaunchKernelGGL(k_nn_fields, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream,
                      net->act[amt_unet::OUT], d, p->g, r, p->dinv, p->diag, p->fixcnt,
@@ -1758,10 +1829,7 @@ int amt_unet_collect(amt_problem* p, int max_samples, int* n_total) {
   if (!p->coeffs_ready) AMT_CHECK(prepare_coeffs(p));
   int taken = 0;
   auto snap = [&](const double* r) -> int {
-    float* b = nullptr;
-    AMT_CHECK(dalloc(p, &b, p->n_dof));
-    hipLaunchKernelGGL(k_d2f, dim3(nblk(p->n_dof)), dim3(kBlock), 0, p->stream, r, b, p->n_dof);
-    p->bank.push_back(b);
+    AMT_CHECK(bank_push(p, r));
     ++taken;
     return AMT_OK;
   };
@@ -1790,22 +1858,37 @@ int amt_unet_collect(amt_problem* p, int max_samples, int* n_total) {
   return AMT_OK;
 }
 
+int amt_unet_collect_warm(amt_problem* p, int max_samples, int* n_total) {
+  if (!p || max_samples < 0) return AMT_EINVAL;
+  if (p->cfg.precond != AMT_PRECOND_MULTIGRID)
+    return fail(p, AMT_EINVAL, "warm-start samples come from multigrid solves");
+  p->warm_collect = max_samples;
+  if (n_total) *n_total = (int)p->bank.size();
+  return AMT_OK;
+}
+
 int amt_unet_train(amt_problem* p, amt_unet* net, int steps, double lr, double* losses,
                    double* jacobi_losses) {
   if (!p || !net || steps < 0) return AMT_EINVAL;
   AMT_CHECK(nn_check(p, net));
   if (p->bank.empty()) return fail(p, AMT_EINVAL, "no training samples (amt_unet_collect)");
+  const double* last_rho = nullptr;
   AMT_CHECK(nn_alloc(p));
   amt_unet* saved = p->net;
   p->net = net;
   if (!p->coeffs_ready) AMT_CHECK(prepare_coeffs(p));
-  hipLaunchKernelGGL(k_nn_feature, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, p->s64, p->g,
-                     p->nn_feat);
   const bool want = losses || jacobi_losses;
   double* lt = nullptr;
   if (want) AMT_CHECK(dalloc(p, &lt, 5 * (size_t)std::max(steps, 1)));
-  for (int s = 0; s < steps; ++s)
-    nn_train_step(p, net, p->bank[s % p->bank.size()], (float)lr, lt ? lt + 5 * s : nullptr);
+  for (int s = 0; s < steps; ++s) {
+    const auto& smp = p->bank[s % p->bank.size()];
+    if (smp.rho != last_rho) {  // the sample's operator, Jacobi scaling and density feature
+      sample_coeffs(p, smp.rho);
+      last_rho = smp.rho;
+    }
+    nn_train_step(p, net, smp.r, (float)lr, lt ? lt + 5 * s : nullptr);
+  }
+  p->coeffs_ready = false;  // the density is the last sample's; the next solve rebuilds
   AMT_CHECK(check_launch(p, "unet train"));
   if (want) {
     std::vector<double> h(5 * (size_t)steps);

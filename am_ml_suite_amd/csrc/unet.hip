@@ -81,6 +81,45 @@ static int net_io(amt_unet* net, int act, const float* in, float* out) {
   return rc == AMT_OK ? AMT_OK : amt::set_global_error(rc, "U-Net transfer failed");
 }
 
+int amt_unet_benchmark(amt_unet* net, int reps, double* ms_per_forward, double* conv_tflops) {
+  if (!net || reps <= 0) return amt::set_global_error(AMT_EINVAL, "invalid argument");
+  net->stream = net->own_stream;
+  // every convolution's multiply-adds: taps x cin x cout per output voxel of its level
+  const int out_level[15] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0};
+  double flops = 0.0;
+  for (size_t li = 0; li < net->L.size(); ++li) {
+    const auto& L = net->L[li];
+    flops += 2.0 * L.taps * (L.c1 + L.c2) * L.cout * (double)net->d[out_level[li]].nv;
+  }
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return amt::set_global_error(AMT_EHIP, "hipEventCreate");
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return amt::set_global_error(AMT_EHIP, "hipEventCreate");
+  }
+  amt_unet_impl::forward(net, true);  // warm-up
+  (void)hipEventRecord(e0, net->stream);
+  for (int r = 0; r < reps; ++r) amt_unet_impl::forward(net, true);
+  (void)hipEventRecord(e1, net->stream);
+  float ms = 0.f;
+  const bool ok = hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!ok) return amt::set_global_error(AMT_EHIP, "forward timing failed");
+  const double per = ms / reps;
+  if (ms_per_forward) *ms_per_forward = per;
+  if (conv_tflops) *conv_tflops = flops / (per * 1e-3) / 1e12;
+  return AMT_OK;
+}
+
+int amt_unet_infer(amt_unet* net, const float* x, float* y) {
+  if (!net || !x || !y) return amt::set_global_error(AMT_EINVAL, "null argument");
+  net->stream = net->own_stream;
+  AMT_CHECK(net_io(net, amt_unet::X0, x, nullptr));
+  amt_unet_impl::forward(net, true);
+  return net_io(net, amt_unet::OUT, nullptr, y);
+}
+
 int amt_unet_forward(amt_unet* net, const float* x, float* y) {
   if (!net || !x || !y) return amt::set_global_error(AMT_EINVAL, "null argument");
   net->stream = net->own_stream;

@@ -1,5 +1,6 @@
 // unet.hpp — BF16 3D U-Net kernels (gfx950): MFMA implicit-GEMM 3x3x3 convolution forward and
-// data-gradient, FP32 weight-gradient, pooling/upsampling, ReLU backward, Adam.
+// data gradient, MFMA weight gradient (deterministic ordered sums), pooling/upsampling, ReLU
+// backward, Adam.
 //
 // Tensors are channels-last on a padded voxel grid (Px, Py, Pz all multiples of 8, z fastest):
 // T[((i*Py + j)*Pz + k) * C + c], BF16 (raw 16-bit patterns), channel counts multiples of 16.
@@ -7,11 +8,8 @@
 // (di+1)*9 + (dj+1)*3 + (dk+1)), so the MFMA B operand of 4 consecutive input channels is one
 // 8-byte load. Accumulation is FP32.
 //
-// MFMA layout used (v_mfma_f32_16x16x16_bf16, wave64), checked against a CPU convolution by
-// tests/test_gpu_unet.py:
-//   A (16x16, M x K): lane l holds A[l % 16][4*(l/16) + e], e = 0..3
-//   B (16x16, K x N): lane l holds B[4*(l/16) + e][l % 16]
-//   D (16x16, M x N): lane l holds D[4*(l/16) + e][l % 16]
+// The convolution runs on v_mfma_f32_16x16x32_bf16 (wave64; layout at k_conv), checked against
+// a CPU convolution by tests/test_gpu_unet.py.
 #pragma once
 
 #include "unet_api.hpp"
@@ -27,144 +25,522 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // Input channels come from x1 (c1 channels) then x2 (c2 channels, may be 0): the U-Net skip
 // concatenation is read in place. Output channels [0, ysplit) go to y1 (row stride ysplit),
 // [ysplit, cout) to y2 (row stride cout - ysplit) — the data gradient of a concatenated input.
-// accum: add to the output instead of overwriting. One wave computes 16 voxels x all output
-// channels (NT = cout/16 accumulator tiles); a block of 4 waves covers 64 voxels.
+// accum: add to the output instead of overwriting.
+//
+// Implicit GEMM on v_mfma_f32_16x16x32_bf16: M = voxels (16 per tile), N = output channels
+// (16 per tile), K = (tap, input channel) walked in chunks of 32, each lane group of 16 lanes
+// taking 8 consecutive channels of one tap, so every A and B fragment is one 16-byte load
+// (channels-last activations, W[tap][co][ci] weights). The pairing of K between the A and B
+// fragments is the same lane slot on both sides, which is all the product needs. A wave computes
+// MT voxel tiles x NTW channel tiles (MT * NTW MFMAs per chunk for MT + NTW fragment loads); the
+// four waves of a block split the channel tiles (WN waves across N) and the voxel tiles; the
+// next chunk's fragments are loaded while the current chunk's MFMAs run.
+//   A fragment: lane l holds A[l % 16][8 (l / 16) .. + 7]; B: B[8 (l / 16) .. + 7][l % 16];
+//   D: lane l holds D[4 (l / 16) + e][l % 16], e = 0..3 (row = voxel, column = channel).
 // ---------------------------------------------------------------------------------------
-template <int TAPS, int NT>
-__global__ __launch_bounds__(256) void k_conv_mfma(
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+union Frag8 {
+  uint4 u;
+  bf16x8 v;
+};
+
+template <int TAPS, int MT, int NTW>
+__global__ __launch_bounds__(256) void k_conv(
     const uint16_t* __restrict__ x1, int c1, const uint16_t* __restrict__ x2, int c2,
     const uint16_t* __restrict__ W, const float* __restrict__ bias, uint16_t* __restrict__ y1,
-    uint16_t* __restrict__ y2, int ysplit, int relu, int accum, VoxDims d) {
-  const int lane = threadIdx.x & 63;
-  const int64_t v0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
-  if (v0 >= d.nv) return;
+    uint16_t* __restrict__ y2, int ysplit, int relu, int accum, VoxDims d, int cout) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int WN = (cout / 16) / NTW;  // waves across the channel tiles (divides 4)
+  const int WM = 4 / WN;
+  const int wn = wv % WN, wm = wv / WN;
+  const int64_t mt0 = ((int64_t)blockIdx.x * WM + wm) * MT;
+  const int nt0 = wn * NTW;
   const int cin = c1 + c2;
-  const int cout = NT * 16;
-  f32x4 acc[NT];
+  const int K = TAPS * cin;
+  const int nch = (K + 31) / 32;
+  const int g = lane >> 4;
+  int vi[MT], vj[MT], vk[MT];
+  bool vok[MT];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  // A row of this lane: voxel v0 + lane%16
-  const int64_t va = v0 + (lane & 15);
-  int ai, aj, ak;
-  vox_decode(va < d.nv ? va : 0, d, ai, aj, ak);
-  const int kg = lane >> 4;  // k-group: channels 4*kg .. 4*kg+3 of the 16-channel chunk
-  for (int tap = 0; tap < TAPS; ++tap) {
+  for (int m = 0; m < MT; ++m) {
+    const int64_t v = (mt0 + m) * 16 + (lane & 15);
+    vok[m] = v < d.nv;
+    vox_decode(vok[m] ? v : 0, d, vi[m], vj[m], vk[m]);
+  }
+  f32x4 acc[MT][NTW];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int n = 0; n < NTW; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  auto load = [&](int c, Frag8 (&a)[MT], Frag8 (&b)[NTW]) {
+    const int k = 32 * c + 8 * g;
+    const bool kin = k < K;
+    const int tap = kin ? k / cin : 0, ci = kin ? k - tap * cin : 0;
     const int di = TAPS == 1 ? 0 : tap / 9 - 1, dj = TAPS == 1 ? 0 : (tap / 3) % 3 - 1,
               dk = TAPS == 1 ? 0 : tap % 3 - 1;
-    const int ni = ai + di, nj = aj + dj, nk = ak + dk;
-    const bool ok = va < d.nv && ni >= 0 && ni < d.px && nj >= 0 && nj < d.py && nk >= 0 &&
-                    nk < d.pz;
-    const int64_t nvox = ((int64_t)ni * d.py + nj) * d.pz + nk;
-    for (int ci0 = 0; ci0 < cin; ci0 += 16) {
-      bf16x4 a = {0, 0, 0, 0};
-      if (ok) {
-        const int ci = ci0 + 4 * kg;
-        const uint16_t* src = ci < c1 ? x1 + nvox * c1 + ci : x2 + nvox * c2 + (ci - c1);
-        const uint2 raw = *(const uint2*)src;
-        a[0] = (short)(raw.x & 0xFFFF);
-        a[1] = (short)(raw.x >> 16);
-        a[2] = (short)(raw.y & 0xFFFF);
-        a[3] = (short)(raw.y >> 16);
-      }
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const int co = 16 * t + (lane & 15);
-        const uint2 raw = *(const uint2*)(W + ((int64_t)tap * cout + co) * cin + ci0 + 4 * kg);
-        bf16x4 b;
-        b[0] = (short)(raw.x & 0xFFFF);
-        b[1] = (short)(raw.x >> 16);
-        b[2] = (short)(raw.y & 0xFFFF);
-        b[3] = (short)(raw.y >> 16);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+    for (int m = 0; m < MT; ++m) {
+      const int ni = vi[m] + di, nj = vj[m] + dj, nk = vk[m] + dk;
+      const bool ok = kin && vok[m] && ni >= 0 && ni < d.px && nj >= 0 && nj < d.py &&
+                      nk >= 0 && nk < d.pz;
+      const int64_t nv = ((int64_t)ni * d.py + nj) * d.pz + nk;
+      a[m].u = ok ? *(const uint4*)(ci < c1 ? x1 + nv * c1 + ci : x2 + nv * c2 + (ci - c1))
+                  : uint4{0u, 0u, 0u, 0u};
+    }
+#pragma unroll
+    for (int n = 0; n < NTW; ++n) {
+      const int co = (nt0 + n) * 16 + (lane & 15);
+      b[n].u = kin ? *(const uint4*)(W + ((int64_t)tap * cout + co) * cin + ci)
+                   : uint4{0u, 0u, 0u, 0u};
+    }
+  };
+  auto mma = [&](const Frag8 (&a)[MT], const Frag8 (&b)[NTW]) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int n = 0; n < NTW; ++n)
+        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m].v, b[n].v, acc[m][n], 0, 0, 0);
+  };
+  Frag8 a0[MT], b0[NTW], a1[MT], b1[NTW];
+  load(0, a0, b0);
+  for (int c = 0; c < nch; c += 2) {
+    if (c + 1 < nch) load(c + 1, a1, b1);
+    mma(a0, b0);
+    if (c + 1 >= nch) break;
+    if (c + 2 < nch) load(c + 2, a0, b0);
+    mma(a1, b1);
+  }
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int n = 0; n < NTW; ++n) {
+      const int co = (nt0 + n) * 16 + (lane & 15);
+      const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t v = (mt0 + m) * 16 + 4 * g + e;
+        if (v >= d.nv) continue;
+        float val = acc[m][n][e] + bv;
+        uint16_t* dst =
+            co < ysplit ? y1 + v * ysplit + co : y2 + v * (cout - ysplit) + (co - ysplit);
+        if (accum) val += bf2f(*dst);
+        if (relu) val = fmaxf(val, 0.f);
+        *dst = f2bf(val);
       }
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// Brick convolution (3x3x3, same operation and arguments as k_conv). A block computes a
+// 4 x 4 x 16 voxel brick (16 M tiles of 16 voxels along z; wave w takes the i-plane w, its four
+// j-rows) for NP output-channel tiles. Input channels go in groups of 16: the group's halo brick
+// (6 x 6 x 18 voxels) is staged in LDS as two 8-channel halves ([half][voxel][8], so the 8 lanes
+// of an LDS access read 128 contiguous bytes), and the 27 taps run from LDS, two taps per
+// 16x16x32 MFMA (K = 2 taps x 16 channels; lane group g takes tap 2c + g / 2, half g % 2). The
+// weight fragments of a channel tile for the group (14 chunks) sit in registers; the next tile's
+// are loaded while this one's MFMAs run.
+// Grid: x = blocks walking the bricks (block x takes bricks x, x + gridDim.x, ...: the next
+// brick's halo is loaded while the current one's MFMAs run, and with a single input group per
+// brick the weight fragments are loaded once), y = groups of NP channel tiles, z = splits of
+// the input-channel groups. With one split the epilogue is k_conv's; with several, each writes FP32 partial sums
+// part[z][v][co] and k_conv_finish adds them in split order. up1: x1 is at half resolution
+// (dimensions d / 2) and is read as its nearest-neighbour upsampling, so the decoder's upsampled
+// input is never written.
+// ---------------------------------------------------------------------------------------
+constexpr int kBrickHalo = 6 * 6 * 18;
+
+template <int NP>
+__global__ __launch_bounds__(256) void k_conv_brick(
+    const uint16_t* __restrict__ x1, int c1, const uint16_t* __restrict__ x2, int c2,
+    const uint16_t* __restrict__ W, const float* __restrict__ bias, uint16_t* __restrict__ y1,
+    uint16_t* __restrict__ y2, int ysplit, int relu, int accum, VoxDims d, int cout,
+    float* __restrict__ part, int up1) {
+  __shared__ uint4 sx[2 * kBrickHalo];  // [half][voxel] 8 channels each
+  const int py1 = up1 ? d.py / 2 : d.py, pz1 = up1 ? d.pz / 2 : d.pz;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, r = lane & 15;
+  const int nbk = (d.pz + 15) / 16, nbj = (d.py + 3) / 4;
+  const int nbricks = ((d.px + 3) / 4) * nbj * nbk;
+  const int cin = c1 + c2, ngr = cin / 16;
+  const int gpk = (ngr + gridDim.z - 1) / gridDim.z;
+  const int gbeg = blockIdx.z * gpk, gend = min(ngr, gbeg + gpk);
+  const int per = gend - gbeg;  // input-channel groups per brick for this split
+  const int nt0 = blockIdx.y * NP;
+  // work items (brick, group), brick-major; block x takes bricks x, x + gridDim.x, ...
+  const int my_bricks = blockIdx.x < nbricks ? (nbricks - 1 - blockIdx.x) / gridDim.x + 1 : 0;
+  const int items = per > 0 ? my_bricks * per : 0;
+  auto brick_of = [&](int t, int& i0, int& j0, int& k0) {
+    const int b = blockIdx.x + (t / per) * gridDim.x;
+    const int bk = b % nbk, bj = (b / nbk) % nbj, bi = b / (nbk * nbj);
+    i0 = 4 * bi;
+    j0 = 4 * bj;
+    k0 = 16 * bk;
+  };
+  // the halo brick of item t: every load issued (consumed an item later)
+  auto load_halo = [&](int t, uint4 (&hv)[6]) {
+    int i0, j0, k0;
+    brick_of(t, i0, j0, k0);
+    const int gr = gbeg + t % per;
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+      const int tt = threadIdx.x + 256 * u;
+      const int vx = tt >> 1, half = tt & 1;
+      const int hk = vx % 18, hj = (vx / 18) % 6, hi = vx / 108;
+      const int gi = i0 - 1 + hi, gj = j0 - 1 + hj, gk = k0 - 1 + hk;
+      const bool ok = tt < 2 * kBrickHalo && gi >= 0 && gi < d.px && gj >= 0 && gj < d.py &&
+                      gk >= 0 && gk < d.pz;
+      const int64_t nv = ((int64_t)gi * d.py + gj) * d.pz + gk;
+      // x1 at half resolution (up1): the nearest-neighbour upsampling read in place
+      const int64_t nv1 =
+          up1 ? ((int64_t)(gi >> 1) * py1 + (gj >> 1)) * pz1 + (gk >> 1) : nv;
+      const int c = 16 * gr + 8 * half;
+      hv[u] = ok ? *(const uint4*)(c < c1 ? x1 + nv1 * c1 + c : x2 + nv * c2 + (c - c1))
+                 : uint4{0u, 0u, 0u, 0u};
+    }
+  };
+  // weight fragments of channel tile n (of this block) for input-channel group gr, 14 tap pairs
+  auto load_b = [&](int gr, int n, Frag8 (&bf)[14]) {
+    const int co = (nt0 + n) * 16 + r;
+    const int ch = 16 * gr + 8 * (g & 1);
+#pragma unroll
+    for (int c = 0; c < 14; ++c) {
+      const int tap = 2 * c + (g >> 1);
+      bf[c].u = tap < 27 ? *(const uint4*)(W + ((int64_t)tap * cout + co) * cin + ch)
+                         : uint4{0u, 0u, 0u, 0u};
+    }
+  };
+  f32x4 acc[4][NP];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < NP; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+  uint4 hv[6];
+  Frag8 bA[14], bB[14];
+  if (items > 0) {
+    load_halo(0, hv);
+    load_b(gbeg, 0, bA);  // with one group per brick the weights stay for every brick
   }
-  // D[row = 4*(lane/16) + e][col = lane%16]: row = voxel offset, col = output channel in tile
+  for (int t = 0; t < items; ++t) {
+    const int gr = gbeg + t % per;
+    if (per > 1 && t > 0) load_b(gr, 0, bA);
+    __syncthreads();  // the previous item's fragment reads are done
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int co = 16 * t + (lane & 15);
+    for (int u = 0; u < 6; ++u) {
+      const int tt = threadIdx.x + 256 * u;
+      if (tt < 2 * kBrickHalo) sx[(tt & 1) * kBrickHalo + (tt >> 1)] = hv[u];
+    }
+    __syncthreads();
+    if (t + 1 < items) load_halo(t + 1, hv);  // in flight during this item's MFMAs
+    // A fragments of tap pair c for the four j-rows (LDS offset of the lane's voxel and tap)
+    auto read_a = [&](int c, Frag8 (&a)[4]) {
+      const int tap = 2 * c + (g >> 1);
+      const int tt = tap < 27 ? tap : 26;  // the padding tap reads a valid slot (times 0)
+      const int di = tt / 9, dj = (tt / 3) % 3, dk = tt % 3;
+      const int base = (g & 1) * kBrickHalo + ((w + di) * 6 + dj) * 18 + r + dk;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int64_t v = v0 + 4 * (lane >> 4) + e;
-      if (v >= d.nv) continue;
-      float val = acc[t][e] + (bias ? bias[co] : 0.f);
-      uint16_t* dst = co < ysplit ? y1 + v * ysplit + co : y2 + v * (cout - ysplit) + (co - ysplit);
-      if (accum) val += bf2f(*dst);
-      if (relu) val = fmaxf(val, 0.f);
-      *dst = f2bf(val);
+      for (int m = 0; m < 4; ++m) a[m].u = sx[base + m * 18];
+    };
+#pragma unroll
+    for (int n = 0; n < NP; ++n) {
+      Frag8(&bc)[14] = (n & 1) ? bB : bA;
+      if (n + 1 < NP) load_b(gr, n + 1, (n & 1) ? bA : bB);
+      // the next pair's fragments are read while this pair's four MFMAs run
+      Frag8 a0[4], a1[4];
+      read_a(0, a0);
+#pragma unroll
+      for (int c = 0; c < 14; c += 2) {
+        read_a(c + 1, a1);
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[m].v, bc[c].v, acc[m][n], 0, 0, 0);
+        if (c + 2 < 14) read_a(c + 2, a0);
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+          acc[m][n] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[m].v, bc[c + 1].v, acc[m][n], 0, 0, 0);
+      }
+    }
+    if ((t + 1) % per) continue;  // more groups of this brick follow
+    // the brick's last group: D (lane rows 4 g + e = voxel k0 + 4 g + e of row (i0 + w, j0 + m),
+    // column r) out, accumulators cleared for the next brick
+    int i0, j0, k0;
+    brick_of(t, i0, j0, k0);
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int n = 0; n < NP; ++n) {
+        const int co = (nt0 + n) * 16 + r;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int kk = k0 + 4 * g + e;
+          if (kk >= d.pz || i0 + w >= d.px || j0 + m >= d.py) continue;  // past the grid
+          const int64_t v = ((int64_t)(i0 + w) * d.py + (j0 + m)) * d.pz + kk;
+          if (part) {
+            part[((int64_t)blockIdx.z * d.nv + v) * cout + co] = acc[m][n][e];
+            continue;
+          }
+          float val = acc[m][n][e] + bv;
+          uint16_t* dst =
+              co < ysplit ? y1 + v * ysplit + co : y2 + v * (cout - ysplit) + (co - ysplit);
+          if (accum) val += bf2f(*dst);
+          if (relu) val = fmaxf(val, 0.f);
+          *dst = f2bf(val);
+        }
+        acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+  }
+  // a split with no groups still owes zero partial sums for its bricks
+  if (part && per <= 0) {
+    for (int b = blockIdx.x; b < nbricks; b += gridDim.x) {
+      const int bk = b % nbk, bj = (b / nbk) % nbj, bi = b / (nbk * nbj);
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < NP; ++n)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int kk = 16 * bk + 4 * g + e, ii = 4 * bi + w, jj = 4 * bj + m;
+            if (kk >= d.pz || ii >= d.px || jj >= d.py) continue;
+            const int64_t v = ((int64_t)ii * d.py + jj) * d.pz + kk;
+            part[((int64_t)blockIdx.z * d.nv + v) * cout + (nt0 + n) * 16 + r] = 0.f;
+          }
     }
   }
 }
 
-// Weight gradient: dW[tap][co][ci] += sum_v dy[v][co] * x[v + off(tap)][ci] over this block's
-// voxel range; block (tap, range). Each thread owns (co, ci) pairs; FP32 atomics across ranges.
-// Also db[co] += sum_v dy[v][co] (tap 13 blocks only).
+// y = act(bias + sum over the splits of part[z][v][co]) (+ the existing value with accum), in
+// split order; one thread per (voxel, channel).
+__global__ void k_conv_finish(const float* __restrict__ part, int ks, VoxDims d, int cout,
+                              const float* __restrict__ bias, uint16_t* __restrict__ y1,
+                              uint16_t* __restrict__ y2, int ysplit, int relu, int accum) {
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= d.nv * cout) return;
+  const int co = (int)(q % cout);
+  const int64_t v = q / cout;
+  float val = 0.f;
+  for (int z = 0; z < ks; ++z) val += part[(int64_t)z * d.nv * cout + q];
+  if (bias) val += bias[co];
+  uint16_t* dst = co < ysplit ? y1 + v * ysplit + co : y2 + v * (cout - ysplit) + (co - ysplit);
+  if (accum) val += bf2f(*dst);
+  if (relu) val = fmaxf(val, 0.f);
+  *dst = f2bf(val);
+}
+
+// ---------------------------------------------------------------------------------------
+// Weight gradient on the matrix cores, deterministic (no atomics):
+//   dW[tap][co][ci] = sum_v dy[v][co] * x[v + off(tap)][ci],   db[co] = sum_v dy[v][co].
+// Per tap a GEMM with M = output channels, N = input channels, K = voxels. Block (x, y) takes
+// the (output tile, input tile) pair y = cot * (cin / 16) + cit and walks the 4 x 4 x 16 voxel
+// bricks x, x + gridDim.x, ... (the next brick's loads in flight during this one's MFMAs). The
+// brick's dy (16 channels) and its input halo (6 x 6 x 18 voxels, 16 channels) are staged in LDS
+// transposed to channel-major rows of 16 voxels along z; the halo as three copies shifted by
+// dk = -1, 0, 1, so that the 8 consecutive voxels a lane contributes to a tap are one aligned
+// 16-byte read. Wave w takes the brick's i-plane w: two K chunks of 32 voxels (lane group g: row
+// j = 2 s + g / 2, z-voxels 8 (g % 2) .. + 7), the dy fragments read once for all taps.
+// The waves' accumulators are summed through LDS at the end and the block writes its record
+// part[x] = {[pair][tap][co % 16][ci % 16] (256 floats per tap and pair), db partials [cout]}
+// (bias entries from the input tile 0 blocks). k_wgrad_chunk and k_wgrad_final add the records
+// in block order.
+// ---------------------------------------------------------------------------------------
+constexpr int kWgDS = 256 + 8;            // dy channel row (elements): 528 B, 16 mod 128
+constexpr int kWgXS = 6 * 6 * 16 + 8;     // halo channel row (elements): 1168 B, 16 mod 128
+constexpr int kWgSmem = 2 * (16 * kWgDS + 3 * 16 * kWgXS);  // 64,512 B
+constexpr int kWgChunk = 16;              // records per first-pass sum
+
 template <int TAPS>
-__global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t* __restrict__ x1, int c1,
+__global__ __launch_bounds__(256, 2) void k_conv_wgrad(const uint16_t* __restrict__ x1, int c1,
                                                     const uint16_t* __restrict__ x2, int c2,
                                                     const uint16_t* __restrict__ dy, int cout,
-                                                    float* __restrict__ dW,
-                                                    float* __restrict__ db, VoxDims d,
-                                                    int64_t range) {
-  __shared__ float sdy[32][128];
-  __shared__ float sx[32][256];
-  const int tap = blockIdx.y;
-  const int cin = c1 + c2;
-  const int di = TAPS == 1 ? 0 : tap / 9 - 1, dj = TAPS == 1 ? 0 : (tap / 3) % 3 - 1,
-            dk = TAPS == 1 ? 0 : tap % 3 - 1;
-  const int npairs = cout * cin;
-  float acc[128];
+                                                    VoxDims d, float* __restrict__ part) {
+  static_assert(2 * TAPS * 256 * 4 + 2 * 16 * 4 <= kWgSmem, "reduction buffer");
+  __shared__ __align__(16) unsigned char smem[kWgSmem];
+  uint16_t* sD = (uint16_t*)smem;
+  uint16_t* sX = sD + 16 * kWgDS;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, r = lane & 15;
+  const int cin = c1 + c2, ncit = cin / 16;
+  const int npairs = (cout / 16) * ncit;
+  const int pr = blockIdx.y, cot = pr / ncit, cit = pr % ncit;
+  const int nbk = (d.pz + 15) / 16, nbj = (d.py + 3) / 4;
+  const int nbricks = ((d.px + 3) / 4) * nbj * nbk;
+  const bool x_first = 16 * cit < c1;  // the input tile lies in x1, else in x2
+  const uint16_t* xs = x_first ? x1 : x2;
+  const int xc = x_first ? c1 : c2, xoff = x_first ? 16 * cit : 16 * cit - c1;
+  auto corner = [&](int b, int& i0, int& j0, int& k0) {
+    const int bk = b % nbk, bj = (b / nbk) % nbj, bi = b / (nbk * nbj);
+    i0 = 4 * bi;
+    j0 = 4 * bj;
+    k0 = 16 * bk;
+  };
+  uint4 dv[2], hv[6];
+  // global loads of brick b: dy (thread: 8 channels of one brick voxel, twice) and the halo
+  // (8 channels of one halo voxel, up to six times); zero outside the grid
+  auto load = [&](int b) {
+    int i0, j0, k0;
+    corner(b, i0, j0, k0);
 #pragma unroll
-  for (int q = 0; q < 128; ++q) acc[q] = 0.f;
+    for (int u = 0; u < 2; ++u) {
+      const int tt = threadIdx.x + 256 * u;
+      const int vx = tt & 255, half = tt >> 8;
+      const int gi = i0 + (vx >> 6), gj = j0 + ((vx >> 4) & 3), gk = k0 + (vx & 15);
+      const bool ok = gi < d.px && gj < d.py && gk < d.pz;
+      dv[u] = ok ? *(const uint4*)(dy + (((int64_t)gi * d.py + gj) * d.pz + gk) * cout +
+                                   16 * cot + 8 * half)
+                 : uint4{0u, 0u, 0u, 0u};
+    }
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+      const int tt = threadIdx.x + 256 * u;
+      const int vx = tt >> 1, half = tt & 1;
+      const int hk = vx % 18, hj = (vx / 18) % 6, hi = vx / 108;
+      const int gi = i0 - 1 + hi, gj = j0 - 1 + hj, gk = k0 - 1 + hk;
+      const bool ok = tt < 2 * 648 && gi >= 0 && gi < d.px && gj >= 0 && gj < d.py && gk >= 0 &&
+                      gk < d.pz;
+      hv[u] = ok ? *(const uint4*)(xs + (((int64_t)gi * d.py + gj) * d.pz + gk) * xc + xoff +
+                                   8 * half)
+                 : uint4{0u, 0u, 0u, 0u};
+    }
+  };
+  // the loaded brick into LDS, transposed: sD[co][voxel], sX[copy][ci][hi][hj][m] = halo
+  // voxel (hi, hj, m + copy) (copy = dk + 1)
+  auto store = [&]() {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int tt = threadIdx.x + 256 * u;
+      const int vx = tt & 255, half = tt >> 8;
+      const uint32_t q[4] = {dv[u].x, dv[u].y, dv[u].z, dv[u].w};
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        sD[(8 * half + e) * kWgDS + vx] = (uint16_t)(q[e >> 1] >> (16 * (e & 1)));
+    }
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+      const int tt = threadIdx.x + 256 * u;
+      if (tt >= 2 * 648) continue;
+      const int vx = tt >> 1, half = tt & 1;
+      const int hk = vx % 18, row = vx / 18;  // row = hi * 6 + hj
+      const uint32_t q[4] = {hv[u].x, hv[u].y, hv[u].z, hv[u].w};
+#pragma unroll
+      for (int c = (TAPS == 1 ? 1 : 0); c < (TAPS == 1 ? 2 : 3); ++c) {
+        const int m = hk - c;
+        if (m < 0 || m >= 16) continue;
+        uint16_t* dst = sX + (c * 16 + 8 * half) * kWgXS + row * 16 + m;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) dst[e * kWgXS] = (uint16_t)(q[e >> 1] >> (16 * (e & 1)));
+      }
+    }
+  };
+  f32x4 acc[TAPS];
+#pragma unroll
+  for (int t = 0; t < TAPS; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
   float bacc = 0.f;
-  const int64_t vbeg = (int64_t)blockIdx.x * range;
-  const int64_t vend = vbeg + range < d.nv ? vbeg + range : d.nv;
-  for (int64_t vc = vbeg; vc < vend; vc += 32) {
-    // stage 32 voxels of dy (cout ch) and the shifted x (cin ch)
-    for (int q = threadIdx.x; q < 32 * cout; q += 256) {
-      const int vv = q / cout, c = q % cout;
-      const int64_t v = vc + vv;
-      sdy[vv][c] = v < vend ? bf2f(dy[v * cout + c]) : 0.f;
-    }
-    for (int q = threadIdx.x; q < 32 * cin; q += 256) {
-      const int vv = q / cin, c = q % cin;
-      const int64_t v = vc + vv;
-      float val = 0.f;
-      if (v < vend) {
-        int i, j, k;
-        vox_decode(v, d, i, j, k);
-        const int ni = i + di, nj = j + dj, nk = k + dk;
-        if (ni >= 0 && ni < d.px && nj >= 0 && nj < d.py && nk >= 0 && nk < d.pz) {
-          const int64_t nv = ((int64_t)ni * d.py + nj) * d.pz + nk;
-          val = c < c1 ? bf2f(x1[nv * c1 + c]) : bf2f(x2[nv * c2 + (c - c1)]);
-        }
-      }
-      sx[vv][c] = val;
-    }
+  const int my = blockIdx.x < nbricks ? (nbricks - 1 - blockIdx.x) / gridDim.x + 1 : 0;
+  if (my > 0) load(blockIdx.x);
+  for (int t = 0; t < my; ++t) {
+    __syncthreads();  // the previous brick's LDS reads are done
+    store();
     __syncthreads();
+    if (t + 1 < my) load(blockIdx.x + (t + 1) * gridDim.x);
+    Frag8 a[2];
 #pragma unroll
-    for (int q = 0; q < 128; ++q) {
-      const int pr = threadIdx.x + 256 * q;
-      if (pr < npairs) {
-        const int co = pr / cin, ci = pr % cin;
-        float s = acc[q];
-        for (int vv = 0; vv < 32; ++vv) s = fmaf(sdy[vv][co], sx[vv][ci], s);
-        acc[q] = s;
+    for (int s = 0; s < 2; ++s) a[s].u = *(const uint4*)(sD + r * kWgDS + w * 64 + 32 * s + 8 * g);
+    if (cit == 0) {
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const uint32_t q[4] = {a[s].u.x, a[s].u.y, a[s].u.z, a[s].u.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          bacc += bf2f((uint16_t)(q[e] & 0xFFFFu)) + bf2f((uint16_t)(q[e] >> 16));
       }
     }
-    if (db && tap == TAPS / 2 && threadIdx.x < cout)
-      for (int vv = 0; vv < 32; ++vv) bacc += sdy[vv][threadIdx.x];
-    __syncthreads();
-  }
 #pragma unroll
-  for (int q = 0; q < 128; ++q) {
-    const int pr = threadIdx.x + 256 * q;
-    if (pr < npairs) {
-      const int co = pr / cin, ci = pr % cin;
-      atomicAdd(dW + ((int64_t)tap * cout + co) * cin + ci, acc[q]);
+    for (int tap = 0; tap < TAPS; ++tap) {
+      const int tp = TAPS == 1 ? 13 : tap;
+      const int di = tp / 9 - 1, dj = (tp / 3) % 3 - 1, dk = tp % 3 - 1;
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        Frag8 b;
+        b.u = *(const uint4*)(sX + ((dk + 1) * 16 + r) * kWgXS +
+                              ((w + di + 1) * 6 + (2 * s + (g >> 1) + dj + 1)) * 16 + 8 * (g & 1));
+        acc[tap] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s].v, b.v, acc[tap], 0, 0, 0);
+      }
     }
   }
-  if (db && tap == TAPS / 2 && threadIdx.x < cout) atomicAdd(db + threadIdx.x, bacc);
+  // sum the four waves (lane (r, g) of every wave holds D[co = 4 g + e][ci = r]) and the lane
+  // groups' bias sums, in a fixed order
+  bacc += __shfl_xor(bacc, 16);
+  bacc += __shfl_xor(bacc, 32);
+  __syncthreads();
+  float* red = (float*)smem;
+  float* rb = red + 2 * TAPS * 256;
+  if (w >= 2) {
+#pragma unroll
+    for (int tap = 0; tap < TAPS; ++tap)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) red[((w - 2) * TAPS + tap) * 256 + e * 64 + lane] = acc[tap][e];
+    if (lane < 16) rb[(w - 2) * 16 + lane] = bacc;
+  }
+  __syncthreads();
+  if (w < 2) {
+#pragma unroll
+    for (int tap = 0; tap < TAPS; ++tap)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[tap][e] += red[(w * TAPS + tap) * 256 + e * 64 + lane];
+    if (lane < 16) bacc += rb[w * 16 + lane];
+  }
+  __syncthreads();
+  if (w == 1) {
+#pragma unroll
+    for (int tap = 0; tap < TAPS; ++tap)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) red[tap * 256 + e * 64 + lane] = acc[tap][e];
+    if (lane < 16) rb[lane] = bacc;
+  }
+  __syncthreads();
+  if (w != 0) return;
+  const int64_t rec = (int64_t)npairs * TAPS * 256 + cout;
+  float* P = part + (int64_t)blockIdx.x * rec;
+#pragma unroll
+  for (int tap = 0; tap < TAPS; ++tap)
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      P[((int64_t)pr * TAPS + tap) * 256 + (4 * g + e) * 16 + r] =
+          acc[tap][e] + red[tap * 256 + e * 64 + lane];
+  if (cit == 0 && lane < 16) P[(int64_t)npairs * TAPS * 256 + 16 * cot + r] = bacc + rb[lane];
+}
+
+// First pass of the ordered sum: sum[c][q] = sum of records c * kWgChunk .. + kWgChunk - 1
+// (fewer in the last chunk), record length rec.
+__global__ void k_wgrad_chunk(const float* __restrict__ part, int nrec, int64_t rec,
+                              float* __restrict__ sum) {
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= rec) return;
+  const int b0 = blockIdx.y * kWgChunk, b1 = min(nrec, b0 + kWgChunk);
+  float s = 0.f;
+  for (int b = b0; b < b1; ++b) s += part[(int64_t)b * rec + q];
+  sum[(int64_t)blockIdx.y * rec + q] = s;
+}
+
+// Second pass: dW[tap][co][ci] += sum over the chunks (in order), db[co] likewise; dW and db
+// adjacent (db = dW + TAPS * cout * cin), one thread per parameter.
+template <int TAPS>
+__global__ void k_wgrad_final(const float* __restrict__ sum, int nchunk, int cout, int cin,
+                              float* __restrict__ dW) {
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t nw = (int64_t)TAPS * cout * cin;
+  if (q >= nw + cout) return;
+  const int ncit = cin / 16, npairs = (cout / 16) * ncit;
+  const int64_t rec = (int64_t)npairs * TAPS * 256 + cout;
+  int64_t idx;
+  if (q < nw) {
+    const int ci = (int)(q % cin), co = (int)((q / cin) % cout), tap = (int)(q / ((int64_t)cin * cout));
+    idx = ((int64_t)((co >> 4) * ncit + (ci >> 4)) * TAPS + tap) * 256 + (co & 15) * 16 + (ci & 15);
+  } else {
+    idx = (int64_t)npairs * TAPS * 256 + (q - nw);
+  }
+  float s = 0.f;
+  for (int c = 0; c < nchunk; ++c) s += sum[(int64_t)c * rec + idx];
+  dW[q] += s;
 }
 
 // Wflip[tap'][ci][co] = W[26 - tap'][co][ci] (data gradient = convolution with flipped,
@@ -196,23 +572,35 @@ __global__ void k_relu_bwd(uint16_t* __restrict__ dy, const uint16_t* __restrict
   if (q < n && !(bf2f(y[q]) > 0.f)) dy[q] = 0;
 }
 
-// 2x2x2 average pooling (fine dims d, coarse dims d/2), channels C.
+// 2x2x2 average pooling (fine dims d, coarse dims d/2), channels C: one thread per 8 channels of
+// a coarse voxel (16-byte loads and store).
 __global__ void k_pool(const uint16_t* __restrict__ x, VoxDims d, int C,
                        uint16_t* __restrict__ y, VoxDims dc) {
   const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (q >= dc.nv * C) return;
-  const int c = (int)(q % C);
-  const int64_t v = q / C;
+  const int C8 = C / 8;
+  if (q >= dc.nv * C8) return;
+  const int c = 8 * (int)(q % C8);
+  const int64_t v = q / C8;
   int i, j, k;
   vox_decode(v, dc, i, j, k);
-  float s = 0.f;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
     const int64_t f = ((int64_t)(2 * i + (a >> 2)) * d.py + (2 * j + ((a >> 1) & 1))) * d.pz +
                       (2 * k + (a & 1));
-    s += bf2f(x[f * C + c]);
+    const uint4 u = *(const uint4*)(x + f * C + c);
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      s[2 * t] += bf2f((uint16_t)(w[t] & 0xFFFFu));
+      s[2 * t + 1] += bf2f((uint16_t)(w[t] >> 16));
+    }
   }
-  y[q] = f2bf(0.125f * s);
+  uint32_t o[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    o[t] = (uint32_t)f2bf(0.125f * s[2 * t]) | ((uint32_t)f2bf(0.125f * s[2 * t + 1]) << 16);
+  *(uint4*)(y + v * C + c) = uint4{o[0], o[1], o[2], o[3]};
 }
 __global__ void k_pool_bwd(const uint16_t* __restrict__ dy, VoxDims dc, int C,
                            uint16_t* __restrict__ dx, VoxDims d, int accum) {
@@ -227,15 +615,18 @@ __global__ void k_pool_bwd(const uint16_t* __restrict__ dy, VoxDims dc, int C,
   if (accum) val += bf2f(dx[q]);
   dx[q] = f2bf(val);
 }
-// nearest-neighbour x2 upsampling and its adjoint
+// nearest-neighbour x2 upsampling (one thread per 8 channels of a fine voxel) and its adjoint
 __global__ void k_upsample(const uint16_t* __restrict__ x, VoxDims dc, int C,
                            uint16_t* __restrict__ y, VoxDims d) {
   const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (q >= d.nv * C) return;
-  const int c = (int)(q % C);
+  const int C8 = C / 8;
+  if (q >= d.nv * C8) return;
+  const int c = 8 * (int)(q % C8);
   int i, j, k;
-  vox_decode(q / C, d, i, j, k);
-  y[q] = x[(((int64_t)(i >> 1) * dc.py + (j >> 1)) * dc.pz + (k >> 1)) * C + c];
+  const int64_t v = q / C8;
+  vox_decode(v, d, i, j, k);
+  *(uint4*)(y + v * C + c) =
+      *(const uint4*)(x + (((int64_t)(i >> 1) * dc.py + (j >> 1)) * dc.pz + (k >> 1)) * C + c);
 }
 __global__ void k_upsample_bwd(const uint16_t* __restrict__ dy, VoxDims d, int C,
                                uint16_t* __restrict__ dx, VoxDims dc) {

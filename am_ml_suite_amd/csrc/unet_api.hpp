@@ -44,6 +44,8 @@ struct amt_unet {
   // {weight of the correction in the preconditioner, running numerator, running denominator}
   double* gain = nullptr;
   uint16_t *wb = nullptr, *wflip = nullptr;
+  float* part = nullptr;   // partial sums of the split brick convolutions
+  float* wpart = nullptr;  // weight-gradient block records and their chunk sums
   int64_t max_w = 0;
   // activations (BF16, channels-last) and their gradients
   enum { X0, A0a, S0, P1, A1a, S1, P2, A2a, S2, P3, B3a, B3, U2, C2a, C2, U1, C1a, C1, U0, C0a,
@@ -60,7 +62,7 @@ struct amt_unet {
 
 namespace amt_unet_impl {
 // All three enqueue on net->stream.
-void forward(amt_unet* n);   // act[X0] -> act[OUT]
+void forward(amt_unet* n, bool infer = false);  // act[X0] -> act[OUT] (infer: U* not written)
 void backward(amt_unet* n);  // grd[OUT] -> parameter gradients (accumulated into n->grad)
 void adam_step(amt_unet* n, float lr);
 }  // namespace amt_unet_impl

@@ -214,6 +214,17 @@ __global__ void k_nn_scale(double* __restrict__ u, const double* __restrict__ z,
   u[i] = gamma * z[i];
 }
 
+// u += gamma z with gamma = dots[0] / dots[1] (0 unless the denominator is positive and both are
+// finite): the energy-optimal step along z for the residual the dots were taken with.
+__global__ void k_nn_step(double* __restrict__ u, const double* __restrict__ z,
+                          const double* __restrict__ dots, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double den = dots[1];
+  const double gamma = (den > 0.0 && isfinite(den) && isfinite(dots[0])) ? dots[0] / den : 0.0;
+  u[i] = fma(gamma, z[i], u[i]);
+}
+
 // partials of r . D^-1 r
 __global__ __launch_bounds__(kBlock) void k_nn_rdr(const float* __restrict__ r,
                                                    const double* __restrict__ dinv, int64_t n,

@@ -46,6 +46,47 @@ int nalloc(amt_unet* n, T** p, size_t count) {
   return AMT_OK;
 }
 
+// Launch shape of a 3x3x3 brick convolution: one channel tile per block (np = 1; more keep too
+// many weight fragments live), grid.y = channel tiles, ks splits of the input-channel groups,
+// doubled until about 512 blocks exist (the small U-Net levels have few bricks).
+struct BrickCfg {
+  int np, ny, ks;
+  int64_t bricks;
+};
+inline BrickCfg brick_cfg(const VoxDims& d, int cin, int cout) {
+  BrickCfg c;
+  c.bricks = (int64_t)((d.px + 3) / 4) * ((d.py + 3) / 4) * ((d.pz + 15) / 16);
+  const int nt = cout / 16, ngr = cin / 16;
+  c.np = 1;  // one channel tile per block: its 14 weight fragments stay in registers
+  c.ny = nt;
+  c.ks = 1;
+  while (c.ks < ngr && c.bricks * c.ny * c.ks < 512) c.ks *= 2;  // two blocks per CU
+  c.ks = std::min(c.ks, ngr);
+  return c;
+}
+// FP32 partial sums the split launches need (floats).
+inline int64_t brick_part_floats(const VoxDims& d, int cin, int cout) {
+  const BrickCfg c = brick_cfg(d, cin, cout);
+  return c.ks > 1 ? (int64_t)c.ks * d.nv * cout : 0;
+}
+
+// Launch shape of a weight gradient (k_conv_wgrad): grid.y = channel-tile pairs, grid.x blocks
+// walking the bricks (about 512 blocks in all); rec floats per block record, nchunk first-pass
+// sums of kWgChunk records.
+struct WgradCfg {
+  int64_t gx, rec, nchunk;
+  int npairs;
+};
+inline WgradCfg wgrad_cfg(const VoxDims& d, int taps, int cin, int cout) {
+  WgradCfg c;
+  const int64_t bricks = (int64_t)((d.px + 3) / 4) * ((d.py + 3) / 4) * ((d.pz + 15) / 16);
+  c.npairs = (cout / 16) * (cin / 16);
+  c.gx = std::max<int64_t>(1, std::min<int64_t>(bricks, 512 / c.npairs));
+  c.rec = (int64_t)c.npairs * taps * 256 + cout;
+  c.nchunk = ceil_div(c.gx, (int64_t)amt::kWgChunk);
+  return c;
+}
+
 inline void net_destroy(amt_unet* n) {
   if (!n) return;
   if (n->stream) (void)hipStreamSynchronize(n->stream);
@@ -107,6 +148,27 @@ inline int net_create(int px, int py, int pz, uint64_t seed, amt_unet** out) {
     if (nalloc(n, &n->act[t], cnt) || nalloc(n, &n->grd[t], cnt)) return bail(AMT_EHIP);
     if (hipMemset(n->act[t], 0, cnt * 2) != hipSuccess) return bail(AMT_EHIP);
   }
+  // partial sums of the split brick convolutions: the largest over every layer, forward and
+  // data gradient (output level of each layer in forward order)
+  {
+    const int out_level[15] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0};
+    int64_t need = 0;
+    for (size_t li = 0; li < n->L.size(); ++li) {
+      const auto& L = n->L[li];
+      if (L.taps != 27) continue;
+      const VoxDims& dl = n->d[out_level[li]];
+      need = std::max(need, brick_part_floats(dl, L.c1 + L.c2, L.cout));
+      need = std::max(need, brick_part_floats(dl, L.cout, L.c1 + L.c2));
+    }
+    if (need > 0 && nalloc(n, &n->part, (size_t)need)) return bail(AMT_EHIP);
+    int64_t wneed = 0;
+    for (size_t li = 0; li < n->L.size(); ++li) {
+      const auto& L = n->L[li];
+      const WgradCfg c = wgrad_cfg(n->d[out_level[li]], L.taps, L.c1 + L.c2, L.cout);
+      wneed = std::max(wneed, (c.gx + c.nchunk) * c.rec);
+    }
+    if (nalloc(n, &n->wpart, (size_t)wneed)) return bail(AMT_EHIP);
+  }
   if (nalloc(n, &n->master, n->n_params) || nalloc(n, &n->grad, n->n_params) ||
       nalloc(n, &n->m, n->n_params) || nalloc(n, &n->v, n->n_params) ||
       nalloc(n, &n->wb, n->n_params) || nalloc(n, &n->wflip, n->max_w) ||
@@ -138,46 +200,73 @@ inline void refresh_bf16(amt_unet* n) {
                      n->stream, n->master, n->wb, n->n_params);
 }
 
-// y = conv(x1 ++ x2) with the layer's weights; NT = output channels / 16.
-inline void conv_launch(hipStream_t st, int taps, int nt, const uint16_t* x1, int c1,
+// y = conv(x1 ++ x2) with the layer's weights; nt = output channels / 16. 3x3x3 layers run as
+// brick convolutions (k_conv_brick, split as brick_cfg says, partial sums in n->part); the
+// pointwise head runs as k_conv (MT voxel tiles of 16 and NTW channel tiles per wave).
+inline void conv_launch(amt_unet* n, int taps, int nt, const uint16_t* x1, int c1,
                         const uint16_t* x2, int c2, const uint16_t* W, const float* bias,
                         uint16_t* y1, uint16_t* y2, int ysplit, int relu, int accum,
-                        const VoxDims& d) {
-  const dim3 grid((unsigned)ceil_div(d.nv, 64)), blk(256);
-#define AMT_CONV_CASE(T, N)                                                                 \
+                        const VoxDims& d, int up1 = 0) {
+  hipStream_t st = n->stream;
+  const int cout = 16 * nt;
+  if (taps == 27) {
+    const BrickCfg c = brick_cfg(d, c1 + c2, cout);
+    float* part = c.ks > 1 ? n->part : nullptr;
+    // two resident blocks per CU (the kernel's occupancy) walk the bricks, one brick at least
+    const int64_t gx = std::max<int64_t>(1, std::min<int64_t>(c.bricks, 512 / (c.ny * c.ks)));
+    const dim3 grid((unsigned)gx, (unsigned)c.ny, (unsigned)c.ks), blk(256);
+#define AMT_BRICK_CASE(NP)                                                                    \
+  if (c.np == NP)                                                                             \
+    hipLaunchKernelGGL((amt::k_conv_brick<NP>), grid, blk, 0, st, x1, c1, x2, c2, W, bias, y1, \
+                       y2, ysplit, relu, accum, d, cout, part, up1);
+    AMT_BRICK_CASE(1)
+#undef AMT_BRICK_CASE
+    if (part)
+      hipLaunchKernelGGL(amt::k_conv_finish, dim3((unsigned)ceil_div(d.nv * cout, 256)),
+                         dim3(256), 0, st, part, c.ks, d, cout, bias, y1, y2, ysplit, relu,
+                         accum);
+    return;
+  }
+  const int64_t mtiles = ceil_div(d.nv, 16);
+#define AMT_CONV_CASE(T, N, MT, NTW)                                                        \
   if (taps == T && nt == N) {                                                               \
-    hipLaunchKernelGGL((amt::k_conv_mfma<T, N>), grid, blk, 0, st, x1, c1, x2, c2, W, bias, \
-                       y1, y2, ysplit, relu, accum, d);                                     \
+    const int wm = 4 / (N / NTW);                                                           \
+    const dim3 grid((unsigned)ceil_div(mtiles, (int64_t)wm * MT)), blk(256);                \
+    hipLaunchKernelGGL((amt::k_conv<T, MT, NTW>), grid, blk, 0, st, x1, c1, x2, c2, W, bias, \
+                       y1, y2, ysplit, relu, accum, d, cout);                               \
     return;                                                                                 \
   }
-  AMT_CONV_CASE(27, 1) AMT_CONV_CASE(27, 2) AMT_CONV_CASE(27, 3) AMT_CONV_CASE(27, 4)
-  AMT_CONV_CASE(27, 6) AMT_CONV_CASE(27, 8) AMT_CONV_CASE(27, 12) AMT_CONV_CASE(1, 1)
+  AMT_CONV_CASE(1, 1, 2, 1)
 #undef AMT_CONV_CASE
 }
 
-inline void forward_layer(amt_unet* n, int li, int in1, int in2, int out) {
+// up1: in1 is the coarse tensor whose upsampling the layer reads (not written out).
+inline void forward_layer(amt_unet* n, int li, int in1, int in2, int out, int up1 = 0) {
   const auto& L = n->L[li];
   const VoxDims& d = n->d[n->act_l[out]];
-  conv_launch(n->stream, L.taps, L.cout / 16, n->act[in1], L.c1, in2 >= 0 ? n->act[in2] : nullptr,
+  conv_launch(n, L.taps, L.cout / 16, n->act[in1], L.c1, in2 >= 0 ? n->act[in2] : nullptr,
               L.c2, n->wb + L.w_off, n->master + L.b_off, n->act[out], nullptr, L.cout, L.relu,
-              0, d);
+              0, d, up1);
 }
 
 inline void pool(amt_unet* n, int in, int out) {
   const VoxDims& dc = n->d[n->act_l[out]];
   const int C = n->act_c[in];
-  hipLaunchKernelGGL(amt::k_pool, dim3(ceil_div(dc.nv * C, 256)), dim3(256), 0, n->stream,
+  hipLaunchKernelGGL(amt::k_pool, dim3(ceil_div(dc.nv * C / 8, 256)), dim3(256), 0, n->stream,
                      n->act[in], n->d[n->act_l[in]], C, n->act[out], dc);
 }
 inline void upsample(amt_unet* n, int in, int out) {
   const VoxDims& d = n->d[n->act_l[out]];
   const int C = n->act_c[in];
-  hipLaunchKernelGGL(amt::k_upsample, dim3(ceil_div(d.nv * C, 256)), dim3(256), 0, n->stream,
-                     n->act[in], n->d[n->act_l[in]], C, n->act[out], d);
+  hipLaunchKernelGGL(amt::k_upsample, dim3(ceil_div(d.nv * C / 8, 256)), dim3(256), 0,
+                     n->stream, n->act[in], n->d[n->act_l[in]], C, n->act[out], d);
 }
 
-// Forward pass on act[X0] -> act[OUT] (stream-ordered, no sync).
-void forward(amt_unet* n) {
+// Forward pass on act[X0] -> act[OUT] (stream-ordered, no sync). With infer, the decoder
+// layers read the coarse tensors' upsampling in place and U2, U1, U0 are not written (the
+// backward's weight gradient needs them, so training runs with infer = false); the output is
+// the same either way.
+void forward(amt_unet* n, bool infer) {
   using U = amt_unet;
   forward_layer(n, 0, U::X0, -1, U::A0a);
   forward_layer(n, 1, U::A0a, -1, U::S0);
@@ -190,14 +279,26 @@ void forward(amt_unet* n) {
   pool(n, U::S2, U::P3);
   forward_layer(n, 6, U::P3, -1, U::B3a);
   forward_layer(n, 7, U::B3a, -1, U::B3);
-  upsample(n, U::B3, U::U2);
-  forward_layer(n, 8, U::U2, U::S2, U::C2a);
+  if (infer) {
+    forward_layer(n, 8, U::B3, U::S2, U::C2a, 1);
+  } else {
+    upsample(n, U::B3, U::U2);
+    forward_layer(n, 8, U::U2, U::S2, U::C2a);
+  }
   forward_layer(n, 9, U::C2a, -1, U::C2);
-  upsample(n, U::C2, U::U1);
-  forward_layer(n, 10, U::U1, U::S1, U::C1a);
+  if (infer) {
+    forward_layer(n, 10, U::C2, U::S1, U::C1a, 1);
+  } else {
+    upsample(n, U::C2, U::U1);
+    forward_layer(n, 10, U::U1, U::S1, U::C1a);
+  }
   forward_layer(n, 11, U::C1a, -1, U::C1);
-  upsample(n, U::C1, U::U0);
-  forward_layer(n, 12, U::U0, U::S0, U::C0a);
+  if (infer) {
+    forward_layer(n, 12, U::C1, U::S0, U::C0a, 1);
+  } else {
+    upsample(n, U::C1, U::U0);
+    forward_layer(n, 12, U::U0, U::S0, U::C0a);
+  }
   forward_layer(n, 13, U::C0a, -1, U::C0);
   forward_layer(n, 14, U::C0, -1, U::OUT);
 }
@@ -213,21 +314,31 @@ inline void backward_layer(amt_unet* n, int li, int in1, int in2, int out, int d
   if (L.relu)
     hipLaunchKernelGGL(amt::k_relu_bwd, dim3(ceil_div(cnt, 256)), dim3(256), 0, n->stream,
                        n->grd[out], n->act[out], cnt);
-  const int64_t range = std::max<int64_t>(2048, ceil_div(d.nv, 32));
-  const dim3 wg((unsigned)ceil_div(d.nv, range), L.taps);
-  if (L.taps == 27)
-    hipLaunchKernelGGL(amt::k_conv_wgrad<27>, wg, dim3(256), 0, n->stream, n->act[in1], L.c1,
-                       in2 >= 0 ? n->act[in2] : nullptr, L.c2, n->grd[out], L.cout,
-                       n->grad + L.w_off, n->grad + L.b_off, d, range);
-  else
-    hipLaunchKernelGGL(amt::k_conv_wgrad<1>, wg, dim3(256), 0, n->stream, n->act[in1], L.c1,
-                       in2 >= 0 ? n->act[in2] : nullptr, L.c2, n->grd[out], L.cout,
-                       n->grad + L.w_off, n->grad + L.b_off, d, range);
-  if (dx1 < 0) return;
+  // weight and bias gradients: block records, then their sums in block order
   const int cin = L.c1 + L.c2;
+  const WgradCfg wc = wgrad_cfg(d, L.taps, cin, L.cout);
+  const uint16_t* x2 = in2 >= 0 ? n->act[in2] : nullptr;
+  float* sums = n->wpart + wc.gx * wc.rec;
+  const dim3 wg((unsigned)wc.gx, (unsigned)wc.npairs);
+  if (L.taps == 27)
+    hipLaunchKernelGGL(amt::k_conv_wgrad<27>, wg, dim3(256), 0, n->stream, n->act[in1], L.c1, x2,
+                       L.c2, n->grd[out], L.cout, d, n->wpart);
+  else
+    hipLaunchKernelGGL(amt::k_conv_wgrad<1>, wg, dim3(256), 0, n->stream, n->act[in1], L.c1, x2,
+                       L.c2, n->grd[out], L.cout, d, n->wpart);
+  hipLaunchKernelGGL(amt::k_wgrad_chunk, dim3((unsigned)ceil_div(wc.rec, 256), (unsigned)wc.nchunk),
+                     dim3(256), 0, n->stream, n->wpart, (int)wc.gx, wc.rec, sums);
+  const unsigned gf = (unsigned)ceil_div(L.n_w + L.cout, 256);
+  if (L.taps == 27)
+    hipLaunchKernelGGL(amt::k_wgrad_final<27>, dim3(gf), dim3(256), 0, n->stream, sums,
+                       (int)wc.nchunk, L.cout, cin, n->grad + L.w_off);
+  else
+    hipLaunchKernelGGL(amt::k_wgrad_final<1>, dim3(gf), dim3(256), 0, n->stream, sums,
+                       (int)wc.nchunk, L.cout, cin, n->grad + L.w_off);
+  if (dx1 < 0) return;
   hipLaunchKernelGGL(amt::k_w_flip, dim3(ceil_div(L.n_w, 256)), dim3(256), 0, n->stream,
                      n->master + L.w_off, L.taps, L.cout, cin, n->wflip);
-  conv_launch(n->stream, L.taps, cin / 16, n->grd[out], L.cout, nullptr, 0, n->wflip, nullptr,
+  conv_launch(n, L.taps, cin / 16, n->grd[out], L.cout, nullptr, 0, n->wflip, nullptr,
               n->grd[dx1], dx2 >= 0 ? n->grd[dx2] : nullptr, L.c1, 0, accum_dx, d);
 }
 

@@ -333,6 +333,13 @@ class TopologyOptimizer:
         self._check(self._L.amt_unet_collect(self._h, int(max_samples), C.byref(n)))
         return n.value
 
+    def collect_warm_samples(self, max_samples):
+        """Arms warm-start sample collection (multigrid handle): later warm solves append their
+        start residuals until max_samples are held. Returns the samples held now."""
+        n = C.c_int()
+        self._check(self._L.amt_unet_collect_warm(self._h, int(max_samples), C.byref(n)))
+        return n.value
+
     def train_unet(self, net, steps, lr=1e-3, with_jacobi=False):
         """Adam steps on the collected samples; returns each step's energy loss (and, with
         with_jacobi, the loss of the Jacobi term alone on the same sample)."""
@@ -433,6 +440,13 @@ class UNet:
         self._check(self._L.amt_unet_forward(self._h, x, y))
         return y
 
+    def infer(self, x):
+        """The inference forward pass (the same output as forward, no training state)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros_like(x)
+        self._check(self._L.amt_unet_infer(self._h, x, y))
+        return y
+
     def backward(self, dy):
         dy = np.ascontiguousarray(dy, dtype=np.float32)
         g = np.zeros(self.n_params, dtype=np.float32)
@@ -448,6 +462,12 @@ class UNet:
     @gain.setter
     def gain(self, value):
         self._check(self._L.amt_unet_set_gain(self._h, float(value)))
+
+    def benchmark(self, reps=5):
+        """(milliseconds per forward pass, convolution TFLOP/s) on the stored input."""
+        ms, tf = C.c_double(), C.c_double()
+        self._check(self._L.amt_unet_benchmark(self._h, int(reps), C.byref(ms), C.byref(tf)))
+        return ms.value, tf.value
 
     def activation(self, name):
         """Stored activation of the last forward, shape (px>>l, py>>l, pz>>l, channels)."""

@@ -180,18 +180,37 @@ int amt_unet_get_params(amt_unet* net, float* params, int64_t n);
 int amt_unet_set_params(amt_unet* net, const float* params, int64_t n);
 /* Forward on host tensors: x, y are [px*py*pz][16] FP32 (stored BF16 on the device). */
 int amt_unet_forward(amt_unet* net, const float* x, float* y);
+/* The forward pass of inference (the preconditioner's): the same output as amt_unet_forward,
+ * without writing the training intermediates (the decoder reads its upsampled inputs in
+ * place), so amt_unet_backward / amt_unet_activation of the decoder's upsampled tensors refer to
+ * the last amt_unet_forward. */
+int amt_unet_infer(amt_unet* net, const float* x, float* y);
+/* Times reps inference forward passes on the network's stored input (after one untimed pass)
+ * with HIP events on its stream: *ms_per_forward, and the convolutions' multiply-add rate over the whole
+ * pass (*conv_tflops, 2 x taps x cin x cout per output voxel of each layer, pooling and
+ * upsampling time included). */
+int amt_unet_benchmark(amt_unet* net, int reps, double* ms_per_forward, double* conv_tflops);
 /* Parameter gradient of sum(y * dy) for the last forward; dparams has num_params entries. */
 int amt_unet_backward(amt_unet* net, const float* dy, float* dparams);
 /* Copy a stored activation of the last forward to the host as FP32. tensor indexes
  * X0 A0a S0 P1 A1a S1 P2 A2a S2 P3 B3a B3 U2 C2a C2 U1 C1a C1 U0 C0a C0 OUT (0..21), the
  * network's tensors in forward order; n = voxels of its level * its channel count. */
 int amt_unet_activation(amt_unet* net, int tensor, float* out, int64_t n);
-/* Attach (or detach with NULL) the network used by AMT_PRECOND_UNET. */
+/* Attach (or detach with NULL) a network. With AMT_PRECOND_UNET it is the preconditioner above.
+ * With AMT_PRECOND_MULTIGRID and warm_start it is a learned warm start: every solve that starts
+ * from the previous displacement u first takes the step u += gamma M(r), r = F_bc - K u,
+ * gamma = r.M(r) / M(r).K M(r) (energy-optimal; 0 if the curvature is not positive), then runs
+ * the flexible CG from there, held to the same FP64 tolerance. */
 int amt_attach_unet(amt_problem* p, amt_unet* net);
 /* Appends training samples: residuals of a plain Jacobi-PCG solve of p's current problem
  * (every 32 passes, up to max_samples). *n_total = samples held. */
 int amt_unet_collect(amt_problem* p, int max_samples, int* n_total);
-/* Adam steps on the held samples (cycled in order). Each step forms U = D^-1 r + theta N
+/* Arms warm-start sample collection on a multigrid handle: while fewer than max_samples are
+ * held, every solve that starts from the previous displacement appends its start residual
+ * F_bc - K u (with the density it belongs to) to the samples amt_unet_train cycles through. */
+int amt_unet_collect_warm(amt_problem* p, int max_samples, int* n_total);
+/* Adam steps on the held samples (cycled in order; each sample is taken with the operator and
+ * density feature of its own density, so the handle's density is the last sample's afterwards). Each step forms U = D^-1 r + theta N
  * (N the network's correction, theta the energy-optimal weight, folded into the network's gain
  * as a running mean) and descends the energy (U.KU - 2 U.r) / (r.D^-1 r); two FP32 operator
  * applications per step. losses / jacobi_losses (optional, steps entries) get each step's

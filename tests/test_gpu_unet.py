@@ -36,12 +36,19 @@ def _device_state(net):
     return S
 
 
-def test_unet_forward_matches_oracle():
+# 16^3: one brick per block at every level, one input-channel group per split; 64 x 32 x 80
+# (non-cubic): 640 level-0 bricks, so blocks walk several bricks, and the three input-channel
+# groups of the first decoder layer run unsplit in one block (several groups per brick)
+_SHAPES = [(16, 16, 16), (64, 32, 80)]
+
+
+@pytest.mark.parametrize("P", _SHAPES)
+def test_unet_forward_matches_oracle(P):
     """MFMA implicit-GEMM forward (BF16 storage, FP32 accumulation). Per layer, on the device's
     own input tensor: the stored output equals the FP64 conv (+ReLU) rounded to BF16 up to
     accumulation-order rounding flips (relative L2 <= 5e-3); pools and upsamplings likewise.
     End to end vs the oracle's BF16-rounded forward: relative L2 <= 3e-2."""
-    net, pb, x = _net_and_oracle_inputs()
+    net, pb, x = _net_and_oracle_inputs(P=P)
     y = net.forward(x.reshape(-1, 16)).reshape(x.shape)
     ref = U.forward(pb, x, rnd=U.bf16)['OUT']
     err = np.linalg.norm(y - ref) / np.linalg.norm(ref)
@@ -63,13 +70,14 @@ def test_unet_forward_matches_oracle():
     assert max(errs) <= 5e-3, errs
 
 
-def test_unet_parameter_gradients_match_oracle():
+@pytest.mark.parametrize("P", _SHAPES)
+def test_unet_parameter_gradients_match_oracle(P):
     """Backward (MFMA data gradient with flipped weights, FP32 weight gradient) vs FP64 numpy
     backprop through the device's stored forward state (same ReLU masks), with BF16 rounding
     at the device's storage points of activation gradients: per-layer relative L2 <= 2e-2.
     (Against the oracle's own forward the masks differ wherever a pre-activation lies within
     BF16 noise of zero; that alone moves these gradients by several percent.)"""
-    net, pb, x = _net_and_oracle_inputs(seed=6)
+    net, pb, x = _net_and_oracle_inputs(seed=6, P=P)
     net.forward(x.reshape(-1, 16))
     dout = np.random.default_rng(7).standard_normal(x.shape)
     g = net.backward(dout.reshape(-1, 16)).astype(np.float64)
@@ -164,4 +172,17 @@ def test_learned_preconditioner_forced_fallback(gain):
     assert (u @ p.F) == pytest.approx(uref @ p.F, rel=1e-6)
     opt.close()
     ref.close()
+    net.close()
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 24), (24, 16, 40)])
+def test_unet_inference_forward_matches_training_forward(shape):
+    """The inference forward (decoder reading its upsampled inputs in place, split brick
+    convolutions on the small levels) gives bitwise the training forward's output."""
+    net = A.UNet(*shape, seed=5)
+    x = np.random.default_rng(7).standard_normal((int(np.prod(shape)), 16)).astype(np.float32)
+    y_train = net.forward(x)
+    y_infer = net.infer(x)
+    assert np.array_equal(y_train, y_infer)
+    assert np.abs(y_train).max() > 0
     net.close()
