@@ -8,33 +8,29 @@
 // (di+1)*9 + (dj+1)*3 + (dk+1)), so the MFMA B operand of 4 consecutive input channels is one
 // 8-byte load. Accumulation is FP32.
 //
-// The convolution runs on v_mfma_f32_16x16x32_bf16 (wave64; layout at k_conv), checked against
-// a CPU convolution by tests/test_gpu_unet.py.
+// The convolutions run on v_mfma_f32_16x16x32_bf16 (wave64; fragment layouts below), checked
+// against a CPU convolution by tests/test_gpu_unet.py.
 #pragma once
 
 #include "unet_api.hpp"
 
 namespace amt {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------------------
-// Convolution (TAPS = 27: 3x3x3 with zero padding; TAPS = 1: pointwise) by MFMA.
+// Convolution layers (3x3x3 with zero padding):
 //   y[v][co] = act( bias[co] + sum_tap sum_ci W[tap][co][ci] * x[v + off(tap)][ci] )
 // Input channels come from x1 (c1 channels) then x2 (c2 channels, may be 0): the U-Net skip
 // concatenation is read in place. Output channels [0, ysplit) go to y1 (row stride ysplit),
 // [ysplit, cout) to y2 (row stride cout - ysplit) — the data gradient of a concatenated input.
 // accum: add to the output instead of overwriting.
 //
-// Implicit GEMM on v_mfma_f32_16x16x32_bf16: M = voxels (16 per tile), N = output channels
-// (16 per tile), K = (tap, input channel) walked in chunks of 32, each lane group of 16 lanes
-// taking 8 consecutive channels of one tap, so every A and B fragment is one 16-byte load
-// (channels-last activations, W[tap][co][ci] weights). The pairing of K between the A and B
-// fragments is the same lane slot on both sides, which is all the product needs. A wave computes
-// MT voxel tiles x NTW channel tiles (MT * NTW MFMAs per chunk for MT + NTW fragment loads); the
-// four waves of a block split the channel tiles (WN waves across N) and the voxel tiles; the
-// next chunk's fragments are loaded while the current chunk's MFMAs run.
+// Implicit GEMM on v_mfma_f32_16x16x32_bf16 (wave64): M = voxels (16 per tile), N = output
+// channels (16 per tile), K = (tap, input channel) in chunks of 32, each lane group of 16 lanes
+// taking 8 consecutive channels of one tap, so every A and B fragment is one 16-byte load. The
+// pairing of K between the A and B fragments is the same lane slot on both sides, which is all
+// the product needs.
 //   A fragment: lane l holds A[l % 16][8 (l / 16) .. + 7]; B: B[8 (l / 16) .. + 7][l % 16];
 //   D: lane l holds D[4 (l / 16) + e][l % 16], e = 0..3 (row = voxel, column = channel).
 // ---------------------------------------------------------------------------------------
@@ -44,95 +40,8 @@ union Frag8 {
   bf16x8 v;
 };
 
-template <int TAPS, int MT, int NTW>
-__global__ __launch_bounds__(256) void k_conv(
-    const uint16_t* __restrict__ x1, int c1, const uint16_t* __restrict__ x2, int c2,
-    const uint16_t* __restrict__ W, const float* __restrict__ bias, uint16_t* __restrict__ y1,
-    uint16_t* __restrict__ y2, int ysplit, int relu, int accum, VoxDims d, int cout) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int WN = (cout / 16) / NTW;  // waves across the channel tiles (divides 4)
-  const int WM = 4 / WN;
-  const int wn = wv % WN, wm = wv / WN;
-  const int64_t mt0 = ((int64_t)blockIdx.x * WM + wm) * MT;
-  const int nt0 = wn * NTW;
-  const int cin = c1 + c2;
-  const int K = TAPS * cin;
-  const int nch = (K + 31) / 32;
-  const int g = lane >> 4;
-  int vi[MT], vj[MT], vk[MT];
-  bool vok[MT];
-#pragma unroll
-  for (int m = 0; m < MT; ++m) {
-    const int64_t v = (mt0 + m) * 16 + (lane & 15);
-    vok[m] = v < d.nv;
-    vox_decode(vok[m] ? v : 0, d, vi[m], vj[m], vk[m]);
-  }
-  f32x4 acc[MT][NTW];
-#pragma unroll
-  for (int m = 0; m < MT; ++m)
-#pragma unroll
-    for (int n = 0; n < NTW; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-  auto load = [&](int c, Frag8 (&a)[MT], Frag8 (&b)[NTW]) {
-    const int k = 32 * c + 8 * g;
-    const bool kin = k < K;
-    const int tap = kin ? k / cin : 0, ci = kin ? k - tap * cin : 0;
-    const int di = TAPS == 1 ? 0 : tap / 9 - 1, dj = TAPS == 1 ? 0 : (tap / 3) % 3 - 1,
-              dk = TAPS == 1 ? 0 : tap % 3 - 1;
-#pragma unroll
-    for (int m = 0; m < MT; ++m) {
-      const int ni = vi[m] + di, nj = vj[m] + dj, nk = vk[m] + dk;
-      const bool ok = kin && vok[m] && ni >= 0 && ni < d.px && nj >= 0 && nj < d.py &&
-                      nk >= 0 && nk < d.pz;
-      const int64_t nv = ((int64_t)ni * d.py + nj) * d.pz + nk;
-      a[m].u = ok ? *(const uint4*)(ci < c1 ? x1 + nv * c1 + ci : x2 + nv * c2 + (ci - c1))
-                  : uint4{0u, 0u, 0u, 0u};
-    }
-#pragma unroll
-    for (int n = 0; n < NTW; ++n) {
-      const int co = (nt0 + n) * 16 + (lane & 15);
-      b[n].u = kin ? *(const uint4*)(W + ((int64_t)tap * cout + co) * cin + ci)
-                   : uint4{0u, 0u, 0u, 0u};
-    }
-  };
-  auto mma = [&](const Frag8 (&a)[MT], const Frag8 (&b)[NTW]) {
-#pragma unroll
-    for (int m = 0; m < MT; ++m)
-#pragma unroll
-      for (int n = 0; n < NTW; ++n)
-        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m].v, b[n].v, acc[m][n], 0, 0, 0);
-  };
-  Frag8 a0[MT], b0[NTW], a1[MT], b1[NTW];
-  load(0, a0, b0);
-  for (int c = 0; c < nch; c += 2) {
-    if (c + 1 < nch) load(c + 1, a1, b1);
-    mma(a0, b0);
-    if (c + 1 >= nch) break;
-    if (c + 2 < nch) load(c + 2, a0, b0);
-    mma(a1, b1);
-  }
-#pragma unroll
-  for (int m = 0; m < MT; ++m)
-#pragma unroll
-    for (int n = 0; n < NTW; ++n) {
-      const int co = (nt0 + n) * 16 + (lane & 15);
-      const float bv = bias ? bias[co] : 0.f;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int64_t v = (mt0 + m) * 16 + 4 * g + e;
-        if (v >= d.nv) continue;
-        float val = acc[m][n][e] + bv;
-        uint16_t* dst =
-            co < ysplit ? y1 + v * ysplit + co : y2 + v * (cout - ysplit) + (co - ysplit);
-        if (accum) val += bf2f(*dst);
-        if (relu) val = fmaxf(val, 0.f);
-        *dst = f2bf(val);
-      }
-    }
-}
-
 // ---------------------------------------------------------------------------------------
-// Brick convolution (3x3x3, same operation and arguments as k_conv). A block computes a
+// Brick convolution (the operation and arguments above). A block computes a
 // 4 x 4 x 16 voxel brick (16 M tiles of 16 voxels along z; wave w takes the i-plane w, its four
 // j-rows) for NP output-channel tiles. Input channels go in groups of 16: the group's halo brick
 // (6 x 6 x 18 voxels) is staged in LDS as two 8-channel halves ([half][voxel][8], so the 8 lanes
@@ -143,12 +52,18 @@ __global__ __launch_bounds__(256) void k_conv(
 // Grid: x = blocks walking the bricks (block x takes bricks x, x + gridDim.x, ...: the next
 // brick's halo is loaded while the current one's MFMAs run, and with a single input group per
 // brick the weight fragments are loaded once), y = groups of NP channel tiles, z = splits of
-// the input-channel groups. With one split the epilogue is k_conv's; with several, each writes FP32 partial sums
-// part[z][v][co] and k_conv_finish adds them in split order. up1: x1 is at half resolution
+// the input-channel groups (every split has at least one group). With one split the block
+// applies bias, accumulation and ReLU itself; with several, each writes FP32 partial sums
+// part[z][v][co] and k_conv_finish adds them in split order (a completion counter with
+// device-scope fences in this kernel measured twice as slow: the fences write back and
+// invalidate L2). up1: x1 is at half resolution
 // (dimensions d / 2) and is read as its nearest-neighbour upsampling, so the decoder's upsampled
 // input is never written.
 // ---------------------------------------------------------------------------------------
 constexpr int kBrickHalo = 6 * 6 * 18;
+// the second channel half starts 64 bytes past a multiple of 128 B: the 8 lanes of a 128-byte
+// LDS store phase (4 voxels x 2 halves) then hit disjoint banks
+constexpr int kBrickHalf = kBrickHalo + 4;
 
 template <int NP>
 __global__ __launch_bounds__(256) void k_conv_brick(
@@ -156,7 +71,7 @@ __global__ __launch_bounds__(256) void k_conv_brick(
     const uint16_t* __restrict__ W, const float* __restrict__ bias, uint16_t* __restrict__ y1,
     uint16_t* __restrict__ y2, int ysplit, int relu, int accum, VoxDims d, int cout,
     float* __restrict__ part, int up1) {
-  __shared__ uint4 sx[2 * kBrickHalo];  // [half][voxel] 8 channels each
+  __shared__ uint4 sx[2 * kBrickHalf];  // [half][voxel] 8 channels each
   const int py1 = up1 ? d.py / 2 : d.py, pz1 = up1 ? d.pz / 2 : d.pz;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, r = lane & 15;
   const int nbk = (d.pz + 15) / 16, nbj = (d.py + 3) / 4;
@@ -227,7 +142,7 @@ __global__ __launch_bounds__(256) void k_conv_brick(
 #pragma unroll
     for (int u = 0; u < 6; ++u) {
       const int tt = threadIdx.x + 256 * u;
-      if (tt < 2 * kBrickHalo) sx[(tt & 1) * kBrickHalo + (tt >> 1)] = hv[u];
+      if (tt < 2 * kBrickHalo) sx[(tt & 1) * kBrickHalf + (tt >> 1)] = hv[u];
     }
     __syncthreads();
     if (t + 1 < items) load_halo(t + 1, hv);  // in flight during this item's MFMAs
@@ -236,7 +151,7 @@ __global__ __launch_bounds__(256) void k_conv_brick(
       const int tap = 2 * c + (g >> 1);
       const int tt = tap < 27 ? tap : 26;  // the padding tap reads a valid slot (times 0)
       const int di = tt / 9, dj = (tt / 3) % 3, dk = tt % 3;
-      const int base = (g & 1) * kBrickHalo + ((w + di) * 6 + dj) * 18 + r + dk;
+      const int base = (g & 1) * kBrickHalf + ((w + di) * 6 + dj) * 18 + r + dk;
 #pragma unroll
       for (int m = 0; m < 4; ++m) a[m].u = sx[base + m * 18];
     };
@@ -244,20 +159,13 @@ __global__ __launch_bounds__(256) void k_conv_brick(
     for (int n = 0; n < NP; ++n) {
       Frag8(&bc)[14] = (n & 1) ? bB : bA;
       if (n + 1 < NP) load_b(gr, n + 1, (n & 1) ? bA : bB);
-      // the next pair's fragments are read while this pair's four MFMAs run
-      Frag8 a0[4], a1[4];
-      read_a(0, a0);
 #pragma unroll
-      for (int c = 0; c < 14; c += 2) {
-        read_a(c + 1, a1);
+      for (int c = 0; c < 14; ++c) {
+        Frag8 a[4];
+        read_a(c, a);
 #pragma unroll
         for (int m = 0; m < 4; ++m)
-          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[m].v, bc[c].v, acc[m][n], 0, 0, 0);
-        if (c + 2 < 14) read_a(c + 2, a0);
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-          acc[m][n] =
-              __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[m].v, bc[c + 1].v, acc[m][n], 0, 0, 0);
+          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m].v, bc[c].v, acc[m][n], 0, 0, 0);
       }
     }
     if ((t + 1) % per) continue;  // more groups of this brick follow
@@ -265,47 +173,44 @@ __global__ __launch_bounds__(256) void k_conv_brick(
     // column r) out, accumulators cleared for the next brick
     int i0, j0, k0;
     brick_of(t, i0, j0, k0);
+    auto put = [&](int64_t v, int co, float val) {
+      if (bias) val += bias[co];
+      uint16_t* dst =
+          co < ysplit ? y1 + v * ysplit + co : y2 + v * (cout - ysplit) + (co - ysplit);
+      if (accum) val += bf2f(*dst);
+      if (relu) val = fmaxf(val, 0.f);
+      *dst = f2bf(val);
+    };
+    if (!part) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < NP; ++n) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int kk = k0 + 4 * g + e;
+            if (kk >= d.pz || i0 + w >= d.px || j0 + m >= d.py) continue;  // past the grid
+            put(((int64_t)(i0 + w) * d.py + (j0 + m)) * d.pz + kk, (nt0 + n) * 16 + r,
+                acc[m][n][e]);
+          }
+          acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+      continue;
+    }
+    // split: this split's FP32 partial sums (k_conv_finish adds them)
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
       for (int n = 0; n < NP; ++n) {
-        const int co = (nt0 + n) * 16 + r;
-        const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int kk = k0 + 4 * g + e;
-          if (kk >= d.pz || i0 + w >= d.px || j0 + m >= d.py) continue;  // past the grid
+          if (kk >= d.pz || i0 + w >= d.px || j0 + m >= d.py) continue;
           const int64_t v = ((int64_t)(i0 + w) * d.py + (j0 + m)) * d.pz + kk;
-          if (part) {
-            part[((int64_t)blockIdx.z * d.nv + v) * cout + co] = acc[m][n][e];
-            continue;
-          }
-          float val = acc[m][n][e] + bv;
-          uint16_t* dst =
-              co < ysplit ? y1 + v * ysplit + co : y2 + v * (cout - ysplit) + (co - ysplit);
-          if (accum) val += bf2f(*dst);
-          if (relu) val = fmaxf(val, 0.f);
-          *dst = f2bf(val);
+          part[((int64_t)blockIdx.z * d.nv + v) * cout + (nt0 + n) * 16 + r] = acc[m][n][e];
         }
         acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
       }
-  }
-  // a split with no groups still owes zero partial sums for its bricks
-  if (part && per <= 0) {
-    for (int b = blockIdx.x; b < nbricks; b += gridDim.x) {
-      const int bk = b % nbk, bj = (b / nbk) % nbj, bi = b / (nbk * nbj);
-#pragma unroll
-      for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int n = 0; n < NP; ++n)
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int kk = 16 * bk + 4 * g + e, ii = 4 * bi + w, jj = 4 * bj + m;
-            if (kk >= d.pz || ii >= d.px || jj >= d.py) continue;
-            const int64_t v = ((int64_t)ii * d.py + jj) * d.pz + kk;
-            part[((int64_t)blockIdx.z * d.nv + v) * cout + (nt0 + n) * 16 + r] = 0.f;
-          }
-    }
   }
 }
 
@@ -325,6 +230,70 @@ __global__ void k_conv_finish(const float* __restrict__ part, int ks, VoxDims d,
   if (accum) val += bf2f(*dst);
   if (relu) val = fmaxf(val, 0.f);
   *dst = f2bf(val);
+}
+
+// Pointwise 16 -> 16 channel layer (the network head and its data gradient), one thread per
+// voxel: y[v][co] = act(bias[co] + sum_ci W[co][ci] x[v][ci]) (+ the existing y with accum), FP32
+// sums in ci order; the weights (FP32, staged in LDS) are read as block-wide broadcasts, the
+// voxel's 16 channels are two 16-byte loads and stores. Memory-bound: 64 B per voxel.
+__global__ __launch_bounds__(256) void k_pointwise16(const uint16_t* __restrict__ x,
+                                                     const uint16_t* __restrict__ W,
+                                                     const float* __restrict__ bias,
+                                                     uint16_t* __restrict__ y, int relu,
+                                                     int accum, int64_t nv) {
+  __shared__ float4 sw[16 * 4 + 4];  // W[co][ci] as 4 x float4 per co, then the bias
+  {
+    float* f = (float*)sw;
+    f[threadIdx.x] = bf2f(W[threadIdx.x]);
+    if (threadIdx.x < 16) f[256 + threadIdx.x] = bias ? bias[threadIdx.x] : 0.f;
+  }
+  __syncthreads();
+  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (v >= nv) return;
+  const uint4* xp = (const uint4*)(x + v * 16);
+  const uint4 u0 = xp[0], u1 = xp[1];
+  const uint32_t q[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+  float xf[16];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    xf[2 * i] = bf2f((uint16_t)(q[i] & 0xFFFFu));
+    xf[2 * i + 1] = bf2f((uint16_t)(q[i] >> 16));
+  }
+  float yo[16];
+  if (accum) {
+    const uint4* yp = (const uint4*)(y + v * 16);
+    const uint4 a0 = yp[0], a1 = yp[1];
+    const uint32_t p[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      yo[2 * i] = bf2f((uint16_t)(p[i] & 0xFFFFu));
+      yo[2 * i + 1] = bf2f((uint16_t)(p[i] >> 16));
+    }
+  }
+  uint32_t o[8];
+#pragma unroll
+  for (int co = 0; co < 16; ++co) {
+    float s = 0.f;
+#pragma unroll
+    for (int c4 = 0; c4 < 4; ++c4) {
+      const float4 w4 = sw[co * 4 + c4];
+      s = fmaf(w4.x, xf[4 * c4], s);
+      s = fmaf(w4.y, xf[4 * c4 + 1], s);
+      s = fmaf(w4.z, xf[4 * c4 + 2], s);
+      s = fmaf(w4.w, xf[4 * c4 + 3], s);
+    }
+    float val = s + ((const float*)sw)[256 + co];
+    if (accum) val += yo[co];
+    if (relu) val = fmaxf(val, 0.f);
+    const uint32_t b = f2bf(val);
+    if (co & 1)
+      o[co >> 1] |= b << 16;
+    else
+      o[co >> 1] = b;
+  }
+  uint4* yq = (uint4*)(y + v * 16);
+  yq[0] = uint4{o[0], o[1], o[2], o[3]};
+  yq[1] = uint4{o[4], o[5], o[6], o[7]};
 }
 
 // ---------------------------------------------------------------------------------------

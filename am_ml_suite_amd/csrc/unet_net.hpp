@@ -62,6 +62,8 @@ inline BrickCfg brick_cfg(const VoxDims& d, int cin, int cout) {
   c.ks = 1;
   while (c.ks < ngr && c.bricks * c.ny * c.ks < 512) c.ks *= 2;  // two blocks per CU
   c.ks = std::min(c.ks, ngr);
+  const int gpk = (ngr + c.ks - 1) / c.ks;  // groups per split; no split left empty
+  c.ks = (ngr + gpk - 1) / gpk;
   return c;
 }
 // FP32 partial sums the split launches need (floats).
@@ -201,8 +203,9 @@ inline void refresh_bf16(amt_unet* n) {
 }
 
 // y = conv(x1 ++ x2) with the layer's weights; nt = output channels / 16. 3x3x3 layers run as
-// brick convolutions (k_conv_brick, split as brick_cfg says, partial sums in n->part); the
-// pointwise head runs as k_conv (MT voxel tiles of 16 and NTW channel tiles per wave).
+// brick convolutions (k_conv_brick, split as brick_cfg says, partial sums in n->part added by
+// k_conv_finish); the
+// pointwise head (16 -> 16 channels, one input tensor, one output tensor) as k_pointwise16.
 inline void conv_launch(amt_unet* n, int taps, int nt, const uint16_t* x1, int c1,
                         const uint16_t* x2, int c2, const uint16_t* W, const float* bias,
                         uint16_t* y1, uint16_t* y2, int ysplit, int relu, int accum,
@@ -227,17 +230,12 @@ inline void conv_launch(amt_unet* n, int taps, int nt, const uint16_t* x1, int c
                          accum);
     return;
   }
-  const int64_t mtiles = ceil_div(d.nv, 16);
-#define AMT_CONV_CASE(T, N, MT, NTW)                                                        \
-  if (taps == T && nt == N) {                                                               \
-    const int wm = 4 / (N / NTW);                                                           \
-    const dim3 grid((unsigned)ceil_div(mtiles, (int64_t)wm * MT)), blk(256);                \
-    hipLaunchKernelGGL((amt::k_conv<T, MT, NTW>), grid, blk, 0, st, x1, c1, x2, c2, W, bias, \
-                       y1, y2, ysplit, relu, accum, d, cout);                               \
-    return;                                                                                 \
-  }
-  AMT_CONV_CASE(1, 1, 2, 1)
-#undef AMT_CONV_CASE
+  // the pointwise head (16 -> 16) and its data gradient; the only 1-tap layer
+  (void)x2;
+  (void)y2;
+  (void)ysplit;
+  hipLaunchKernelGGL(amt::k_pointwise16, dim3((unsigned)ceil_div(d.nv, 256)), dim3(256), 0, st,
+                     x1, W, bias, y1, relu, accum, d.nv);
 }
 
 // up1: in1 is the coarse tensor whose upsampling the layer reads (not written out).
