@@ -16,6 +16,11 @@ With --paths ...,large the "large" object reports the 256x128x128 bracket (12,83
 100 OC iterations with the V-cycle's fine-level smoothing on the matrix cores (split-BF16
 MFMA operator, FP32 accumulation) against the same run with FP32 vector smoothing (FP64
 flexible CG either way), with the density max-abs between them.
+The "learned" object (default) reports the U-Net: warm_start trains it on the warm-start
+residuals of 8 other 120x60x60 problems and times 12 OC iterations of the headline problem with
+and without the learned step u += gamma M(r) before each multigrid solve; fcg_96x48x48 trains it
+as a flexible-CG preconditioner on Jacobi-PCG residuals and solves a held-out 96x48x48 problem
+against plain Jacobi-PCG and the multigrid FCG.
 The "batch" object reports the dataset-generation workload: 64 generated problems of sizes
 U{16..64}^3 (13.0M DOFs together) advanced in lockstep by the batched engine (FP32 matvec,
 FP64 vectors and reductions, residual replacement, every solve to the 1e-8 true residual).
@@ -33,6 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per k_fcg_apply launch on this workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
 # scaled by a plain 256 MiB copy (profiles/fcg_apply_bracket120x60x60_hbm_traffic.md)
@@ -247,6 +253,75 @@ def run_learned(shape=(96, 48, 48), train_steps=400, samples=64):
     return out
 
 
+def run_learned_warm(n_seeds=8, oc_train=40, steps_per_seed=500, eval_iters=12):
+    """The learned warm start on the headline workload (120x60x60, multigrid flexible CG): the
+    U-Net is trained on the warm-start residuals r = F - K u_prev of n_seeds other bracket-like
+    problems (seeds 11..), one sample per OC iteration, steps_per_seed Adam steps after each
+    problem; then the seed-2 problem runs eval_iters OC iterations with and without the learned
+    step u += gamma M(r) before each warm multigrid solve. Reports wall time per OC iteration,
+    FCG iterations, compliance agreement and the network's forward throughput on this grid."""
+    import am_ml_suite_amd as A
+    from am_ml_suite_amd.problems import make_config
+    shape = (120, 60, 60)
+
+    def handle(seed, iters):
+        p = A.bracket_like_problem(*shape, seed=seed)
+        o = A.TopologyOptimizer(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3),
+                                make_config(p, preconditioner="multigrid", warm_start=True,
+                                            max_iterations=iters))
+        o.set_loads(p.F, p.fixed)
+        o.set_density(p.rho0)
+        o.begin_run()
+        return o
+
+    t0 = time.perf_counter()
+    net, total = None, 0
+    for s in range(n_seeds):
+        o = handle(11 + s, oc_train)
+        if net is None:
+            net = A.UNet(optimizer=o, seed=3)
+        o.collect_warm_samples(oc_train)
+        for _ in range(oc_train):
+            o.step()
+        total += o.collect_warm_samples(oc_train)
+        o.train_unet(net, steps_per_seed, lr=1e-3)
+        o.close()
+    t_train = time.perf_counter() - t0
+    fwd_ms, fwd_tf = net.benchmark(10)
+    runs = {}
+    for key, use in (("multigrid", False), ("learned_multigrid", True)):
+        o = handle(2, eval_iters)
+        if use:
+            o.attach_unet(net)
+        o.sync()
+        t0 = time.perf_counter()
+        res = [o.step() for _ in range(eval_iters)]
+        o.sync()
+        dt = time.perf_counter() - t0
+        runs[key] = dict(ms_per_oc=1e3 * dt / eval_iters,
+                         fcg_iters=[r.solver_iterations for r in res],
+                         max_true_rel_residual=max(r.true_rel_residual for r in res),
+                         compliance=[r.compliance for r in res])
+        o.close()
+    cm = np.array(runs["multigrid"].pop("compliance"))
+    cl = np.array(runs["learned_multigrid"].pop("compliance"))
+    out = dict(workload="bracket_like_120x60x60_seed2", oc_iterations=eval_iters,
+               train_problems=n_seeds, train_samples=total,
+               train_steps=n_seeds * steps_per_seed, train_s=t_train,
+               unet_forward_ms=fwd_ms, unet_conv_tflops=fwd_tf,
+               unet_conv_frac_bf16_peak=fwd_tf / BF16_PEAK_TFLOPS,
+               max_rel_compliance_diff=float(np.abs(cl / cm - 1).max()),
+               speedup_learned_vs_multigrid=(runs["multigrid"]["ms_per_oc"]
+                                             / runs["learned_multigrid"]["ms_per_oc"]),
+               **runs)
+    log("  [learned warm start] %d samples, %d steps, %.1fs; multigrid %.2f ms/OC (%.1f FCG its), "
+        "learned+multigrid %.2f ms/OC (%.1f FCG its); U-Net forward %.3f ms, %.0f TFLOP/s"
+        % (total, n_seeds * steps_per_seed, t_train, runs["multigrid"]["ms_per_oc"],
+           np.mean(runs["multigrid"]["fcg_iters"]), runs["learned_multigrid"]["ms_per_oc"],
+           np.mean(runs["learned_multigrid"]["fcg_iters"]), fwd_ms, fwd_tf))
+    return out
+
+
 def run_batch(warmup, steps, solver="jacobi"):
     import am_ml_suite_amd as A
     from am_ml_suite_amd.problems import make_config
@@ -321,7 +396,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--paths", default="plain,fast,batch,simp_py")
+    ap.add_argument("--paths", default="plain,fast,learned,batch,simp_py")
     ap.add_argument("--large-steps", type=int, default=100)
     ap.add_argument("--fast-smoother", choices=("fp32", "mfma"), default="fp32",
                     help="fine-level smoothing of the fast path's V-cycle: the FP32 vector "
@@ -365,8 +440,10 @@ def main():
         large = run_large(args.large_steps)
     learned = None
     if "learned" in paths:
+        log("learned: U-Net warm start on the 120x60x60 workload")
+        learned = dict(warm_start=run_learned_warm())
         log("learned: U-Net preconditioner on 96x48x48")
-        learned = run_learned()
+        learned["fcg_96x48x48"] = run_learned(train_steps=2000)
     ai_line = None
     if "ai" in paths:
         log("ai: density predictor")
@@ -458,6 +535,13 @@ def main():
                               if main_key == "fast" else "FP64 Jacobi-PCG")},
         "roofline": roofline, "cpu_baseline": cpu, "batch": batch,
     }
+    if main_key == "fast":
+        line["metric_note"] = (
+            "value is the mixed-precision path without a network: FP64 flexible CG, FP32 "
+            "Galerkin multigrid V-cycle, warm start from the previous OC iteration. The learned "
+            "part (U-Net warm start trained on the GPU) is timed on the same workload in "
+            "learned.warm_start, against this path without it (speedup_learned_vs_multigrid), "
+            "and is not in value")
     if large is not None:
         line["large"] = large
     if simp_py is not None:
