@@ -40,21 +40,25 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per k_fcg_apply launch on this workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-# scaled by a plain 256 MiB copy (profiles/fcg_apply_bracket120x60x60_hbm_traffic.md)
-FCG_APPLY_TRAFFIC_BYTES = 49.39e6
+# HBM bytes per launch of the fast path's FP64 operator on this workload, by operator candidate
+# (amt_config.operator_tile), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a plain
+# 256 MiB copy (profiles/fcg_apply_bracket120x60x60_hbm_traffic.md)
+FCG_APPLY_TRAFFIC_BYTES = {0: 49.39e6, 1: 73.98e6, 2: 66.39e6, 3: 62.07e6, 4: 66.81e6,
+                          5: 62.17e6, 6: 58.51e6, 7: 60.65e6, 8: 58.30e6, 9: 55.18e6}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_path(problem, warmup, steps, precision, warm_start, profile, precond="jacobi"):
+def run_path(problem, warmup, steps, precision, warm_start, profile, precond="jacobi",
+             operator_tile=-1):
     import am_ml_suite_amd as A
     from am_ml_suite_amd.problems import make_config
     p = problem
     cfg = make_config(p, precision=precision, warm_start=warm_start,
-                      max_iterations=warmup + steps, preconditioner=precond)
+                      max_iterations=warmup + steps, preconditioner=precond,
+                      operator_tile=operator_tile)
     opt = A.TopologyOptimizer(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3), cfg)
     rho = p.rho0.copy()
     opt.set_loads(p.F, p.fixed)
@@ -80,8 +84,9 @@ def run_path(problem, warmup, steps, precision, warm_start, profile, precond="ja
     dt = time.perf_counter() - t0
     prof = opt.profile_read() if profile else (0, 0.0)
     dens = opt.get_density()
+    op = opt.operator_info(0) if precond == "multigrid" else None
     opt.close()
-    return dict(time=dt, results=res, density=dens, prof=prof)
+    return dict(time=dt, results=res, density=dens, prof=prof, op=op)
 
 
 def run_large(steps):
@@ -401,6 +406,9 @@ def main():
     ap.add_argument("--fast-smoother", choices=("fp32", "mfma"), default="fp32",
                     help="fine-level smoothing of the fast path's V-cycle: the FP32 vector "
                          "operator (default) or the split-BF16 matrix-core operator")
+    ap.add_argument("--operator-tile", type=int, default=-1,
+                    help="level-0 operator launch of the fast path (amt_config.operator_tile): "
+                         "-1 autotuned (default), or a fixed candidate (for counter runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a plain copy kernel of "
                          "known size, to scale the FETCH_SIZE / WRITE_SIZE counters)")
@@ -433,7 +441,7 @@ def main():
         log("fast: FP64 flexible CG + FP32 multigrid V-cycle + warm start")
         out["fast"] = run_path(p, args.warmup, args.steps,
                                "mixed_bf16" if args.fast_smoother == "mfma" else "fp64", True,
-                               True, "multigrid")
+                               True, "multigrid", args.operator_tile)
 
     large = None
     if "large" in paths:
@@ -487,10 +495,17 @@ def main():
     if launches:
         avg_s = ms / launches / 1e3
         achieved = bytes_per_launch / avg_s / 1e9
+        kernel, traffic = "k_pcg_apply<double>", None
+        if main_key == "fast":
+            op = m["op"]
+            if op["candidate"] == 0:
+                kernel = "k_fcg_apply (4x4x16 node gather tile)"
+            else:
+                kernel = ("k_fcg_apply_s<%d, %d> (element-column streaming tile, %d node planes)"
+                          % (op["tj"], op["tk"], op["planes"]))
+            traffic = FCG_APPLY_TRAFFIC_BYTES.get(op["candidate"])
         roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=achieved / HBM_PEAK_GBS,
-                        traffic=FCG_APPLY_TRAFFIC_BYTES if main_key == "fast" else None,
-                        kernel="k_fcg_apply" if main_key == "fast" else "k_pcg_apply<double>",
+                        frac=achieved / HBM_PEAK_GBS, traffic=traffic, kernel=kernel,
                         avg_launch_us=avg_s * 1e6, timed_launches=launches,
                         algorithmic_bytes_per_launch=bytes_per_launch,
                         flops_per_launch=flops_per_launch,
