@@ -123,6 +123,7 @@ struct amt_problem {
   StencilEntry* stencil = nullptr;
   double* stencil_w = nullptr;
   int n_stencil = 0;
+  int stencil_reach = 0;  // max |offset| of the stencil (the LDS-tiled filter takes <= 3)
   double* partials = nullptr;
   int64_t region_off[R_COUNT] = {0};
   int stride_apply = 0, stride_red = 0;
@@ -253,6 +254,24 @@ void prof_end(amt_problem* p, hipEvent_t a);
 void prof_harvest(amt_problem* p);
 
 // ---- operator coefficients for the current density ("assembly") ----
+// dcf = filtered dc: the LDS-tiled kernel for stencils reaching at most 3 elements, the
+// global-gather one beyond (same result bit for bit).
+void launch_filter(amt_problem* p) {
+  const int R = p->stencil_reach;
+  if (R <= 3) {
+    int ty, tz;
+    filter_tile_dims(R, ty, tz);
+    const GridDims& g = p->g;
+    const int64_t blocks = (int64_t)((g.nx + kFilterTX - 1) / kFilterTX) * ((g.ny + ty - 1) / ty) *
+                           ((g.nz + tz - 1) / tz);
+    hipLaunchKernelGGL(k_filter_tile, dim3((unsigned)blocks), dim3(kBlock), 0, p->stream, p->dc,
+                       p->rho, p->stencil, p->stencil_w, p->n_stencil, g, R, ty, tz, p->dcf);
+  } else {
+    hipLaunchKernelGGL(k_filter, dim3(nblk(p->n_e)), dim3(kBlock), 0, p->stream, p->dc, p->rho,
+                       p->stencil, p->stencil_w, p->n_stencil, p->g, p->dcf);
+  }
+}
+
 int prepare_coeffs(amt_problem* p) {
   const GridDims g = p->g;
   hipLaunchKernelGGL(k_simp, dim3(nblk(p->n_e)), dim3(kBlock), 0, p->stream, p->rho, p->s64,
@@ -1323,8 +1342,7 @@ int run_iteration(amt_problem* p, amt_iteration_result* out) {
   AMT_CHECK(check_launch(p, "sensitivities"));
   if (p->cfg.verbose) HIP_TRY(p, hipStreamSynchronize(p->stream));
   const double t3 = now_s();
-  hipLaunchKernelGGL(k_filter, dim3(nblk(p->n_e)), dim3(kBlock), 0, p->stream, p->dc, p->rho,
-                     p->stencil, p->stencil_w, p->n_stencil, p->g, p->dcf);
+  launch_filter(p);
   AMT_CHECK(oc_update(p, nullptr));
   // one host read per OC iteration after the solve: OC state, compliance, true residual
   hipLaunchKernelGGL(k_iter_scalars, dim3(1), dim3(64), 0, p->stream, p->oc, p->scal + S_COMPL,
@@ -1515,6 +1533,8 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
   std::vector<double> sw;
   build_stencil(cfg->filter_radius, st, sw);
   p->n_stencil = (int)st.size();
+  for (const auto& o : st)
+    p->stencil_reach = std::max({p->stencil_reach, std::abs(o.dx), std::abs(o.dy), std::abs(o.dz)});
   CREATE_TRY(dalloc(p, &p->stencil, st.size()));
   CREATE_TRY(dalloc(p, &p->stencil_w, sw.size()));
   CREATE_TRY(dalloc(p, &p->dbg_idx, 1));
@@ -1759,9 +1779,8 @@ int amt_sensitivities(amt_problem* p, const double* u, double* dc) {
 int amt_filter(amt_problem* p, const double* dc, double* dcf) {
   if (!p || !dc || !dcf) return AMT_EINVAL;
   AMT_CHECK(copy_in(p, p->dc, dc, sizeof(double) * p->n_e));
-  hipLaunchKernelGGL(k_filter, dim3(nblk(p->n_e)), dim3(kBlock), 0, p->stream, p->dc, p->rho,
-                     p->stencil, p->stencil_w, p->n_stencil, p->g, p->dcf);
-  AMT_CHECK(check_launch(p, "k_filter"));
+  launch_filter(p);
+  AMT_CHECK(check_launch(p, "filter"));
   return copy_out(p, dcf, p->dcf, sizeof(double) * p->n_e);
 }
 

@@ -382,6 +382,59 @@ __global__ __launch_bounds__(kBlock) void k_filter(const double* __restrict__ dc
   dcf[e] = den > 1e-15 ? num / den : dc[e];
 }
 
+// LDS-tiled sensitivity filter: k_filter's operation, bit for bit. A block owns a 32 x ty x tz
+// element tile (x, the slot-fastest axis, along the threads) and stages rho and dc of the tile and
+// a halo of R elements (R = the stencil's reach, <= 3) into LDS with coalesced loads. Each thread
+// then walks the stencil for its elements in the same order, with the same bounds test and the
+// same arithmetic (no contraction), reading the neighbours from LDS; the stencil table is read
+// with block-uniform loads. Tile by reach: R <= 1: 32 x 8 x 8; R = 2: 32 x 8 x 4; R = 3:
+// 32 x 4 x 4 (halo at most 38 x 10 x 10 = 3,800 elements, 61 KB).
+constexpr int kFilterTX = 32;
+constexpr int kFilterLds = 3840;
+inline void filter_tile_dims(int R, int& ty, int& tz) {
+  ty = R <= 2 ? 8 : 4;
+  tz = R <= 1 ? 8 : 4;
+}
+__global__ __launch_bounds__(kBlock) void k_filter_tile(const double* __restrict__ dc,
+                                                        const double* __restrict__ rho,
+                                                        const StencilEntry* __restrict__ st,
+                                                        const double* __restrict__ w, int n_st,
+                                                        GridDims g, int R, int ty, int tz,
+                                                        double* __restrict__ dcf) {
+#pragma clang fp contract(off)
+  __shared__ double srho[kFilterLds], sdc[kFilterLds];
+  const int hx = kFilterTX + 2 * R, hy = ty + 2 * R, hz = tz + 2 * R;
+  const int ntx = (g.nx + kFilterTX - 1) / kFilterTX, nty = (g.ny + ty - 1) / ty;
+  const int b = blockIdx.x;
+  const int x0 = (b % ntx) * kFilterTX, y0 = ((b / ntx) % nty) * ty, z0 = (b / (ntx * nty)) * tz;
+  const int64_t nxy = (int64_t)g.nx * g.ny;
+  for (int q = threadIdx.x; q < hx * hy * hz; q += kBlock) {
+    const int gx = x0 - R + q % hx, gy = y0 - R + (q / hx) % hy, gz = z0 - R + q / (hx * hy);
+    const bool in = gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz;
+    const int64_t m = in ? (int64_t)gz * nxy + (int64_t)gy * g.nx + gx : 0;
+    srho[q] = in ? rho[m] : 0.0;  // halo cells outside the grid are never read
+    sdc[q] = in ? dc[m] : 0.0;
+  }
+  __syncthreads();
+  for (int le = threadIdx.x; le < kFilterTX * ty * tz; le += kBlock) {
+    const int lx = le % kFilterTX, ly = (le / kFilterTX) % ty, lz = le / (kFilterTX * ty);
+    const int ix = x0 + lx, iy = y0 + ly, iz = z0 + lz;
+    if (ix >= g.nx || iy >= g.ny || iz >= g.nz) continue;
+    const int c0 = ((lz + R) * hy + (ly + R)) * hx + (lx + R);
+    double num = 0.0, den = 0.0;
+    for (int t = 0; t < n_st; ++t) {
+      const StencilEntry o = st[t];
+      const int cx = ix + o.dx, cy = iy + o.dy, cz = iz + o.dz;
+      if (cx < 0 || cx >= g.nx || cy < 0 || cy >= g.ny || cz < 0 || cz >= g.nz) continue;
+      const int h = c0 + (o.dz * hy + o.dy) * hx + o.dx;
+      const double wr = w[t] * srho[h];
+      num = num + wr * sdc[h];
+      den = den + wr;
+    }
+    dcf[(int64_t)iz * nxy + (int64_t)iy * g.nx + ix] = den > 1e-15 ? num / den : sdc[c0];
+  }
+}
+
 // Writes the neighbour list of slot e exactly as k_filter walks it.
 __global__ void k_filter_list(int64_t e, const StencilEntry* __restrict__ st,
                               const double* __restrict__ w, int n_st, GridDims g,

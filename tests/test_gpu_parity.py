@@ -130,9 +130,13 @@ def test_filter_neighbour_lists_bit_exact(r):
         assert np.array_equal(gw, w[offs[e]:offs[e + 1]])  # bit-exact weights
 
 
-@pytest.mark.parametrize("r", [1.5, 3.5])
-def test_filter_values_match_oracle(r):
-    nx, ny, nz = 11, 8, 6
+@pytest.mark.parametrize("shape", [(11, 8, 6), (70, 17, 11)])
+@pytest.mark.parametrize("r", [0.9, 1.2, 1.5, 2.0, 2.5, 3.5, 4.2])
+def test_filter_values_match_oracle(r, shape):
+    """Filtered sensitivities against the oracle, bit for bit: the LDS-tiled kernel (stencil
+    reach 0..3 at r = 0.9..3.5; 70 x 17 x 11 spans several ragged 32 x ty x tz tiles per axis) and
+    the global-gather kernel (reach 4 at r = 4.2)."""
+    nx, ny, nz = shape
     rng = np.random.default_rng(6)
     rho = rng.uniform(0.0, 1.0, nx * ny * nz)
     rho[:10] = 0.0
