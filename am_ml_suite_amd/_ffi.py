@@ -20,7 +20,7 @@ EXPORTS = [
     "amt_config_default", "amt_create", "amt_destroy", "amt_last_error", "amt_set_loads",
     "amt_set_density", "amt_get_density", "amt_get_displacement", "amt_begin_run",
     "amt_run_iteration", "amt_run", "amt_apply_operator", "amt_apply_operator_prec",
-    "amt_get_diagonal", "amt_solve",
+    "amt_get_diagonal", "amt_solve", "amt_apply_multigrid",
     "amt_sensitivities", "amt_filter", "amt_filter_neighbors", "amt_oc_update", "amt_sync",
     "amt_device_info", "amt_profile_enable", "amt_profile_read", "amt_element_stiffness",
     "amt_unet_create", "amt_unet_create_for", "amt_unet_destroy", "amt_unet_num_params",
@@ -48,6 +48,7 @@ class AmtConfig(C.Structure):
         ("density_index", C.c_int), ("precision", C.c_int), ("precond", C.c_int),
         ("warm_start", C.c_int), ("inner_tol", C.c_double), ("inner_maxit", C.c_int),
         ("verbose", C.c_int), ("use_graphs", C.c_int), ("operator_tile", C.c_int),
+        ("mg_fused", C.c_int),
     ]
 
 
@@ -126,6 +127,7 @@ def load():
     L.amt_apply_operator_prec.argtypes = [P, _dp, _dp, C.c_int, C.c_int]
     L.amt_get_diagonal.argtypes = [P, _dp, C.c_int, C.POINTER(C.c_double)]
     L.amt_solve.argtypes = [P, _dp, C.POINTER(AmtSolveStats)]
+    L.amt_apply_multigrid.argtypes = [P, _dp, _dp]
     L.amt_sensitivities.argtypes = [P, _dp, _dp]
     L.amt_filter.argtypes = [P, _dp, _dp]
     L.amt_filter_neighbors.argtypes = [P, C.c_int64, C.c_void_p, C.c_void_p, C.c_int,

@@ -139,10 +139,12 @@ struct amt_problem {
   // geometric multigrid (precond == AMT_PRECOND_MULTIGRID)
   struct MgLevel {
     MgGrid g;
+    MgBrick brick{};     // 4x4x4-node bricks of the fused kernels (levels >= 1)
     float* A = nullptr;  // 27-point block stencil (levels >= 1)
     float *wd = nullptr, *rhs = nullptr, *x = nullptr, *t = nullptr;
   };
   std::vector<MgLevel> mg;
+  int mg_fuse_from = 0;       // first level of the fused V-cycle kernels (0: none)
   double* mg_G = nullptr;     // 8 x 576 Galerkin child tables
   float* mg_ainv = nullptr;   // coarsest dense inverse
   int mg_nc = 0;              // coarsest DOFs
@@ -356,6 +358,9 @@ int mg_build_hierarchy(amt_problem* p, const double* Ke) {
     if (f.ex == 1 && f.ey == 1 && f.ez == 1 && p->mg.size() > 1) break;
     amt_problem::MgLevel L;
     L.g = mg_grid((f.ex + 1) / 2, (f.ey + 1) / 2, (f.ez + 1) / 2);
+    L.brick.nbi = (L.g.nnx + kFB - 1) / kFB;
+    L.brick.nbj = (L.g.nny + kFB - 1) / kFB;
+    L.brick.nbk = (L.g.nnz + kFB - 1) / kFB;
     const int64_t nd = 3 * L.g.n_nodes;
     AMT_CHECK(dalloc(p, &L.A, (size_t)9 * kStencilBlocks * L.g.n_nodes));
     AMT_CHECK(dalloc(p, &L.wd, nd));
@@ -368,6 +373,17 @@ int mg_build_hierarchy(amt_problem* p, const double* Ke) {
   AMT_CHECK(dalloc(p, &p->rr[1], p->n_dof));
   p->mg_nc = (int)(3 * p->mg.back().g.n_nodes);
   if (p->mg_nc > kMgCoarseDofs) return fail(p, AMT_EINVAL, "multigrid coarsest level too large");
+  // fused kernels on the stencil levels of at most kFuseMaxNodes nodes (the levels shrink, so
+  // they are a tail of the hierarchy), when there is a stencil level above the dense one
+  p->mg_fuse_from = 0;
+  const int nl = (int)p->mg.size();
+  if (p->cfg.mg_fused && nl >= 3 && p->mg_nc <= kDenseMax) {
+    for (int l = 1; l <= nl - 2; ++l)
+      if (p->mg[l].g.n_nodes <= kFuseMaxNodes) {
+        p->mg_fuse_from = l;
+        break;
+      }
+  }
   AMT_CHECK(dalloc(p, &p->mg_ainv, (size_t)p->mg_nc * p->mg_nc));
   std::vector<double> G(8 * 576);
   mg_child_tables(Ke, G.data());
@@ -671,21 +687,47 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q,
                        mg[0].x, mg[0].t, done);
   else
     launch_mg0_pre(p, p->op_post, r, done);
-  // each restriction also writes the next level's pre-smoothing step x0 = wd * rhs (not for
-  // the dense coarsest level)
-  hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[1].g.n_nodes)), dim3(kBlock), 0, p->stream,
-                     mg[0].t, mg[0].g, mg[1].rhs, mg[1].g, L > 2 ? mg[1].wd : nullptr, mg[1].x);
-  for (size_t l = 1; l + 1 < L; ++l) {
+  // Levels 1 .. f-1 run one launch per step (each restriction also writes the next level's
+  // pre-smoothing step x0 = wd * rhs); levels f .. L-2 and the dense level run the fused
+  // kernels (k_mg_down / k_mg_rdense / k_mg_up). Without fused levels, f = L - 1: every
+  // stencil level runs the one-step kernels and the dense solve is its own launch.
+  const int nl = (int)L;
+  const bool fused = p->mg_fuse_from > 0;
+  const int f = fused ? p->mg_fuse_from : nl - 1;
+  if (!fused || f > 1)
+    hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[1].g.n_nodes)), dim3(kBlock), 0, p->stream,
+                       mg[0].t, mg[0].g, mg[1].rhs, mg[1].g, nl > 2 ? mg[1].wd : nullptr, mg[1].x);
+  for (int l = 1; l < f; ++l) {
     hipLaunchKernelGGL(k_mg_pre, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
                        mg[l].A, mg[l].g, mg[l].rhs, mg[l].x, mg[l].t);
-    hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[l + 1].g.n_nodes)), dim3(kBlock), 0,
-                       p->stream, mg[l].t, mg[l].g, mg[l + 1].rhs, mg[l + 1].g,
-                       l + 2 < L ? mg[l + 1].wd : nullptr, mg[l + 1].x);
+    if (!fused || l + 1 < f)
+      hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[l + 1].g.n_nodes)), dim3(kBlock), 0,
+                         p->stream, mg[l].t, mg[l].g, mg[l + 1].rhs, mg[l + 1].g,
+                         l + 2 < nl ? mg[l + 1].wd : nullptr, mg[l + 1].x);
   }
-  hipLaunchKernelGGL(k_mg_dense, dim3((p->mg_nc + 8 * kWaves - 1) / (8 * kWaves)), dim3(kBlock),
-                     0, p->stream, p->mg_ainv, p->mg_nc, mg[L - 1].rhs, mg[L - 1].x);
-  const float* res = mg[L - 1].x;  // correction of the level below the current one
-  for (size_t l = L - 2; l >= 1; --l) {
+  const float* res;  // correction of the level below the current one
+  if (fused) {
+    auto bgrid = [](const MgBrick& b) { return dim3(8u * (unsigned)((b.nbi * b.nbj * b.nbk + 7) / 8)); };
+    for (int l = f; l <= nl - 2; ++l)
+      hipLaunchKernelGGL(k_mg_down, bgrid(mg[l].brick), dim3(kFusedThreads), 0, p->stream,
+                         mg[l - 1].t, mg[l - 1].g, mg[l].A, mg[l].wd, mg[l].g, mg[l].brick,
+                         mg[l].rhs, mg[l].t);
+    hipLaunchKernelGGL(k_mg_rdense, dim3((p->mg_nc + 15) / 16), dim3(kBlock), 0, p->stream,
+                       mg[nl - 2].t, mg[nl - 2].g, mg[nl - 1].g, p->mg_ainv, p->mg_nc,
+                       mg[nl - 1].x);
+    res = mg[nl - 1].x;
+    for (int l = nl - 2; l >= f; --l) {
+      hipLaunchKernelGGL(k_mg_up, bgrid(mg[l].brick), dim3(kFusedThreads), 0, p->stream, res,
+                         mg[l + 1].g, mg[l].A, mg[l].wd, mg[l].rhs, mg[l].g, mg[l].brick,
+                         mg[l].t);
+      res = mg[l].t;
+    }
+  } else {
+    hipLaunchKernelGGL(k_mg_dense, dim3((p->mg_nc + 8 * kWaves - 1) / (8 * kWaves)), dim3(kBlock),
+                       0, p->stream, p->mg_ainv, p->mg_nc, mg[nl - 1].rhs, mg[nl - 1].x);
+    res = mg[nl - 1].x;
+  }
+  for (int l = f - 1; l >= 1; --l) {
     hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
                        res, mg[l + 1].g, mg[l].x, mg[l].g, nullptr);
     hipLaunchKernelGGL(k_mg_post, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
@@ -1403,6 +1445,7 @@ void amt_config_default(amt_config* c) {
   c->verbose = 0;
   c->use_graphs = 1;
   c->operator_tile = -1;
+  c->mg_fused = 1;
 }
 
 int amt_element_stiffness(double E, double nu, double h, double* Ke) {
@@ -1432,6 +1475,8 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
   if (cfg->precision != AMT_PREC_FP64 && cfg->precision != AMT_PREC_MIXED_FP32 &&
       cfg->precision != AMT_PREC_MIXED_BF16)
     return fail(nullptr, AMT_EINVAL, "unknown precision");
+  if (cfg->mg_fused != 0 && cfg->mg_fused != 1)
+    return fail(nullptr, AMT_EINVAL, "mg_fused must be 0 or 1");
   if (cfg->operator_tile < -1 || cfg->operator_tile >= kOpCandidates)
     return fail(nullptr, AMT_EINVAL, "operator_tile must be -1 (autotune) or a candidate 0.." +
                                          std::to_string(kOpCandidates - 1));
@@ -1764,6 +1809,21 @@ int amt_solve(amt_problem* p, double* u, amt_solve_stats* stats) {
   if (stats) *stats = st;
   if (u) return copy_out(p, u, p->u, sizeof(double) * p->n_dof);
   return AMT_OK;
+}
+
+int amt_apply_multigrid(amt_problem* p, const double* r, double* z) {
+  if (!p || !r || !z) return AMT_EINVAL;
+  if (p->cfg.precond != AMT_PRECOND_MULTIGRID || p->mg.empty())
+    return fail(p, AMT_EINVAL, "not a multigrid handle");
+  if (!p->loads_set) return fail(p, AMT_EINVAL, "loads not set (amt_set_loads)");
+  if (!p->coeffs_ready) AMT_CHECK(prepare_coeffs(p));
+  AMT_CHECK(tune_ops(p));
+  // the level-0 sweeps return at once once a solve is done: clear the flag first
+  HIP_TRY(p, hipMemsetAsync((char*)p->fcg + offsetof(FcgState, done), 0, sizeof(int), p->stream));
+  AMT_CHECK(copy_in(p, p->r, r, sizeof(double) * p->n_dof));
+  mg_vcycle(p, p->r, p->z, nullptr);
+  AMT_CHECK(check_launch(p, "multigrid V-cycle"));
+  return copy_out(p, z, p->z, sizeof(double) * p->n_dof);
 }
 
 int amt_sensitivities(amt_problem* p, const double* u, double* dc) {

@@ -373,12 +373,12 @@ __device__ __forceinline__ void mg_gather(const MgGrid& g, int64_t n, int i, int
       }
 }
 
-// Row r of the stencil product at node n = (i, j, k): sum_d sum_c A(n, n+d)[r][c] nb[d][c],
-// nb zero outside the grid (mg_gather). A backward block whose neighbour is outside the grid
-// is read at n instead (finite, times nb = 0): every load is unconditional.
-__device__ __forceinline__ float stencil_row(const float* __restrict__ A, const MgGrid& g,
-                                             int64_t n, int i, int j, int k, int r,
-                                             const float (&nb)[27][3]) {
+// Row r of the stencil product at node n = (i, j, k): sum_d sum_c A(n, n+d)[r][c] nb(d, c),
+// nb zero outside the grid. A backward block whose neighbour is outside the grid is read at n
+// instead (finite, times nb = 0): every load is unconditional.
+template <typename NB>
+__device__ __forceinline__ float stencil_row_f(const float* __restrict__ A, const MgGrid& g,
+                                               int64_t n, int i, int j, int k, int r, NB nb) {
   const int64_t sj = g.nnz, si = (int64_t)g.nny * g.nnz, nn = g.n_nodes;
   float y = 0.f;
 #pragma unroll
@@ -386,16 +386,22 @@ __device__ __forceinline__ float stencil_row(const float* __restrict__ A, const 
     const int di = d / 9 - 1, dj = (d / 3) % 3 - 1, dk = d % 3 - 1;
     if (d >= 13) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) y = fmaf(A[(int64_t)((d - 13) * 9 + 3 * r + c) * nn + n], nb[d][c], y);
+      for (int c = 0; c < 3; ++c) y = fmaf(A[(int64_t)((d - 13) * 9 + 3 * r + c) * nn + n], nb(d, c), y);
     } else {
       const bool ok = (i + di) >= 0 && (i + di) < g.nnx && (j + dj) >= 0 &&
                       (j + dj) < g.nny && (k + dk) >= 0 && (k + dk) < g.nnz;
       const int64_t m = ok ? n + di * si + dj * sj + dk : n;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) y = fmaf(A[(int64_t)((13 - d) * 9 + 3 * c + r) * nn + m], nb[d][c], y);
+      for (int c = 0; c < 3; ++c) y = fmaf(A[(int64_t)((13 - d) * 9 + 3 * c + r) * nn + m], nb(d, c), y);
     }
   }
   return y;
+}
+// The same with the neighbour values gathered in registers (mg_gather).
+__device__ __forceinline__ float stencil_row(const float* __restrict__ A, const MgGrid& g,
+                                             int64_t n, int i, int j, int k, int r,
+                                             const float (&nb)[27][3]) {
+  return stencil_row_f(A, g, n, i, j, k, r, [&](int d, int c) { return nb[d][c]; });
 }
 
 // Stencil level, row (node n, component r): t = rhs - A x0 with the pre-smoothing step from
@@ -428,6 +434,56 @@ __device__ __forceinline__ void mg_post_row(const float* __restrict__ A, const M
   out[q] = x[q] + wd[q] * (rhs[q] - y);
 }
 
+// Restriction sum (P^T r_f)(N) at coarse node N = (I, J, K), r_f read through f(fine DOF):
+// the 27 fine nodes 2N + a in the order (x, y, z), weight 1 on the node itself and 1/2 per axis
+// beside it. A fine node outside the grid is read at the clamped position with weight 0, so
+// the 81 loads carry no branch and issue together.
+template <typename F>
+__device__ __forceinline__ void mg_restrict_sum(F f, const MgGrid& fg, int I, int J, int K,
+                                                float (&acc)[3]) {
+  acc[0] = acc[1] = acc[2] = 0.f;
+#pragma unroll
+  for (int ax = -1; ax <= 1; ++ax)
+#pragma unroll
+    for (int ay = -1; ay <= 1; ++ay)
+#pragma unroll
+      for (int az = -1; az <= 1; ++az) {
+        const int mx = 2 * I + ax, my = 2 * J + ay, mz = 2 * K + az;
+        const bool ok = mx >= 0 && mx < fg.nnx && my >= 0 && my < fg.nny && mz >= 0 && mz < fg.nnz;
+        const float w = ok ? (ax ? 0.5f : 1.f) * (ay ? 0.5f : 1.f) * (az ? 0.5f : 1.f) : 0.f;
+        const int64_t m = mg_node(fg, min(max(mx, 0), fg.nnx - 1), min(max(my, 0), fg.nny - 1),
+                                  min(max(mz, 0), fg.nnz - 1));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] = fmaf(w, f(3 * m + c), acc[c]);
+      }
+}
+
+// Prolongation sum (P x_c)(n) at fine node n = (i, j, k), x_c read through f(coarse DOF): the
+// (up to) eight trilinear parents in the order (x, y, z); an even coordinate has one parent,
+// and its second parent repeats the first with weight 0.
+template <typename F>
+__device__ __forceinline__ void mg_prolong_sum(F f, const MgGrid& cg, int i, int j, int k,
+                                               float (&acc)[3]) {
+  const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+  const int i1 = (i & 1) ? i0 + 1 : i0, j1 = (j & 1) ? j0 + 1 : j0, k1 = (k & 1) ? k0 + 1 : k0;
+  const float wi[2] = {(i & 1) ? 0.5f : 1.f, (i & 1) ? 0.5f : 0.f};
+  const float wj[2] = {(j & 1) ? 0.5f : 1.f, (j & 1) ? 0.5f : 0.f};
+  const float wk[2] = {(k & 1) ? 0.5f : 1.f, (k & 1) ? 0.5f : 0.f};
+  const int ci[2] = {i0, i1}, cj[2] = {j0, j1}, ck[2] = {k0, k1};
+  acc[0] = acc[1] = acc[2] = 0.f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int64_t N = mg_node(cg, ci[a], cj[b], ck[c]);
+        const float w = wi[a] * wj[b] * wk[c];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[q] = fmaf(w, f(3 * N + q), acc[q]);
+      }
+}
+
 // Restriction r_c = P^T r_f at coarse node N; with wdc, also the coarse level's
 // pre-smoothing step from zero, x0c = wdc * r_c.
 __device__ __forceinline__ void mg_restrict_node(const float* __restrict__ rf, const MgGrid& fg,
@@ -436,23 +492,8 @@ __device__ __forceinline__ void mg_restrict_node(const float* __restrict__ rf, c
                                                  float* __restrict__ x0c) {
   int I, J, K;
   mg_decode(N, cg, I, J, K);
-  float acc[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-  for (int ax = -1; ax <= 1; ++ax)
-#pragma unroll
-    for (int ay = -1; ay <= 1; ++ay)
-#pragma unroll
-      for (int az = -1; az <= 1; ++az) {
-        // a fine node outside the grid is read at the clamped position with weight 0, so the
-        // 81 loads carry no branch and issue together
-        const int mx = 2 * I + ax, my = 2 * J + ay, mz = 2 * K + az;
-        const bool ok = mx >= 0 && mx < fg.nnx && my >= 0 && my < fg.nny && mz >= 0 && mz < fg.nnz;
-        const float w = ok ? (ax ? 0.5f : 1.f) * (ay ? 0.5f : 1.f) * (az ? 0.5f : 1.f) : 0.f;
-        const int64_t m = mg_node(fg, min(max(mx, 0), fg.nnx - 1), min(max(my, 0), fg.nny - 1),
-                                  min(max(mz, 0), fg.nnz - 1));
-#pragma unroll
-        for (int c = 0; c < 3; ++c) acc[c] = fmaf(w, rf[3 * m + c], acc[c]);
-      }
+  float acc[3];
+  mg_restrict_sum([&](int64_t q) { return rf[q]; }, fg, I, J, K, acc);
 #pragma unroll
   for (int c = 0; c < 3; ++c) rc[3 * N + c] = acc[c];
   if (wdc) {
@@ -468,25 +509,8 @@ __device__ __forceinline__ void mg_prolong_node(const float* __restrict__ xc, co
                                                 int64_t n, const uint8_t* __restrict__ fixcnt) {
   int i, j, k;
   mg_decode(n, fg, i, j, k);
-  const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
-  const int i1 = (i & 1) ? i0 + 1 : i0, j1 = (j & 1) ? j0 + 1 : j0, k1 = (k & 1) ? k0 + 1 : k0;
-  // an even coordinate has one parent: its second parent repeats the first with weight 0
-  const float wi[2] = {(i & 1) ? 0.5f : 1.f, (i & 1) ? 0.5f : 0.f};
-  const float wj[2] = {(j & 1) ? 0.5f : 1.f, (j & 1) ? 0.5f : 0.f};
-  const float wk[2] = {(k & 1) ? 0.5f : 1.f, (k & 1) ? 0.5f : 0.f};
-  float acc[3] = {0.f, 0.f, 0.f};
-  const int ci[2] = {i0, i1}, cj[2] = {j0, j1}, ck[2] = {k0, k1};
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const int64_t N = mg_node(cg, ci[a], cj[b], ck[c]);
-        const float w = wi[a] * wj[b] * wk[c];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) acc[q] = fmaf(w, xc[3 * N + q], acc[q]);
-      }
+  float acc[3];
+  mg_prolong_sum([&](int64_t q) { return xc[q]; }, cg, i, j, k, acc);
 #pragma unroll
   for (int q = 0; q < 3; ++q)
     if (!fixcnt || !fixcnt[3 * n + q]) xf[3 * n + q] += acc[q];
@@ -587,6 +611,372 @@ __global__ __launch_bounds__(kBlock) void k_mg_dense(const float* __restrict__ A
   const int row0 = blockIdx.x * 8 * kWaves + (threadIdx.x >> 6);
   if (row0 >= n) return;
   mg_dense_rows8(Ainv, n, r, x, row0, kWaves);
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused coarse-level V-cycle. On the small stencil levels a launch costs its dispatch (~1.7 us
+// in a graph) plus its dependent memory round trips, more than its arithmetic: a one-row-per-
+// thread stencil sweep issues 162 loads per thread, which leave in three batches of at most 63
+// outstanding loads. From level f on, each level's work is therefore two launches over bricks
+// of at most 4x4x4 nodes, and the dense level one more:
+//   k_mg_down   (level l): rhs_l = P^T t_{l-1} on the brick and its one-node shell, x0 = wd rhs_l
+//                (the pre-smoothing step from zero), t_l = rhs_l - A_l x0 on the brick;
+//   k_mg_rdense (dense level L-1): x_{L-1} = Ainv P^T t_{L-2};
+//   k_mg_up     (level l): x = wd rhs_l + P c_{l+1} on the brick and its shell (c: the coarse
+//                correction), post-smoothing t_l = x + wd (rhs_l - A_l x) on the brick.
+// A block has 576 threads, one per (row component r, plane di, brick node): the thread's 27
+// stencil coefficients (the 3x3 neighbour blocks at offset di) are loaded first and arrive
+// together with the brick's input vectors, so a launch waits for one memory round trip. The
+// restriction is separable and runs in LDS on the fine region behind the shell, whose values
+// are each loaded once. The three plane partials of a row are added in LDS in fixed order.
+// The arithmetic is the same linear map as the one-step kernels above, summed in another order.
+// ---------------------------------------------------------------------------------------
+constexpr int kFB = 4;                   // brick extent per axis (coarse-level nodes)
+constexpr int kFBN = kFB * kFB * kFB;    // 64 brick nodes
+constexpr int kFS = kFB + 2;             // brick plus shell, per axis
+constexpr int kFSN = kFS * kFS * kFS;    // 216
+constexpr int kFF = 2 * kFB + 5;         // fine region behind the shell, per axis
+constexpr int kFusedThreads = 9 * kFBN;  // 576
+constexpr int kFuseMaxNodes = 16384;     // levels fused (at most this many nodes)
+
+// Bricks per axis; the bricks tile the level from node 0, the last ones reach past the grid.
+struct MgBrick {
+  int nbi, nbj, nbk;
+};
+
+// Brick of this block, dealt XCD-contiguously (hardware block b runs on XCD b % 8; XCD x takes
+// a contiguous run of bricks, so neighbouring bricks' shells are read through one L2).
+// Grid: 8 * ceil(n_bricks / 8) blocks.
+__device__ __forceinline__ bool mg_brick(const MgBrick& b, int& i0, int& j0, int& k0) {
+  const int nb = b.nbi * b.nbj * b.nbk;
+  const int per = (nb + 7) / 8;
+  const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  if (logical >= nb) return false;
+  k0 = (logical % b.nbk) * kFB;
+  const int t = logical / b.nbk;
+  j0 = (t % b.nbj) * kFB;
+  i0 = (t / b.nbj) * kFB;
+  return true;
+}
+
+// This thread's plane task: row component r and neighbour plane di (both uniform per wave)
+// of brick node (li, lj, lk); valid when the node lies inside the grid.
+struct MgTask {
+  int r, di, li, lj, lk;
+  bool valid;
+};
+__device__ __forceinline__ MgTask mg_task(const MgGrid& g, int i0, int j0, int k0) {
+  MgTask t;
+  const int rd = (int)threadIdx.x / kFBN, ln = (int)threadIdx.x % kFBN;
+  t.r = rd / 3;
+  t.di = rd % 3 - 1;
+  t.lk = ln % kFB;
+  t.lj = (ln / kFB) % kFB;
+  t.li = ln / (kFB * kFB);
+  t.valid = i0 + t.li < g.nnx && j0 + t.lj < g.nny && k0 + t.lk < g.nnz;
+  return t;
+}
+
+// Coefficients A(n, n + (DI, dj, dk))[r][c] for the nine (dj, dk), c fastest; a backward block
+// whose neighbour is outside the grid is read at n (finite; its x is zero). 32-bit offsets: the
+// fused levels hold at most kFuseMaxNodes nodes.
+template <int DI>
+__device__ __forceinline__ void mg_plane_coefs(const float* __restrict__ A, const MgGrid& g,
+                                               int n, int i, int j, int k, int r,
+                                               float (&co)[27]) {
+  const int sj = g.nnz, si = g.nny * g.nnz, nn = (int)g.n_nodes;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    const int dj = e / 3 - 1, dk = e % 3 - 1;
+    const int d = (DI + 1) * 9 + (dj + 1) * 3 + (dk + 1);
+    if (d >= 13) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) co[3 * e + c] = A[(unsigned)(((d - 13) * 9 + 3 * r + c) * nn + n)];
+    } else {
+      const bool ok = (i + DI) >= 0 && (i + DI) < g.nnx && (j + dj) >= 0 && (j + dj) < g.nny &&
+                      (k + dk) >= 0 && (k + dk) < g.nnz;
+      const int m = ok ? n + DI * si + dj * sj + dk : n;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) co[3 * e + c] = A[(unsigned)(((13 - d) * 9 + 3 * c + r) * nn + m)];
+    }
+  }
+}
+__device__ __forceinline__ void mg_task_coefs(const float* __restrict__ A, const MgGrid& g,
+                                              const MgTask& t, int i0, int j0, int k0,
+                                              float (&co)[27]) {
+  // an invalid task loads node 0's coefficients (in bounds, never used)
+  const int i = t.valid ? i0 + t.li : 0, j = t.valid ? j0 + t.lj : 0, k = t.valid ? k0 + t.lk : 0;
+  const int n = (i * g.nny + j) * g.nnz + k;
+  if (t.di < 0)
+    mg_plane_coefs<-1>(A, g, n, i, j, k, t.r, co);
+  else if (t.di == 0)
+    mg_plane_coefs<0>(A, g, n, i, j, k, t.r, co);
+  else
+    mg_plane_coefs<1>(A, g, n, i, j, k, t.r, co);
+}
+
+// The three components of node m (32-bit node index) of a node-major vector.
+struct F3 {
+  float v[3];
+};
+__device__ __forceinline__ F3 mg_load3(const float* __restrict__ x, int m) {
+  const float* p = x + (unsigned)(3 * m);
+  F3 r;
+  r.v[0] = p[0];
+  r.v[1] = p[1];
+  r.v[2] = p[2];
+  return r;
+}
+
+// Plane partial of the task's row: sum over (dj, dk, c) of co * x, with x read from the shell
+// array xs (node-major, 3 floats a node, zeros outside the grid).
+__device__ __forceinline__ float mg_task_partial(const float (&co)[27], const float* xs,
+                                                 const MgTask& t) {
+  const int q0 = ((t.li + 1 + t.di) * kFS + (t.lj + 1)) * kFS + (t.lk + 1);
+  float p = 0.f;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    const int q = q0 + (e / 3 - 1) * kFS + (e % 3 - 1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) p = fmaf(co[3 * e + c], xs[3 * q + c], p);
+  }
+  return p;
+}
+
+// Shell position of brick node (li, lj, lk).
+__device__ __forceinline__ int mg_shell_q(int li, int lj, int lk) {
+  return ((li + 1) * kFS + (lj + 1)) * kFS + (lk + 1);
+}
+
+// The brick's rows from the plane partials P (index (3r + di + 1) * 64 + brick node): threads
+// below 192 take row (brick node, r = threadIdx / 64) and call f(n, r, shell position, y).
+template <typename Fn>
+__device__ __forceinline__ void mg_brick_rows(const float* P, const MgGrid& g, int i0, int j0,
+                                              int k0, Fn f) {
+  if (threadIdx.x >= 3 * kFBN) return;
+  const int r = (int)threadIdx.x / kFBN, ln = (int)threadIdx.x % kFBN;
+  const int lk = ln % kFB, lj = (ln / kFB) % kFB, li = ln / (kFB * kFB);
+  if (i0 + li >= g.nnx || j0 + lj >= g.nny || k0 + lk >= g.nnz) return;
+  const float y = (P[(3 * r) * kFBN + ln] + P[(3 * r + 1) * kFBN + ln]) + P[(3 * r + 2) * kFBN + ln];
+  f(((i0 + li) * g.nny + j0 + lj) * g.nnz + k0 + lk, r, mg_shell_q(li, lj, lk), y);
+}
+
+__global__ __launch_bounds__(kFusedThreads) void k_mg_down(
+    const float* __restrict__ tf, MgGrid fg, const float* __restrict__ A,
+    const float* __restrict__ wd, MgGrid g, MgBrick b, float* __restrict__ rhs,
+    float* __restrict__ t_out) {
+  // fine region F (x, y, z), z-restricted G (x, y, kz), y-restricted H (x, jy, kz) in F's
+  // space, shell rhs R and x0 X, plane partials P; 3 floats a node
+  __shared__ float F[3 * kFF * kFF * kFF], G[3 * kFF * kFF * kFS];
+  __shared__ float R[3 * kFSN], X[3 * kFSN], P[9 * kFBN];
+  int i0, j0, k0;
+  if (!mg_brick(b, i0, j0, k0)) return;  // whole block: no barrier follows for it
+  const MgTask tk = mg_task(g, i0, j0, k0);
+  float co[27];
+  mg_task_coefs(A, g, tk, i0, j0, k0, co);
+  // the fine region (zeros outside the fine grid; one node per thread and step) and the shell's
+  // smoother weights (zeros outside the grid), issued behind the coefficients
+  const int fx0 = 2 * i0 - 3, fy0 = 2 * j0 - 3, fz0 = 2 * k0 - 3;
+  constexpr int kNF = kFF * kFF * kFF;
+  constexpr int kFLoads = (kNF + kFusedThreads - 1) / kFusedThreads;
+  F3 fv[kFLoads];
+#pragma unroll
+  for (int u = 0; u < kFLoads; ++u) {
+    const int q = threadIdx.x + u * kFusedThreads;
+    const int x = fx0 + q / (kFF * kFF), y = fy0 + (q / kFF) % kFF, z = fz0 + q % kFF;
+    fv[u].v[0] = fv[u].v[1] = fv[u].v[2] = 0.f;
+    if (q < kNF && x >= 0 && x < fg.nnx && y >= 0 && y < fg.nny && z >= 0 && z < fg.nnz)
+      fv[u] = mg_load3(tf, (x * fg.nny + y) * fg.nnz + z);
+  }
+  F3 wv;
+  wv.v[0] = wv.v[1] = wv.v[2] = 0.f;
+  {
+    const int q = threadIdx.x;
+    const int I = i0 - 1 + q / (kFS * kFS), J = j0 - 1 + (q / kFS) % kFS, K = k0 - 1 + q % kFS;
+    if (q < kFSN && I >= 0 && I < g.nnx && J >= 0 && J < g.nny && K >= 0 && K < g.nnz)
+      wv = mg_load3(wd, (I * g.nny + J) * g.nnz + K);
+  }
+#pragma unroll
+  for (int u = 0; u < kFLoads; ++u) {
+    const int q = threadIdx.x + u * kFusedThreads;
+    if (q < kNF) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) F[3 * q + c] = fv[u].v[c];
+    }
+  }
+  __syncthreads();
+  // separable restriction (weights 1/2, 1, 1/2): z, then y, then x; a pass task is (outer
+  // index, o, component), the steps of a thread unrolled so their LDS reads overlap
+  constexpr int kZ = 3 * kFF * kFF * kFS, kY = 3 * kFF * kFS * kFS;
+#pragma unroll
+  for (int u = 0; u < (kZ + kFusedThreads - 1) / kFusedThreads; ++u) {
+    const int q = threadIdx.x + u * kFusedThreads;
+    if (q < kZ) {
+      const int c = q % 3, v = q / 3, o = v % kFS, xy = v / kFS;
+      const int ib = 3 * (xy * kFF + 2 * o) + c;
+      G[q] = fmaf(0.5f, F[ib + 6], 0.5f * F[ib] + F[ib + 3]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < (kY + kFusedThreads - 1) / kFusedThreads; ++u) {
+    const int q = threadIdx.x + u * kFusedThreads;
+    if (q < kY) {
+      const int c = q % 3, v = q / 3, kz = v % kFS, o = (v / kFS) % kFS, x = v / (kFS * kFS);
+      const int ib = 3 * ((x * kFF + 2 * o) * kFS + kz) + c;
+      F[q] = fmaf(0.5f, G[ib + 6 * kFS], 0.5f * G[ib] + G[ib + 3 * kFS]);
+    }
+  }
+  __syncthreads();
+  // x: one shell node per thread, then x0 = wd rhs there; the brick's rhs is stored
+  if (threadIdx.x < kFSN) {
+    const int q = threadIdx.x;
+    const int li = q / (kFS * kFS), lj = (q / kFS) % kFS, lk = q % kFS;
+    const int I = i0 - 1 + li, J = j0 - 1 + lj, K = k0 - 1 + lk;
+    const bool own = li >= 1 && li <= kFB && lj >= 1 && lj <= kFB && lk >= 1 && lk <= kFB &&
+                     I < g.nnx && J < g.nny && K < g.nnz;
+    float* dst = rhs + (unsigned)(3 * ((I * g.nny + J) * g.nnz + K));
+    const int ib = 3 * (2 * li * kFS * kFS + lj * kFS + lk);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float rv =
+          fmaf(0.5f, F[ib + 6 * kFS * kFS + c], 0.5f * F[ib + c] + F[ib + 3 * kFS * kFS + c]);
+      R[3 * q + c] = rv;
+      X[3 * q + c] = wv.v[c] * rv;
+      if (own) dst[c] = rv;
+    }
+  }
+  __syncthreads();
+  P[threadIdx.x] = tk.valid ? mg_task_partial(co, X, tk) : 0.f;
+  __syncthreads();
+  mg_brick_rows(P, g, i0, j0, k0, [&](int n, int r, int q, float y) {
+    t_out[(unsigned)(3 * n + r)] = R[3 * q + r] - y;
+  });
+}
+
+__global__ __launch_bounds__(kFusedThreads) void k_mg_up(
+    const float* __restrict__ xc, MgGrid cg, const float* __restrict__ A,
+    const float* __restrict__ wd, const float* __restrict__ rhs, MgGrid g, MgBrick b,
+    float* __restrict__ out) {
+  // coarse correction region C (the parents of the shell), shell rhs, wd and x, partials P
+  constexpr int kC = kFB / 2 + 3;
+  __shared__ float C[3 * kC * kC * kC], RS[3 * kFSN], WS[3 * kFSN], X[3 * kFSN], P[9 * kFBN];
+  int i0, j0, k0;
+  if (!mg_brick(b, i0, j0, k0)) return;  // whole block: no barrier follows for it
+  // coarse parents of the in-grid shell nodes lie in [max(i0 - 1, 0) / 2, that + kC)
+  const int ci0 = max(i0 - 1, 0) >> 1, cj0 = max(j0 - 1, 0) >> 1, ck0 = max(k0 - 1, 0) >> 1;
+  const MgTask tk = mg_task(g, i0, j0, k0);
+  float co[27];
+  mg_task_coefs(A, g, tk, i0, j0, k0, co);
+  F3 cv, rv, wv;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) cv.v[c] = rv.v[c] = wv.v[c] = 0.f;
+  if (threadIdx.x < kC * kC * kC) {
+    const int q = threadIdx.x;
+    const int x = ci0 + q / (kC * kC), y = cj0 + (q / kC) % kC, z = ck0 + q % kC;
+    if (x < cg.nnx && y < cg.nny && z < cg.nnz) cv = mg_load3(xc, (x * cg.nny + y) * cg.nnz + z);
+  }
+  const int q = threadIdx.x;
+  const int I = i0 - 1 + q / (kFS * kFS), J = j0 - 1 + (q / kFS) % kFS, K = k0 - 1 + q % kFS;
+  const bool in = q < kFSN && I >= 0 && I < g.nnx && J >= 0 && J < g.nny && K >= 0 && K < g.nnz;
+  if (in) {
+    const int m = (I * g.nny + J) * g.nnz + K;
+    rv = mg_load3(rhs, m);
+    wv = mg_load3(wd, m);
+  }
+  if (threadIdx.x < kC * kC * kC) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) C[3 * threadIdx.x + c] = cv.v[c];
+  }
+  __syncthreads();
+  // x = wd rhs + P c on the shell (zeros outside the grid)
+  if (q < kFSN) {
+    float acc[3] = {0.f, 0.f, 0.f};
+    if (in) {
+      const int pa[2] = {(I >> 1) - ci0, ((I + 1) >> 1) - ci0};
+      const int pb[2] = {(J >> 1) - cj0, ((J + 1) >> 1) - cj0};
+      const int pc[2] = {(K >> 1) - ck0, ((K + 1) >> 1) - ck0};
+      const float fa[2] = {(I & 1) ? 0.5f : 1.f, (I & 1) ? 0.5f : 0.f};
+      const float fb[2] = {(J & 1) ? 0.5f : 1.f, (J & 1) ? 0.5f : 0.f};
+      const float fc[2] = {(K & 1) ? 0.5f : 1.f, (K & 1) ? 0.5f : 0.f};
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+          for (int w = 0; w < 2; ++w) {
+            const float wt = fa[s] * fb[t] * fc[w];
+            const int e = 3 * ((pa[s] * kC + pb[t]) * kC + pc[w]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[c] = fmaf(wt, C[e + c], acc[c]);
+          }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      X[3 * q + c] = in ? wv.v[c] * rv.v[c] + acc[c] : 0.f;
+      RS[3 * q + c] = rv.v[c];
+      WS[3 * q + c] = wv.v[c];
+    }
+  }
+  __syncthreads();
+  P[threadIdx.x] = tk.valid ? mg_task_partial(co, X, tk) : 0.f;
+  __syncthreads();
+  mg_brick_rows(P, g, i0, j0, k0, [&](int n, int r, int q, float y) {
+    const int e = 3 * q + r;
+    out[(unsigned)(3 * n + r)] = X[e] + WS[e] * (RS[e] - y);
+  });
+}
+
+// Dense level: x = Ainv P^T t. Block b takes rows [16b, 16b + 16): its threads below nc form
+// the whole right-hand side (27 fine values each), each of its four waves four rows (lanes over
+// columns lane + 64v, then the butterfly sum). Grid: ceil(nc / 16) blocks of 256 threads.
+constexpr int kDenseMax = 192;
+__global__ __launch_bounds__(kBlock) void k_mg_rdense(const float* __restrict__ tf, MgGrid fg,
+                                                      MgGrid cg, const float* __restrict__ Ainv,
+                                                      int nc, float* __restrict__ x) {
+  __shared__ float rc[kDenseMax];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int row0 = blockIdx.x * 16 + 4 * w;
+  float av[4][3];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      const int row = row0 + u, col = lane + 64 * v;
+      av[u][v] = (row < nc && col < nc) ? Ainv[row * nc + col] : 0.f;
+    }
+  if ((int)threadIdx.x < nc) {
+    const int c = threadIdx.x % 3, N = threadIdx.x / 3;
+    const int K = N % cg.nnz, J = (N / cg.nnz) % cg.nny, I = N / (cg.nnz * cg.nny);
+    float acc = 0.f;
+#pragma unroll
+    for (int ax = -1; ax <= 1; ++ax)
+#pragma unroll
+      for (int ay = -1; ay <= 1; ++ay)
+#pragma unroll
+        for (int az = -1; az <= 1; ++az) {
+          const int mx = 2 * I + ax, my = 2 * J + ay, mz = 2 * K + az;
+          const bool ok = mx >= 0 && mx < fg.nnx && my >= 0 && my < fg.nny && mz >= 0 && mz < fg.nnz;
+          const float wt = ok ? (ax ? 0.5f : 1.f) * (ay ? 0.5f : 1.f) * (az ? 0.5f : 1.f) : 0.f;
+          const int m = (min(max(mx, 0), fg.nnx - 1) * fg.nny + min(max(my, 0), fg.nny - 1)) * fg.nnz +
+                        min(max(mz, 0), fg.nnz - 1);
+          acc = fmaf(wt, tf[3 * m + c], acc);
+        }
+    rc[threadIdx.x] = acc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    float acc = 0.f;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      const int col = lane + 64 * v;
+      if (col < nc) acc = fmaf(av[u][v], rc[col], acc);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0 && row0 + u < nc) x[row0 + u] = acc;
+  }
 }
 
 // Dense inverse of the coarsest Galerkin operator, on the device in one 1024-thread block

@@ -67,6 +67,7 @@ class SIMPConfig:
     inner_max_iterations: int = 20000
     use_graphs: bool = True
     operator_tile: int = -1           # level-0 operator launch: -1 autotune, else a candidate
+    mg_fused: bool = True             # fused V-cycle kernels on the small levels
 
 
 @dataclass
@@ -141,6 +142,7 @@ def make_amt_config(nx, ny, nz, element_size, material, config):
     c.verbose = 0
     c.use_graphs = int(bool(config.use_graphs))
     c.operator_tile = int(config.operator_tile)
+    c.mg_fused = int(bool(config.mg_fused))
     return c
 
 
@@ -292,6 +294,13 @@ class TopologyOptimizer:
                        outer_iterations=st.outer_iterations, fallback=bool(st.fallback),
                        rel_residual=st.rel_residual,
                        true_rel_residual=st.true_rel_residual, solve_time_s=st.solve_time_s)
+
+    def apply_multigrid(self, r):
+        """z = M r, one multigrid V-cycle (preconditioner="multigrid" only)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros(self.n_dofs)
+        self._check(self._L.amt_apply_multigrid(self._h, r, z))
+        return z
 
     def sensitivities(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)

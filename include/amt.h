@@ -74,6 +74,11 @@ typedef struct amt_config {
                             element-column streaming tile of shape s (0: 16x16, 1: 8x32, 2: 4x64
                             threads along j x k) with 4, 8 or 16 (l = 0, 1, 2) node planes per
                             block. All streaming shapes give the same operator values. */
+  int mg_fused;          /* multigrid V-cycle on the levels of at most 16,384 nodes: 1 (default)
+                            two brick-tiled launches per level (restriction + pre-smoothing,
+                            prolongation + post-smoothing) and one for the dense level; 0 one
+                            launch per smoothing or transfer step. Same linear map, summed in
+                            another order. */
 } amt_config;
 
 /* IterationResult (topology_optimizer.hpp:65-74) plus solver detail. */
@@ -147,6 +152,12 @@ typedef struct amt_solve_stats {
 /* solveWithBCs (linear_solver.cpp:186-203) for the current density and loads; u (n_dof)
  * receives the displacement. */
 int amt_solve(amt_problem* p, double* u, amt_solve_stats* stats);
+
+/* One application z = M r of the multigrid preconditioner (handles with precond
+ * AMT_PRECOND_MULTIGRID) for the current density and loads; r and z have n_dof entries. No
+ * counterpart in the reference (its IncompleteCholesky preconditioner is internal to Eigen's
+ * ConjugateGradient, linear_solver.cpp:133-140); it exposes the V-cycle for tests. */
+int amt_apply_multigrid(amt_problem* p, const double* r, double* z);
 
 /* computeSensitivities (topology_optimizer.cpp:126-161) for u (n_dof) and the current
  * density; dc has n_e entries (slot order). */

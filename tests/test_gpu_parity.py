@@ -254,9 +254,11 @@ def test_multigrid_solve_operator_tiles():
         assert st["converged"] and st["true_rel_residual"] <= 1.5e-8, (tile, st)
         out[tile] = (u, st["iterations"], info)
         opt.close()
+    # the solves stop at a 1e-8 relative residual: c - c* = -u*.r, which is bounded by
+    # |u*| |r| and not by 1e-8 c (the loads sit on a few nodes, u* is spread over all)
     c0 = out[0][0] @ p.F
     for tile, (u, its, info) in out.items():
-        assert (u @ p.F) == pytest.approx(c0, rel=1e-9)
+        assert (u @ p.F) == pytest.approx(c0, rel=1e-8)
     its = [v[1] for v in out.values()]
     assert max(its) - min(its) <= 2, its
     assert all(i["us"] > 0 for i in out[-1][2])  # the autotuner ran and timed its picks
@@ -435,13 +437,16 @@ def test_multigrid_run_both_update_paths(shape):
                for r in mg.iteration_results)
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("shape", [(30, 10, 10), (13, 7, 5), (2, 1, 1)])
-def test_multigrid_run_matches_oracle(shape):
-    """OC loop with the multigrid solver (odd sizes exercise the padded coarsening): compliance
-    within 1e-6 relative of the oracle's Jacobi-PCG path, density within 1e-4."""
+def test_multigrid_run_matches_oracle(shape, fused):
+    """OC loop with the multigrid solver (odd sizes exercise the padded coarsening), with the
+    fused small-level V-cycle kernels (from level 1 here; the 2x1x1 hierarchy has no stencil
+    level and runs the one-step kernels either way) and without: compliance within 1e-6
+    relative of the oracle's Jacobi-PCG path, density within 1e-4."""
     p = A.cantilever_problem(*shape)
     its = 20
-    cfg = make_config(p, preconditioner="multigrid", max_iterations=its)
+    cfg = make_config(p, preconditioner="multigrid", max_iterations=its, mg_fused=fused)
     opt = A.TopologyOptimizer(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3), cfg)
     res = opt.run(p.rho0, p.F, p.fixed)
     ocfg = O.make_config(p.nx, p.ny, p.nz, max_iterations=its, pcg_tol=1e-8, pcg_maxit=100000,
@@ -453,6 +458,70 @@ def test_multigrid_run_matches_oracle(shape):
     assert np.abs(res.final_density - rho_ref).max() <= 1e-4
     assert all(r.solver_converged and r.true_rel_residual <= 1.5e-8
                for r in res.iteration_results)
+
+
+@pytest.mark.parametrize("shape", [(120, 60, 60), (160, 80, 80), (61, 33, 17), (13, 7, 5)])
+def test_multigrid_fused_preconditioner_matches_one_step_kernels(shape):
+    """One V-cycle z = M r with the fused small-level kernels against the one-launch-per-step
+    kernels on the same random residual: the same linear map summed in another order, so the
+    two agree to FP32 rounding (relative L2 difference <= 1e-5). The symmetry defect
+    |r.M s - s.M r| / (|r| |M s|) and the linearity defect |M(r + 2s) - Mr - 2Ms| / |M(r + 2s)|
+    of the fused form are no larger than the one-step form's (x2, or 1e-6); both come from the
+    FP32 arithmetic of the smoothers, whose weights span the 1e8 penalty and the 1e-9 void
+    stiffness."""
+    p = A.bracket_like_problem(*shape)
+    rng = np.random.default_rng(7)
+    r = rng.standard_normal(p.n_dofs)
+    sv = rng.standard_normal(p.n_dofs)
+    out, sym, lin = {}, {}, {}
+    for fused in (True, False):
+        cfg = make_config(p, preconditioner="multigrid", mg_fused=fused)
+        opt = A.TopologyOptimizer(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3), cfg)
+        opt.set_loads(p.F, p.fixed)
+        opt.set_density(p.rho0)
+        mr, ms, mrs = (opt.apply_multigrid(v) for v in (r, sv, r + 2 * sv))
+        sym[fused] = abs(r @ ms - sv @ mr) / (np.linalg.norm(r) * np.linalg.norm(ms))
+        lin[fused] = np.linalg.norm(mrs - (mr + 2 * ms)) / np.linalg.norm(mrs)
+        out[fused] = mr
+        opt.close()
+    rel = np.linalg.norm(out[True] - out[False]) / np.linalg.norm(out[False])
+    print("%s: fused vs one-step V-cycle relative difference %.2e; symmetry defect %.2e / %.2e, "
+          "linearity defect %.2e / %.2e (fused / one-step)"
+          % (shape, rel, sym[True], sym[False], lin[True], lin[False]))
+    assert rel <= 1e-5
+    assert sym[True] <= max(2 * sym[False], 1e-6)
+    assert lin[True] <= max(2 * lin[False], 1e-6)
+
+
+@pytest.mark.parametrize("shape", [(120, 60, 60), (160, 80, 80), (61, 33, 17)])
+def test_multigrid_fused_levels_match_one_step_kernels(shape):
+    """The fused small-level V-cycle (brick-tiled restriction + pre-smoothing and prolongation +
+    post-smoothing launches, dense level in one launch) against the one-launch-per-step
+    V-cycle: the same linear preconditioner summed in another order, so 3 warm-started OC
+    iterations agree to the solve tolerance (compliance within 1e-8 relative, density within
+    1e-5, FCG iteration counts within 2). The hierarchies fuse from level 2 (120x60x60),
+    level 3 (160x80x80) and level 1 (61x33x17)."""
+    p = A.bracket_like_problem(*shape)
+    out = {}
+    for fused in (True, False):
+        cfg = make_config(p, preconditioner="multigrid", warm_start=True, max_iterations=3,
+                          mg_fused=fused)
+        opt = A.TopologyOptimizer(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3), cfg)
+        res = opt.run(p.rho0, p.F, p.fixed)
+        out[fused] = res
+        assert all(r.solver_converged and r.true_rel_residual <= 1.5e-8
+                   for r in res.iteration_results), fused
+        opt.close()
+    a, b = out[True], out[False]
+    ca, cb = np.array(a.compliance_history), np.array(b.compliance_history)
+    its_a = [r.solver_iterations for r in a.iteration_results]
+    its_b = [r.solver_iterations for r in b.iteration_results]
+    print("%s: compliance rel diff %.2e, density max-abs %.2e, FCG its fused %s one-step %s" % (
+        shape, np.abs(ca / cb - 1).max(), np.abs(a.final_density - b.final_density).max(),
+        its_a, its_b))
+    assert np.abs(ca / cb - 1).max() <= 1e-8
+    assert np.abs(a.final_density - b.final_density).max() <= 1e-5
+    assert all(abs(x - y) <= 2 for x, y in zip(its_a, its_b))
 
 
 @pytest.mark.parametrize("face", [0, 1, 2, 3, 4, 5])
