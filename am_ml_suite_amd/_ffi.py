@@ -48,7 +48,7 @@ class AmtConfig(C.Structure):
         ("density_index", C.c_int), ("precision", C.c_int), ("precond", C.c_int),
         ("warm_start", C.c_int), ("inner_tol", C.c_double), ("inner_maxit", C.c_int),
         ("verbose", C.c_int), ("use_graphs", C.c_int), ("operator_tile", C.c_int),
-        ("mg_fused", C.c_int),
+        ("mg_fused", C.c_int), ("mg_coarse_cheb", C.c_int),
     ]
 
 

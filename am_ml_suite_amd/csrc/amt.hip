@@ -64,6 +64,27 @@ constexpr int kNnPatience = 16;  // learned FCG: chunks allowed without a 10% |r
 constexpr double kNnGainMomentum = 0.05;  // running means behind the correction's weight
 constexpr float kMgOmega = 0.6f;  // damped-Jacobi weight of the V-cycle smoother...
 constexpr float kMgTheta = 1.9f;  // ...capped per row at theta / sum_j |A_ij| (spectrum < 2)
+// Chebyshev smoother of the levels >= 2: polynomial in M^-1 A with M = diag(row sums of |A|),
+// whose spectrum lies in (0, 1] (Gershgorin); the polynomial damps [1 / kChebRatio, 1].
+// Step s (from 1): d' = a_s d + b_s M^-1 (rhs - A x), x' = x + d' (the three-term recurrence
+// of the Chebyshev iteration, with a_1 = 0).
+constexpr double kChebRatio = 20.0;
+struct ChebCoef {
+  double a, b;
+};
+inline ChebCoef cheb_coef(int step) {
+  const double lmax = 1.0, lmin = 1.0 / kChebRatio;
+  const double th = 0.5 * (lmax + lmin), de = 0.5 * (lmax - lmin), sigma = th / de;
+  double rho = 1.0 / sigma;
+  if (step <= 1) return {0.0, 1.0 / th};
+  ChebCoef c{0.0, 0.0};
+  for (int k = 2; k <= step; ++k) {
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    c = {rn * rho, 2.0 * rn / de};
+    rho = rn;
+  }
+  return c;
+}
 // coarsen until a level has at most this many DOFs; the coarsest level's dense inverse runs in
 // one block with up to 6 x 6 register-resident entries per thread (n <= 192)
 constexpr int64_t kMgCoarseDofs = 192;
@@ -142,6 +163,9 @@ struct amt_problem {
     MgBrick brick{};     // 4x4x4-node bricks of the fused kernels (levels >= 1)
     float* A = nullptr;  // 27-point block stencil (levels >= 1)
     float *wd = nullptr, *rhs = nullptr, *x = nullptr, *t = nullptr;
+    // Chebyshev levels: 1 / row sums and five iterate / direction buffers
+    float* mi = nullptr;
+    float* cx[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   };
   std::vector<MgLevel> mg;
   int mg_fuse_from = 0;       // first level of the fused V-cycle kernels (0: none)
@@ -377,6 +401,13 @@ int mg_build_hierarchy(amt_problem* p, const double* Ke) {
   // they are a tail of the hierarchy), when there is a stencil level above the dense one
   p->mg_fuse_from = 0;
   const int nl = (int)p->mg.size();
+  // Chebyshev smoother on the stencil levels from 2 on
+  if (p->cfg.mg_coarse_cheb > 0)
+    for (int l = 2; l <= nl - 2; ++l) {
+      const int64_t nd = 3 * p->mg[l].g.n_nodes;
+      AMT_CHECK(dalloc(p, &p->mg[l].mi, nd));
+      for (auto& b : p->mg[l].cx) AMT_CHECK(dalloc(p, &b, nd));
+    }
   if (p->cfg.mg_fused && nl >= 3 && p->mg_nc <= kDenseMax) {
     for (int l = 1; l <= nl - 2; ++l)
       if (p->mg[l].g.n_nodes <= kFuseMaxNodes) {
@@ -408,7 +439,8 @@ int mg_setup(amt_problem* p) {
                          dim3(9 * kCoarsenNodes), 0,
                          p->stream, mg[l - 1].A, mg[l - 1].g, mg[l].g, mg[l].A);
     hipLaunchKernelGGL(k_mg_stencil_wdinv, dim3(nblk(3 * mg[l].g.n_nodes)), dim3(kBlock), 0,
-                       p->stream, mg[l].A, mg[l].g, kMgOmega, kMgTheta, mg[l].wd);
+                       p->stream, mg[l].A, mg[l].g, kMgOmega, kMgTheta, mg[l].wd, mg[l].mi,
+                       (float)cheb_coef(1).b);
   }
   // coarsest level: dense inverse on the device (no host round trip, the OC iteration stays
   // asynchronous)
@@ -687,52 +719,119 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q,
                        mg[0].x, mg[0].t, done);
   else
     launch_mg0_pre(p, p->op_post, r, done);
-  // Levels 1 .. f-1 run one launch per step (each restriction also writes the next level's
-  // pre-smoothing step x0 = wd * rhs); levels f .. L-2 and the dense level run the fused
-  // kernels (k_mg_down / k_mg_rdense / k_mg_up). Without fused levels, f = L - 1: every
-  // stencil level runs the one-step kernels and the dense solve is its own launch.
-  const int nl = (int)L;
+  // Stencil levels 1 .. L-2, then the dense level L-1. Levels below f run one launch per step
+  // (each restriction also writes the level's first smoothing step from zero); levels from f
+  // on the fused brick kernels (k_mg_down / k_mg_up) and the dense level k_mg_rdense. Levels
+  // with buffers for it (l >= 2, mg_coarse_cheb > 0) smooth with the Chebyshev polynomial:
+  // pre from zero, then post from the corrected iterate, k steps each, the residual for the
+  // restriction formed after the last pre step.
+  const int nl = (int)L, k_ch = p->cfg.mg_coarse_cheb;
   const bool fused = p->mg_fuse_from > 0;
   const int f = fused ? p->mg_fuse_from : nl - 1;
-  if (!fused || f > 1)
-    hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[1].g.n_nodes)), dim3(kBlock), 0, p->stream,
-                       mg[0].t, mg[0].g, mg[1].rhs, mg[1].g, nl > 2 ? mg[1].wd : nullptr, mg[1].x);
-  for (int l = 1; l < f; ++l) {
-    hipLaunchKernelGGL(k_mg_pre, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
-                       mg[l].A, mg[l].g, mg[l].rhs, mg[l].x, mg[l].t);
-    if (!fused || l + 1 < f)
-      hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[l + 1].g.n_nodes)), dim3(kBlock), 0,
-                         p->stream, mg[l].t, mg[l].g, mg[l + 1].rhs, mg[l + 1].g,
-                         l + 2 < nl ? mg[l + 1].wd : nullptr, mg[l + 1].x);
-  }
-  const float* res;  // correction of the level below the current one
-  if (fused) {
-    auto bgrid = [](const MgBrick& b) { return dim3(8u * (unsigned)((b.nbi * b.nbj * b.nbk + 7) / 8)); };
-    for (int l = f; l <= nl - 2; ++l)
+  auto bgrid = [](const MgBrick& b) { return dim3(8u * (unsigned)((b.nbi * b.nbj * b.nbk + 7) / 8)); };
+  auto plane_grid = [](const MgGrid& g) { return dim3(mg_plane_grid(g.n_nodes)); };
+  // Chebyshev steps s0 .. k on level l from buffers (xi, di); returns the buffer of the result
+  auto cheb_steps = [&](int l, int s0, int xi, int di) {
+    for (int st = s0; st <= k_ch; ++st) {
+      int xo = 0, dn = -1;
+      while (xo == xi || xo == di) ++xo;
+      if (st < k_ch) {
+        dn = 0;
+        while (dn == xi || dn == di || dn == xo) ++dn;
+      }
+      const ChebCoef c = cheb_coef(st);
+      hipLaunchKernelGGL(k_mg_cheb, plane_grid(mg[l].g), dim3(3 * kPlaneNodes), 0, p->stream,
+                         mg[l].A, mg[l].g, mg[l].mi, mg[l].rhs, mg[l].cx[xi], mg[l].cx[di],
+                         (float)c.a, (float)c.b, mg[l].cx[xo], dn >= 0 ? mg[l].cx[dn] : nullptr);
+      xi = xo;
+      di = dn;
+    }
+    return xi;
+  };
+  int xpre[32] = {};  // Chebyshev levels: buffer of the pre-smoothed iterate
+  for (int l = 1; l <= nl - 2; ++l) {
+    const bool fz = fused && l >= f, ch = mg[l].mi != nullptr;
+    if (!fz) {
+      hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
+                         mg[l - 1].t, mg[l - 1].g, mg[l].rhs, mg[l].g, mg[l].wd,
+                         ch ? mg[l].cx[0] : mg[l].x);
+      if (!ch) {
+        hipLaunchKernelGGL(k_mg_pre, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0,
+                           p->stream, mg[l].A, mg[l].g, mg[l].rhs, mg[l].x, mg[l].t);
+        continue;
+      }
+      xpre[l] = cheb_steps(l, 2, 0, 0);  // x1 = d1 in buffer 0
+    } else if (!ch) {
       hipLaunchKernelGGL(k_mg_down, bgrid(mg[l].brick), dim3(kFusedThreads), 0, p->stream,
                          mg[l - 1].t, mg[l - 1].g, mg[l].A, mg[l].wd, mg[l].g, mg[l].brick,
-                         mg[l].rhs, mg[l].t);
+                         mg[l].rhs, mg[l].t, MgCheb{});
+      continue;
+    } else {
+      MgCheb c;
+      c.mi = mg[l].mi;
+      c.a = (float)cheb_coef(2).a;
+      c.b = (float)cheb_coef(2).b;
+      c.x_out = mg[l].cx[0];
+      c.d_out = mg[l].cx[1];
+      hipLaunchKernelGGL(k_mg_down, bgrid(mg[l].brick), dim3(kFusedThreads), 0, p->stream,
+                         mg[l - 1].t, mg[l - 1].g, mg[l].A, mg[l].wd, mg[l].g, mg[l].brick,
+                         mg[l].rhs, mg[l].t, c);
+      xpre[l] = cheb_steps(l, 3, 0, 1);
+    }
+    // the residual after the Chebyshev pre-smoothing, restricted by the next level
+    hipLaunchKernelGGL(k_mg_pre3, plane_grid(mg[l].g), dim3(3 * kPlaneNodes), 0, p->stream,
+                       mg[l].A, mg[l].g, mg[l].rhs, mg[l].cx[xpre[l]], mg[l].t);
+  }
+  if (fused) {
     hipLaunchKernelGGL(k_mg_rdense, dim3((p->mg_nc + 15) / 16), dim3(kBlock), 0, p->stream,
                        mg[nl - 2].t, mg[nl - 2].g, mg[nl - 1].g, p->mg_ainv, p->mg_nc,
                        mg[nl - 1].x);
-    res = mg[nl - 1].x;
-    for (int l = nl - 2; l >= f; --l) {
-      hipLaunchKernelGGL(k_mg_up, bgrid(mg[l].brick), dim3(kFusedThreads), 0, p->stream, res,
-                         mg[l + 1].g, mg[l].A, mg[l].wd, mg[l].rhs, mg[l].g, mg[l].brick,
-                         mg[l].t);
-      res = mg[l].t;
-    }
   } else {
+    hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(mg[nl - 1].g.n_nodes)), dim3(kBlock), 0,
+                       p->stream, mg[nl - 2].t, mg[nl - 2].g, mg[nl - 1].rhs, mg[nl - 1].g,
+                       nullptr, mg[nl - 1].x);
     hipLaunchKernelGGL(k_mg_dense, dim3((p->mg_nc + 8 * kWaves - 1) / (8 * kWaves)), dim3(kBlock),
                        0, p->stream, p->mg_ainv, p->mg_nc, mg[nl - 1].rhs, mg[nl - 1].x);
-    res = mg[nl - 1].x;
   }
-  for (int l = f - 1; l >= 1; --l) {
-    hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
-                       res, mg[l + 1].g, mg[l].x, mg[l].g, nullptr);
-    hipLaunchKernelGGL(k_mg_post, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
-                       mg[l].A, mg[l].g, mg[l].wd, mg[l].rhs, mg[l].x, mg[l].t);
-    res = mg[l].t;
+  const float* res = mg[nl - 1].x;  // correction of the level below the current one
+  for (int l = nl - 2; l >= 1; --l) {
+    const bool fz = fused && l >= f, ch = mg[l].mi != nullptr;
+    if (!ch) {
+      if (fz) {
+        hipLaunchKernelGGL(k_mg_up, bgrid(mg[l].brick), dim3(kFusedThreads), 0, p->stream, res,
+                           mg[l + 1].g, mg[l].A, mg[l].wd, mg[l].rhs, mg[l].g, mg[l].brick,
+                           mg[l].t, MgCheb{});
+      } else {
+        hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
+                           res, mg[l + 1].g, mg[l].x, mg[l].g, nullptr);
+        hipLaunchKernelGGL(k_mg_post, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0,
+                           p->stream, mg[l].A, mg[l].g, mg[l].wd, mg[l].rhs, mg[l].x, mg[l].t);
+      }
+      res = mg[l].t;
+      continue;
+    }
+    const int xp = xpre[l];
+    int out;
+    if (fz) {
+      int x1 = 0, d1 = 1;
+      if (x1 == xp) x1 = 2;
+      if (d1 == xp) d1 = 2;
+      MgCheb c;
+      c.mi = mg[l].mi;
+      c.b = (float)cheb_coef(1).b;
+      c.x_out = mg[l].cx[x1];
+      c.d_out = mg[l].cx[d1];
+      c.xpre = mg[l].cx[xp];
+      hipLaunchKernelGGL(k_mg_up, bgrid(mg[l].brick), dim3(kFusedThreads), 0, p->stream, res,
+                         mg[l + 1].g, mg[l].A, mg[l].wd, mg[l].rhs, mg[l].g, mg[l].brick,
+                         mg[l].t, c);
+      out = cheb_steps(l, 2, x1, d1);
+    } else {
+      hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
+                         res, mg[l + 1].g, mg[l].cx[xp], mg[l].g, nullptr);
+      out = cheb_steps(l, 1, xp, xp);
+    }
+    res = mg[l].cx[out];
   }
   hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(p->n_n)), dim3(kBlock), 0, p->stream, res,
                      mg[1].g, mg[0].x, mg[0].g, p->fixcnt);
@@ -1446,6 +1545,7 @@ void amt_config_default(amt_config* c) {
   c->use_graphs = 1;
   c->operator_tile = -1;
   c->mg_fused = 1;
+  c->mg_coarse_cheb = 3;
 }
 
 int amt_element_stiffness(double E, double nu, double h, double* Ke) {
@@ -1477,6 +1577,8 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
     return fail(nullptr, AMT_EINVAL, "unknown precision");
   if (cfg->mg_fused != 0 && cfg->mg_fused != 1)
     return fail(nullptr, AMT_EINVAL, "mg_fused must be 0 or 1");
+  if (cfg->mg_coarse_cheb != 0 && (cfg->mg_coarse_cheb < 2 || cfg->mg_coarse_cheb > 4))
+    return fail(nullptr, AMT_EINVAL, "mg_coarse_cheb must be 0 or 2..4");
   if (cfg->operator_tile < -1 || cfg->operator_tile >= kOpCandidates)
     return fail(nullptr, AMT_EINVAL, "operator_tile must be -1 (autotune) or a candidate 0.." +
                                          std::to_string(kOpCandidates - 1));

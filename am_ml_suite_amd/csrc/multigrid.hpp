@@ -319,9 +319,13 @@ __global__ __launch_bounds__(kBlock) void k_mg_wd0(const double* __restrict__ s,
   wd[3 * n + r] = smoother_weight(diag, rowabs, omega, theta);
 }
 
-// Stencil level, thread per (node, row): the same weights from the stored stencil.
+// Stencil level, thread per (node, row): the same weights from the stored stencil. With mi
+// (a Chebyshev level), mi = 1 / sum_j |A_ij| (the preconditioner of the Chebyshev smoother,
+// under which the spectrum of M^-1 A lies in (0, 1] by Gershgorin) and wd = mi * w1, the
+// smoother's first step from zero; both 0 on rows without a positive diagonal.
 __global__ void k_mg_stencil_wdinv(const float* __restrict__ A, MgGrid g, float omega,
-                                   float theta, float* __restrict__ wd) {
+                                   float theta, float* __restrict__ wd,
+                                   float* __restrict__ mi = nullptr, float w1 = 0.f) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= 3 * g.n_nodes) return;
   const int64_t n = t / 3;
@@ -339,7 +343,14 @@ __global__ void k_mg_stencil_wdinv(const float* __restrict__ A, MgGrid g, float 
 #pragma unroll
     for (int c = 0; c < 3; ++c) rowabs += fabs((double)stencil_coef(A, g.n_nodes, n, d, r, c, sj, si));
   }
-  wd[t] = smoother_weight((double)A[(int64_t)(4 * r) * g.n_nodes + n], rowabs, omega, theta);
+  const double diag = (double)A[(int64_t)(4 * r) * g.n_nodes + n];
+  if (mi) {
+    const float m = (diag > 0.0 && rowabs > 0.0) ? (float)(1.0 / rowabs) : 0.f;
+    mi[t] = m;
+    wd[t] = m * w1;
+  } else {
+    wd[t] = smoother_weight(diag, rowabs, omega, theta);
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -564,6 +575,123 @@ inline unsigned mg_row_grid(int64_t n_nodes) {
   return 8u * (unsigned)((3 * nr + 7) / 8);
 }
 
+// Plane-split stencil sweeps: a block takes 64 consecutive nodes of one row component r, and
+// wave w the neighbour plane di = w - 1: its 27 coefficients and 27 neighbour values, 54 loads
+// that leave in one batch (a whole row per thread is 162 loads in three). Wave 0 adds the
+// three plane partials in fixed order. (node chunk, r) pairs are dealt XCD-contiguously.
+// Grid: 8 * ceil(3 * ceil(n_nodes / 64) / 8) blocks of 192 threads.
+constexpr int kPlaneNodes = 64;
+__device__ __forceinline__ bool mg_plane_block(const MgGrid& g, int& n, int& r) {
+  const int nr = (int)((g.n_nodes + kPlaneNodes - 1) / kPlaneNodes);
+  const int per = (3 * nr + 7) / 8;
+  const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  if (logical >= 3 * nr) return false;
+  r = logical % 3;
+  n = (logical / 3) * kPlaneNodes + (int)(threadIdx.x & 63);
+  return true;
+}
+inline unsigned mg_plane_grid(int64_t n_nodes) {
+  const int64_t nr = (n_nodes + kPlaneNodes - 1) / kPlaneNodes;
+  return 8u * (unsigned)((3 * nr + 7) / 8);
+}
+
+// Partial of row r at node n = (i, j, k) over the neighbour plane DI: the sum over (dj, dk, c)
+// of A(n, n + (DI, dj, dk))[r][c] x(n + (DI, dj, dk))[c], with x zero outside the grid (read
+// at n there, like a backward block's coefficients).
+template <int DI>
+__device__ __forceinline__ float mg_plane_row(const float* __restrict__ A, const MgGrid& g, int n,
+                                              int i, int j, int k, int r,
+                                              const float* __restrict__ x) {
+  const int sj = g.nnz, si = g.nny * g.nnz;
+  const int64_t nn = g.n_nodes;
+  float p = 0.f;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    const int dj = e / 3 - 1, dk = e % 3 - 1;
+    const int d = (DI + 1) * 9 + (dj + 1) * 3 + (dk + 1);
+    const bool ok = (i + DI) >= 0 && (i + DI) < g.nnx && (j + dj) >= 0 && (j + dj) < g.nny &&
+                    (k + dk) >= 0 && (k + dk) < g.nnz;
+    const int m = ok ? n + DI * si + dj * sj + dk : n;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float xv = x[(unsigned)(3 * m + c)];
+      const float co = d >= 13 ? A[(int64_t)((d - 13) * 9 + 3 * r + c) * nn + n]
+                               : A[(int64_t)((13 - d) * 9 + 3 * c + r) * nn + m];
+      p = fmaf(co, ok ? xv : 0.f, p);
+    }
+  }
+  return p;
+}
+// y = (A x)[3n + r] for the calling block's chunk, from the three waves; the returned value
+// is valid in wave 0 only.
+__device__ __forceinline__ float mg_plane_apply(const float* __restrict__ A, const MgGrid& g,
+                                                int n, int r, const float* __restrict__ x,
+                                                float (*part)[kPlaneNodes]) {
+  const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  const int nc = n < g.n_nodes ? n : 0;  // past the grid: node 0 (in bounds, unused)
+  const int k = nc % g.nnz, t = nc / g.nnz, j = t % g.nny, i = t / g.nny;
+  float p;
+  if (w == 0)
+    p = mg_plane_row<-1>(A, g, nc, i, j, k, r, x);
+  else if (w == 1)
+    p = mg_plane_row<0>(A, g, nc, i, j, k, r, x);
+  else
+    p = mg_plane_row<1>(A, g, nc, i, j, k, r, x);
+  if (w > 0) part[w - 1][lane] = p;
+  __syncthreads();
+  return (p + part[0][lane]) + part[1][lane];
+}
+
+__global__ __launch_bounds__(3 * kPlaneNodes) void k_mg_pre3(const float* __restrict__ A,
+                                                            MgGrid g,
+                                                            const float* __restrict__ rhs,
+                                                            const float* __restrict__ x0,
+                                                            float* __restrict__ t) {
+  __shared__ float part[2][kPlaneNodes];
+  int n, r;
+  if (!mg_plane_block(g, n, r)) return;  // whole block: no barrier follows for it
+  const float y = mg_plane_apply(A, g, n, r, x0, part);
+  if (threadIdx.x < kPlaneNodes && n < g.n_nodes) {
+    const unsigned q = 3 * n + r;
+    t[q] = rhs[q] - y;
+  }
+}
+
+__global__ __launch_bounds__(3 * kPlaneNodes) void k_mg_post3(const float* __restrict__ A,
+                                                             MgGrid g,
+                                                             const float* __restrict__ wd,
+                                                             const float* __restrict__ rhs,
+                                                             const float* __restrict__ x,
+                                                             float* __restrict__ out) {
+  __shared__ float part[2][kPlaneNodes];
+  int n, r;
+  if (!mg_plane_block(g, n, r)) return;  // whole block: no barrier follows for it
+  const float y = mg_plane_apply(A, g, n, r, x, part);
+  if (threadIdx.x < kPlaneNodes && n < g.n_nodes) {
+    const unsigned q = 3 * n + r;
+    out[q] = x[q] + wd[q] * (rhs[q] - y);
+  }
+}
+
+// One step of the Chebyshev smoother on a stencil level: r = mi (rhs - A x), d' = a d + b r
+// (d ignored when a = 0), x' = x + d'. d_out may be null (last step).
+__global__ __launch_bounds__(3 * kPlaneNodes) void k_mg_cheb(
+    const float* __restrict__ A, MgGrid g, const float* __restrict__ mi,
+    const float* __restrict__ rhs, const float* __restrict__ x, const float* __restrict__ d,
+    float a, float b, float* __restrict__ x_out, float* __restrict__ d_out) {
+  __shared__ float part[2][kPlaneNodes];
+  int n, r;
+  if (!mg_plane_block(g, n, r)) return;  // whole block: no barrier follows for it
+  const float y = mg_plane_apply(A, g, n, r, x, part);
+  if (threadIdx.x < kPlaneNodes && n < g.n_nodes) {
+    const unsigned q = 3 * n + r;
+    const float rr = mi[q] * (rhs[q] - y);
+    const float dn = a != 0.f ? fmaf(a, d[q], b * rr) : b * rr;
+    x_out[q] = x[q] + dn;
+    if (d_out) d_out[q] = dn;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_mg_pre(const float* __restrict__ A, MgGrid g,
                                                    const float* __restrict__ rhs,
                                                    const float* __restrict__ x0,
@@ -642,6 +770,17 @@ constexpr int kFuseMaxNodes = 16384;     // levels fused (at most this many node
 // Bricks per axis; the bricks tile the level from node 0, the last ones reach past the grid.
 struct MgBrick {
   int nbi, nbj, nbk;
+};
+
+// Chebyshev mode of the fused kernels (mi null: damped Jacobi). The brick rows take one
+// smoother step r = mi (rhs - A x), d' = a d + b r, x' = x + d' and store x' and d'; k_mg_up
+// starts from x = xpre + P c (xpre: the level's pre-smoothed iterate) with a = 0.
+struct MgCheb {
+  const float* mi = nullptr;
+  float a = 0.f, b = 0.f;
+  float* x_out = nullptr;
+  float* d_out = nullptr;
+  const float* xpre = nullptr;
 };
 
 // Brick of this block, dealt XCD-contiguously (hardware block b runs on XCD b % 8; XCD x takes
@@ -764,7 +903,7 @@ __device__ __forceinline__ void mg_brick_rows(const float* P, const MgGrid& g, i
 __global__ __launch_bounds__(kFusedThreads) void k_mg_down(
     const float* __restrict__ tf, MgGrid fg, const float* __restrict__ A,
     const float* __restrict__ wd, MgGrid g, MgBrick b, float* __restrict__ rhs,
-    float* __restrict__ t_out) {
+    float* __restrict__ t_out, MgCheb ch) {
   // fine region F (x, y, z), z-restricted G (x, y, kz), y-restricted H (x, jy, kz) in F's
   // space, shell rhs R and x0 X, plane partials P; 3 floats a node
   __shared__ float F[3 * kFF * kFF * kFF], G[3 * kFF * kFF * kFS];
@@ -850,14 +989,22 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_down(
   P[threadIdx.x] = tk.valid ? mg_task_partial(co, X, tk) : 0.f;
   __syncthreads();
   mg_brick_rows(P, g, i0, j0, k0, [&](int n, int r, int q, float y) {
-    t_out[(unsigned)(3 * n + r)] = R[3 * q + r] - y;
+    const unsigned e = 3 * n + r;
+    if (ch.mi) {  // second Chebyshev step (X holds x1 = d1)
+      const float x1 = X[3 * q + r];
+      const float dn = fmaf(ch.a, x1, ch.b * (ch.mi[e] * (R[3 * q + r] - y)));
+      ch.x_out[e] = x1 + dn;
+      ch.d_out[e] = dn;
+    } else {
+      t_out[e] = R[3 * q + r] - y;
+    }
   });
 }
 
 __global__ __launch_bounds__(kFusedThreads) void k_mg_up(
     const float* __restrict__ xc, MgGrid cg, const float* __restrict__ A,
     const float* __restrict__ wd, const float* __restrict__ rhs, MgGrid g, MgBrick b,
-    float* __restrict__ out) {
+    float* __restrict__ out, MgCheb ch) {
   // coarse correction region C (the parents of the shell), shell rhs, wd and x, partials P
   constexpr int kC = kFB / 2 + 3;
   __shared__ float C[3 * kC * kC * kC], RS[3 * kFSN], WS[3 * kFSN], X[3 * kFSN], P[9 * kFBN];
@@ -882,7 +1029,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(
   if (in) {
     const int m = (I * g.nny + J) * g.nnz + K;
     rv = mg_load3(rhs, m);
-    wv = mg_load3(wd, m);
+    wv = mg_load3(ch.mi ? ch.xpre : wd, m);  // Chebyshev: the pre-smoothed x itself
   }
   if (threadIdx.x < kC * kC * kC) {
 #pragma unroll
@@ -913,7 +1060,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      X[3 * q + c] = in ? wv.v[c] * rv.v[c] + acc[c] : 0.f;
+      X[3 * q + c] = in ? (ch.mi ? wv.v[c] : wv.v[c] * rv.v[c]) + acc[c] : 0.f;
       RS[3 * q + c] = rv.v[c];
       WS[3 * q + c] = wv.v[c];
     }
@@ -923,7 +1070,14 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(
   __syncthreads();
   mg_brick_rows(P, g, i0, j0, k0, [&](int n, int r, int q, float y) {
     const int e = 3 * q + r;
-    out[(unsigned)(3 * n + r)] = X[e] + WS[e] * (RS[e] - y);
+    const unsigned o = 3 * n + r;
+    if (ch.mi) {  // first post-smoothing Chebyshev step
+      const float dn = ch.b * (ch.mi[o] * (RS[e] - y));
+      ch.x_out[o] = X[e] + dn;
+      ch.d_out[o] = dn;
+    } else {
+      out[o] = X[e] + WS[e] * (RS[e] - y);
+    }
   });
 }
 

@@ -68,6 +68,7 @@ class SIMPConfig:
     use_graphs: bool = True
     operator_tile: int = -1           # level-0 operator launch: -1 autotune, else a candidate
     mg_fused: bool = True             # fused V-cycle kernels on the small levels
+    mg_coarse_cheb: int = 3           # Chebyshev degree of the smoother on levels >= 2 (0: Jacobi)
 
 
 @dataclass
@@ -143,6 +144,7 @@ def make_amt_config(nx, ny, nz, element_size, material, config):
     c.use_graphs = int(bool(config.use_graphs))
     c.operator_tile = int(config.operator_tile)
     c.mg_fused = int(bool(config.mg_fused))
+    c.mg_coarse_cheb = int(config.mg_coarse_cheb)
     return c
 
 

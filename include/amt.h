@@ -79,6 +79,9 @@ typedef struct amt_config {
                             prolongation + post-smoothing) and one for the dense level; 0 one
                             launch per smoothing or transfer step. Same linear map, summed in
                             another order. */
+  int mg_coarse_cheb;    /* smoother of the multigrid levels from 2 on: 0 damped Jacobi (as on
+                            levels 0 and 1), k = 2..4 the degree-k Chebyshev polynomial in
+                            M^-1 A, M = diag(sum_j |A_ij|) (default 3) */
 } amt_config;
 
 /* IterationResult (topology_optimizer.hpp:65-74) plus solver detail. */
