@@ -91,10 +91,20 @@ constexpr int64_t kMgCoarseDofs = 192;
 
 // Level-0 operator candidates: 0 = the 4 x 4 x 16 node gather tile (tile_op.hpp); 1 + 3 s + l =
 // the element-column streaming tile (stream_op.hpp) of shape kStreamShape[s] (threads along j,
-// k) with kStreamPlanes[l] node planes per block.
+// k) with kStreamPlanes[l] node planes per block (4, 8, 16); 10 + 2 s + l = the same shapes with
+// kStreamPlanes[3 + l] planes (2, 3).
 constexpr int kStreamShape[3][2] = {{16, 16}, {8, 32}, {4, 64}};
-constexpr int kStreamPlanes[3] = {4, 8, 16};
-constexpr int kOpCandidates = 1 + 3 * 3;
+constexpr int kStreamPlanes[5] = {4, 8, 16, 2, 3};
+constexpr int kOpCandidates = 1 + 3 * 5;
+inline void stream_candidate(int cand, int& shape, int& planes) {
+  if (cand <= 9) {
+    shape = (cand - 1) / 3;
+    planes = kStreamPlanes[(cand - 1) % 3];
+  } else {
+    shape = (cand - 10) / 2;
+    planes = kStreamPlanes[3 + (cand - 10) % 2];
+  }
+}
 
 // Partials regions (doubles) inside one buffer; see amt_problem::part()
 enum Region { R_APPLY = 0, R_UPD, R_INIT, R_OC, R_MISC, R_COUNT };
@@ -476,9 +486,9 @@ amt_problem::OpSel make_op_sel(const amt_problem* p, int cand) {
     s.nb = p->nb_tile;
     return s;
   }
-  s.shape = (cand - 1) / 3;
-  s.sg = make_stream_grid(p->g, kStreamShape[s.shape][0], kStreamShape[s.shape][1],
-                          kStreamPlanes[(cand - 1) % 3]);
+  int planes = 0;
+  stream_candidate(cand, s.shape, planes);
+  s.sg = make_stream_grid(p->g, kStreamShape[s.shape][0], kStreamShape[s.shape][1], planes);
   s.nb = stream_blocks(s.sg);
   return s;
 }

@@ -381,7 +381,7 @@ class TopologyOptimizer:
 
     def operator_timings(self, kind=0):
         """The autotuner's launch time (microseconds) of every candidate for that kind."""
-        out = np.zeros(10)
+        out = np.zeros(16)
         self._check(self._L.amt_operator_timings(self._h, int(kind), out, out.size))
         return out
 

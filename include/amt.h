@@ -73,7 +73,8 @@ typedef struct amt_config {
                             first solve (default); 0 the 4x4x16 node gather tile; 1 + 3 s + l the
                             element-column streaming tile of shape s (0: 16x16, 1: 8x32, 2: 4x64
                             threads along j x k) with 4, 8 or 16 (l = 0, 1, 2) node planes per
-                            block. All streaming shapes give the same operator values. */
+                            block; 10 + 2 s + l the same shapes with 2 or 3 (l = 0, 1) planes.
+                            All streaming shapes give the same operator values. */
   int mg_fused;          /* multigrid V-cycle on the levels of at most 16,384 nodes: 1 (default)
                             two brick-tiled launches per level (restriction + pre-smoothing,
                             prolongation + post-smoothing) and one for the dense level; 0 one

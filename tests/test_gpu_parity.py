@@ -200,8 +200,8 @@ def test_oc_update_exact_tie_matches_oracle():
 # ---- level-0 operator launches (gather tile and element-column streaming tiles) ------------
 @pytest.mark.parametrize("shape", [(7, 5, 4), (33, 18, 70), (2, 1, 1)])
 def test_operator_tiles_match_assembled_csr(shape):
-    """Every level-0 operator candidate (operator_tile 0: the node gather tile; 1..9: the
-    Hadamard-form element-column streaming tile in three shapes x three plane counts) against
+    """Every level-0 operator candidate (operator_tile 0: the node gather tile; 1..15: the
+    Hadamard-form element-column streaming tile in three shapes x five plane counts) against
     the oracle's assembled CSR with penalty rows, 1e-13 relative; all streaming candidates give
     bitwise identical results (the same per-element products summed in the same order); the
     FP32 streaming operator within 2e-6 of the FP64 one. 33x18x70 spans several tiles along
@@ -216,7 +216,7 @@ def test_operator_tiles_match_assembled_csr(shape):
     x = rng.standard_normal(n)
     ref = O.spmv(rp, col, val, x)
     ys = {}
-    for tile in range(10):
+    for tile in range(16):
         opt = _opt(nx, ny, nz, E_min=1e-9, rho_min=0.01, density_index="consistent",
                    operator_tile=tile)
         opt.set_loads(F, fixed)
@@ -230,7 +230,7 @@ def test_operator_tiles_match_assembled_csr(shape):
             y32 = opt.apply_operator(x, with_bc=True, precision="mixed_fp32")
             assert _rel(y32, y) <= 2e-6, (tile, _rel(y32, y))
         opt.close()
-    for tile in range(2, 10):
+    for tile in range(2, 16):
         assert np.array_equal(ys[tile], ys[1]), tile
 
 
@@ -242,7 +242,7 @@ def test_multigrid_solve_operator_tiles():
     count follows the tile, so the iterates differ in rounding.)"""
     p = A.bracket_like_problem(40, 20, 20)
     out = {}
-    for tile in (0, 1, 5, 9, -1):
+    for tile in (0, 1, 5, 9, 13, -1):
         opt = A.TopologyOptimizer(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3),
                                   make_config(p, preconditioner="multigrid", operator_tile=tile))
         opt.set_loads(p.F, p.fixed)
