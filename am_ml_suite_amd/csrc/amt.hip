@@ -111,7 +111,7 @@ enum Region { R_APPLY = 0, R_UPD, R_INIT, R_OC, R_MISC, R_COUNT };
 
 // Device scalar slots
 enum Scal { S_MAXDIAG = 0, S_P, S_COMPL, S_NDESIGN, S_RR, S_BB, S_FLAG, S_NNZZ, S_NNRDR,
-            S_NNUAU, S_NNUR, S_NND0, S_NNTH = S_NND0 + 5, S_COUNT };
+            S_NNUAU, S_NNUR, S_NND0, S_NNTH = S_NND0 + 5, S_PROJ, S_COUNT = S_PROJ + 2 };
 
 
 }  // namespace
@@ -148,6 +148,7 @@ struct amt_problem {
   double *F = nullptr, *Fb = nullptr, *diag = nullptr, *dinv = nullptr;
   float* dinv32 = nullptr;
   double *u = nullptr, *r = nullptr, *z = nullptr, *p[2] = {nullptr, nullptr}, *q = nullptr;
+  double* u2 = nullptr;  // warm_start 2: the displacement before the previous one (zeros at first)
   float *r32 = nullptr, *z32 = nullptr, *p32[2] = {nullptr, nullptr}, *q32 = nullptr,
         *d32 = nullptr;
   double *dc = nullptr, *dcf = nullptr;
@@ -923,7 +924,23 @@ int solve_mg(amt_problem* p, amt_solve_stats* st) {
     // the start residual of a warm solve is a training sample while collection is armed; with a
     // network attached it also takes the learned step u += gamma M(r)
     if ((int)p->bank.size() < p->warm_collect) AMT_CHECK(bank_push(p, p->r));
-    if (p->net) AMT_CHECK(learned_warm_start(p));
+    if (p->net) {
+      AMT_CHECK(learned_warm_start(p));
+    } else if (p->cfg.warm_start == 2 && p->u2) {
+      // energy-optimal start in span{u_prev, u_prev2}: w = K u2 into q
+      launch_apply64(p, p->op_apply, p->u2, p->q, p->scal + S_P);
+      const unsigned nb = nred(p->n_dof);
+      hipLaunchKernelGGL(k_proj_dots, dim3(nb), dim3(kBlock), 0, p->stream, p->u, p->u2, p->Fb,
+                         p->r, p->q, p->n_dof, p->part(R_OC), p->stride_red);
+      hipLaunchKernelGGL(k_proj_coef, dim3(1), dim3(kBlock), 0, p->stream,
+                         (const double*)p->part(R_OC), (int)nb, p->stride_red, p->scal + S_PROJ);
+      hipLaunchKernelGGL(k_proj_apply, dim3(nb), dim3(kBlock), 0, p->stream, p->u, p->u2, p->Fb,
+                         p->r, p->q, p->n_dof, (const double*)(p->scal + S_PROJ),
+                         p->part(R_MISC), p->stride_red);
+      hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kBlock), 0, p->stream, p->part(R_MISC),
+                         (int)nb, p->stride_red, p->scal + S_RR);
+      AMT_CHECK(check_launch(p, "two-solution warm start"));
+    }
   }
   hipLaunchKernelGGL(k_fcg_start, dim3(1), dim3(64), 0, p->stream, p->fcg, p->scal + S_RR,
                      p->scal + S_BB, p->cfg.pcg_tol, p->cfg.pcg_maxit);
@@ -1585,6 +1602,8 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
   if (cfg->precision != AMT_PREC_FP64 && cfg->precision != AMT_PREC_MIXED_FP32 &&
       cfg->precision != AMT_PREC_MIXED_BF16)
     return fail(nullptr, AMT_EINVAL, "unknown precision");
+  if (cfg->warm_start < 0 || cfg->warm_start > 2)
+    return fail(nullptr, AMT_EINVAL, "warm_start must be 0, 1 or 2");
   if (cfg->mg_fused != 0 && cfg->mg_fused != 1)
     return fail(nullptr, AMT_EINVAL, "mg_fused must be 0 or 1");
   if (cfg->mg_coarse_cheb != 0 && (cfg->mg_coarse_cheb < 2 || cfg->mg_coarse_cheb > 4))
@@ -1645,6 +1664,11 @@ int amt_create(const amt_config* cfg, amt_problem** out) {
   CREATE_TRY(dalloc(p, &p->p[0], nd));
   CREATE_TRY(dalloc(p, &p->p[1], nd));
   CREATE_TRY(dalloc(p, &p->q, nd));
+  if (cfg->warm_start == 2 && cfg->precond == AMT_PRECOND_MULTIGRID) {
+    CREATE_TRY(dalloc(p, &p->u2, nd));
+    if (hipMemset(p->u2, 0, sizeof(double) * nd) != hipSuccess)
+      return bail(fail(p, AMT_EHIP, "hipMemset failed"));
+  }
   if (cfg->precision == AMT_PREC_MIXED_FP32 || cfg->precision == AMT_PREC_MIXED_BF16) {
     CREATE_TRY(dalloc(p, &p->r32, nd));
     CREATE_TRY(dalloc(p, &p->z32, nd));

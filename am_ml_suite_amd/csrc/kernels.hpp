@@ -318,6 +318,76 @@ __global__ __launch_bounds__(kBlock) void k_dot(const double* __restrict__ a,
 }
 
 // ---------------------------------------------------------------------------------------
+// Warm start in span{u1, u2} (u1 the previous displacement, u2 the one before): the start
+// that minimises the energy 1/2 u^T K u - F.u over the span. With r = F - K u1 and w = K u2,
+// the 2x2 system needs u1.F, u1.r, u2.F, u2.r and u2.w only, and the new start's residual is
+// (1 - c1) F + c1 r - c2 w: one operator application (K u2) in all. It is never worse in
+// energy than u1 alone; a degenerate system (u2 = 0 or parallel to u1) falls back to the
+// energy-optimal multiple of u1.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_proj_dots(const double* __restrict__ u1,
+                                                      const double* __restrict__ u2,
+                                                      const double* __restrict__ F,
+                                                      const double* __restrict__ r,
+                                                      const double* __restrict__ w, int64_t n,
+                                                      double* partials, int stride) {
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  AMT_GRID_STRIDE(i, n) {
+    const double a = u1[i], b = u2[i], f = F[i], rr = r[i];
+    v[0] = fma(a, f, v[0]);
+    v[1] = fma(a, rr, v[1]);
+    v[2] = fma(b, f, v[2]);
+    v[3] = fma(b, rr, v[3]);
+    v[4] = fma(b, w[i], v[4]);
+  }
+  store_partials_many<5>(v, partials, stride);
+}
+
+// c = argmin of the energy over span{u1, u2} (one block); out[0..1] = c1, c2.
+__global__ __launch_bounds__(kBlock) void k_proj_coef(const double* partials, int nb, int stride,
+                                                      double* out) {
+  double t[5];
+  reduce_partials<5>(partials, nb, stride, t);
+  if (threadIdx.x) return;
+  const double a11 = t[0] - t[1], a12 = t[2] - t[3], a22 = t[4], b1 = t[0], b2 = t[2];
+  const double det = a11 * a22 - a12 * a12;
+  double c1 = 1.0, c2 = 0.0;
+  if (a11 > 0.0 && a22 > 0.0 && det > 1e-10 * a11 * a22) {
+    c1 = (b1 * a22 - b2 * a12) / det;
+    c2 = (a11 * b2 - a12 * b1) / det;
+  } else if (a11 > 0.0) {
+    c1 = b1 / a11;
+  }
+  if (!isfinite(c1) || !isfinite(c2)) {
+    c1 = 1.0;
+    c2 = 0.0;
+  }
+  out[0] = c1;
+  out[1] = c2;
+}
+
+// u <- c1 u + c2 u2, u2 <- u (the old value), r <- (1 - c1) F + c1 r - c2 w; partials of |r|^2.
+__global__ __launch_bounds__(kBlock) void k_proj_apply(double* __restrict__ u,
+                                                       double* __restrict__ u2,
+                                                       const double* __restrict__ F,
+                                                       double* __restrict__ r,
+                                                       const double* __restrict__ w, int64_t n,
+                                                       const double* __restrict__ c,
+                                                       double* partials, int stride) {
+  const double c1 = c[0], c2 = c[1];
+  double acc[1] = {0.0};
+  AMT_GRID_STRIDE(i, n) {
+    const double a = u[i];
+    u[i] = fma(c1, a, c2 * u2[i]);
+    u2[i] = a;
+    const double rn = fma(1.0 - c1, F[i], fma(c1, r[i], -c2 * w[i]));
+    r[i] = rn;
+    acc[0] = fma(rn, rn, acc[0]);
+  }
+  store_partials<1>(acc, partials, stride);
+}
+
+// ---------------------------------------------------------------------------------------
 // Sensitivities (topology_optimizer.cpp:126-161): per slot e (x-fastest decode),
 // ce = u_e^T Ke0 u_e, dc = -p rho^(p-1) ce.
 // ---------------------------------------------------------------------------------------

@@ -62,7 +62,7 @@ class SIMPConfig:
     density_index: str = "reference"  # or "consistent"
     precision: str = "fp64"           # or "mixed_fp32", "mixed_bf16" (operator on MFMA)
     preconditioner: str = "jacobi"    # or "multigrid" (FP32 V-cycle inside FP64 flexible CG)
-    warm_start: bool = False
+    warm_start: int = 0               # 1 (or True): previous displacement; 2: span of the last two
     inner_tolerance: float = 1e-4
     inner_max_iterations: int = 20000
     use_graphs: bool = True
@@ -137,7 +137,7 @@ def make_amt_config(nx, ny, nz, element_size, material, config):
                    "mixed_bf16": _ffi.AMT_PREC_MIXED_BF16}[config.precision]
     c.precond = {"jacobi": _ffi.AMT_PRECOND_JACOBI, "unet": _ffi.AMT_PRECOND_UNET,
                  "multigrid": _ffi.AMT_PRECOND_MULTIGRID}[config.preconditioner]
-    c.warm_start = int(bool(config.warm_start))
+    c.warm_start = int(config.warm_start)  # True -> 1; 2: combination of the last two solutions
     c.inner_tol = config.inner_tolerance
     c.inner_maxit = int(config.inner_max_iterations)
     c.verbose = 0

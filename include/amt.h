@@ -64,7 +64,9 @@ typedef struct amt_config {
   int density_index;     /* AMT_INDEX_* */
   int precision;         /* AMT_PREC_* */
   int precond;           /* AMT_PRECOND_* */
-  int warm_start;        /* 1: start each solve from the previous displacement */
+  int warm_start;        /* 1: start each solve from the previous displacement; 2 (multigrid
+                            solver): from the energy-minimising combination of the previous two
+                            displacements (one extra operator application per solve) */
   double inner_tol;      /* mixed precision: relative reduction per FP32 inner solve */
   int inner_maxit;
   int verbose;

@@ -555,6 +555,34 @@ def test_multigrid_chebyshev_coarse_smoother_cuts_iterations():
     assert sum(its_a) <= 0.85 * sum(its_b)
 
 
+def test_two_solution_warm_start():
+    """warm_start 2 (each solve starts from the energy-minimising combination of the previous
+    two displacements) against warm_start 1 (the previous displacement) on 8 OC iterations of
+    the 120x60x60 bracket with the multigrid solver: the same answers (every solve at the 1e-8
+    FP64 residual, compliance within 1e-8, density within 1e-5) in no more flexible-CG
+    iterations in total."""
+    p = A.bracket_like_problem(120, 60, 60)
+    out = {}
+    for ws in (2, 1):
+        cfg = make_config(p, preconditioner="multigrid", warm_start=ws, max_iterations=8)
+        opt = A.TopologyOptimizer(p.nx, p.ny, p.nz, 1.0, A.MaterialProperties(1.0, 0.3), cfg)
+        res = opt.run(p.rho0, p.F, p.fixed)
+        assert all(r.solver_converged and r.true_rel_residual <= 1.5e-8
+                   for r in res.iteration_results), ws
+        out[ws] = res
+        opt.close()
+    a, b = out[2], out[1]
+    its_a = [r.solver_iterations for r in a.iteration_results]
+    its_b = [r.solver_iterations for r in b.iteration_results]
+    ca, cb = np.array(a.compliance_history), np.array(b.compliance_history)
+    print("FCG iterations warm_start 2 %s, warm_start 1 %s; compliance rel diff %.2e, density "
+          "max-abs %.2e" % (its_a, its_b, np.abs(ca / cb - 1).max(),
+                             np.abs(a.final_density - b.final_density).max()))
+    assert np.abs(ca / cb - 1).max() <= 1e-8
+    assert np.abs(a.final_density - b.final_density).max() <= 1e-5
+    assert sum(its_a) <= sum(its_b)
+
+
 @pytest.mark.parametrize("face", [0, 1, 2, 3, 4, 5])
 def test_multigrid_clamp_on_every_face(face):
     """Multigrid FCG on a 27x14x19 grid (odd sizes: the coarse grids extend one layer past
