@@ -44,7 +44,9 @@ FP64_PEAK_TFLOPS = 78.6
 # (amt_config.operator_tile), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a plain
 # 256 MiB copy (profiles/fcg_apply_bracket120x60x60_hbm_traffic.md)
 FCG_APPLY_TRAFFIC_BYTES = {0: 49.39e6, 1: 73.98e6, 2: 66.39e6, 3: 62.07e6, 4: 66.81e6,
-                          5: 62.17e6, 6: 58.51e6, 7: 60.65e6, 8: 58.30e6, 9: 55.18e6}
+                          5: 62.17e6, 6: 58.51e6, 7: 60.65e6, 8: 58.30e6, 9: 55.18e6,
+                          10: 66.42e6, 11: 70.08e6, 12: 56.91e6, 13: 63.91e6, 14: 51.37e6,
+                          15: 55.39e6}
 
 
 def log(msg):
