@@ -83,7 +83,9 @@ TopologyOptimizer::TopologyOptimizer(int nx, int ny, int nz, Scalar element_size
   c.density_index = (int)config.density_index;
   c.precision = (int)config.precision;
   c.precond = (int)config.preconditioner;
-  c.warm_start = config.warm_start ? 1 : 0;
+  // the multigrid solver starts from the energy-minimising combination of the last two
+  // displacements (amt_config.warm_start 2), the other solvers from the last one
+  c.warm_start = config.warm_start ? (config.preconditioner == Preconditioner::Multigrid ? 2 : 1) : 0;
   c.inner_tol = config.inner_tolerance;
   c.verbose = 0;
   c.use_graphs = 1;

@@ -73,7 +73,8 @@ struct SIMPConfig {
   DensityIndex density_index = DensityIndex::Reference;
   Precision precision = Precision::FP64;
   Preconditioner preconditioner = Preconditioner::Jacobi;
-  bool warm_start = false;
+  bool warm_start = false;  // Multigrid: from the energy-minimising combination of the last two
+                            // displacements; Jacobi: from the last displacement
   Scalar inner_tolerance = 1e-4;
 };
 
