@@ -546,7 +546,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "bracket_like_120x60x60_seed2", "n_dofs": n_dof,
                    "n_elements": n_e, "pcg_tol": 1e-8, "path": main_key,
-                   "solver": ("FP64 flexible CG + FP32 Galerkin multigrid V-cycle, warm start"
+                   "solver": ("FP64 flexible CG + FP32 Galerkin multigrid V-cycle (Chebyshev "
+                              "smoother on levels >= 2), warm start from the last two "
+                              "displacements"
                               + (", fine-level smoothing on MFMA (split BF16)"
                                  if args.fast_smoother == "mfma" else "")
                               if main_key == "fast" else "FP64 Jacobi-PCG")},
@@ -555,7 +557,8 @@ def main():
     if main_key == "fast":
         line["metric_note"] = (
             "value is the mixed-precision path without a network: FP64 flexible CG, FP32 "
-            "Galerkin multigrid V-cycle, warm start from the previous OC iteration. The learned "
+            "Galerkin multigrid V-cycle, warm start from the energy-minimising combination of the "
+            "previous two displacements. The learned "
             "part (U-Net warm start trained on the GPU) is timed on the same workload in "
             "learned.warm_start, against this path without it (speedup_learned_vs_multigrid), "
             "and is not in value")
