@@ -81,6 +81,11 @@ __global__ __launch_bounds__(256) void k_conv_brick(
   const int gbeg = blockIdx.z * gpk, gend = min(ngr, gbeg + gpk);
   const int per = gend - gbeg;  // input-channel groups per brick for this split
   const int nt0 = blockIdx.y * NP;
+  // the block's output channels are fixed: their biases are read once (a read per output value
+  // would wait on the next brick's halo loads, which share the load counter)
+  float bco[NP];
+#pragma unroll
+  for (int n = 0; n < NP; ++n) bco[n] = bias ? bias[(nt0 + n) * 16 + r] : 0.f;
   // work items (brick, group), brick-major; block x takes bricks x, x + gridDim.x, ...
   const int my_bricks = blockIdx.x < nbricks ? (nbricks - 1 - blockIdx.x) / gridDim.x + 1 : 0;
   const int items = per > 0 ? my_bricks * per : 0;
@@ -159,13 +164,22 @@ __global__ __launch_bounds__(256) void k_conv_brick(
     for (int n = 0; n < NP; ++n) {
       Frag8(&bc)[14] = (n & 1) ? bB : bA;
       if (n + 1 < NP) load_b(gr, n + 1, (n & 1) ? bA : bB);
+      // the next tap pair's four A fragments are read while this pair's MFMAs run
+      // (the scheduling groups keep that order: four LDS reads, then four MFMAs)
+      Frag8 a0[4], a1[4];
+      read_a(0, a0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
 #pragma unroll
       for (int c = 0; c < 14; ++c) {
-        Frag8 a[4];
-        read_a(c, a);
+        Frag8(&cur)[4] = (c & 1) ? a1 : a0;
+        if (c + 1 < 14) {
+          read_a(c + 1, (c & 1) ? a0 : a1);
+          __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+        }
 #pragma unroll
         for (int m = 0; m < 4; ++m)
-          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m].v, bc[c].v, acc[m][n], 0, 0, 0);
+          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[m].v, bc[c].v, acc[m][n], 0, 0, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
       }
     }
     if ((t + 1) % per) continue;  // more groups of this brick follow
@@ -174,7 +188,6 @@ __global__ __launch_bounds__(256) void k_conv_brick(
     int i0, j0, k0;
     brick_of(t, i0, j0, k0);
     auto put = [&](int64_t v, int co, float val) {
-      if (bias) val += bias[co];
       uint16_t* dst =
           co < ysplit ? y1 + v * ysplit + co : y2 + v * (cout - ysplit) + (co - ysplit);
       if (accum) val += bf2f(*dst);
@@ -191,7 +204,7 @@ __global__ __launch_bounds__(256) void k_conv_brick(
             const int kk = k0 + 4 * g + e;
             if (kk >= d.pz || i0 + w >= d.px || j0 + m >= d.py) continue;  // past the grid
             put(((int64_t)(i0 + w) * d.py + (j0 + m)) * d.pz + kk, (nt0 + n) * 16 + r,
-                acc[m][n][e]);
+                acc[m][n][e] + bco[n]);
           }
           acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
