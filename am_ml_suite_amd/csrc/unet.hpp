@@ -26,13 +26,13 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // [ysplit, cout) to y2 (row stride cout - ysplit) — the data gradient of a concatenated input.
 // accum: add to the output instead of overwriting.
 //
-// Implicit GEMM on v_mfma_f32_16x16x32_bf16 (wave64): M = voxels (16 per tile), N = output
-// channels (16 per tile), K = (tap, input channel) in chunks of 32, each lane group of 16 lanes
+// Implicit GEMM on v_mfma_f32_16x16x32_bf16 (wave64): M = output channels (16 per tile), N =
+// voxels (16 per tile), K = (tap, input channel) in chunks of 32, each lane group of 16 lanes
 // taking 8 consecutive channels of one tap, so every A and B fragment is one 16-byte load. The
 // pairing of K between the A and B fragments is the same lane slot on both sides, which is all
 // the product needs.
 //   A fragment: lane l holds A[l % 16][8 (l / 16) .. + 7]; B: B[8 (l / 16) .. + 7][l % 16];
-//   D: lane l holds D[4 (l / 16) + e][l % 16], e = 0..3 (row = voxel, column = channel).
+//   D: lane l holds D[4 (l / 16) + e][l % 16], e = 0..3 (row = channel, column = voxel).
 // ---------------------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 union Frag8 {
@@ -41,51 +41,81 @@ union Frag8 {
 };
 
 // ---------------------------------------------------------------------------------------
-// Brick convolution (the operation and arguments above). A block computes a
-// 4 x 4 x 16 voxel brick (16 M tiles of 16 voxels along z; wave w takes the i-plane w, its four
-// j-rows) for NP output-channel tiles. Input channels go in groups of 16: the group's halo brick
-// (6 x 6 x 18 voxels) is staged in LDS as two 8-channel halves ([half][voxel][8], so the 8 lanes
-// of an LDS access read 128 contiguous bytes), and the 27 taps run from LDS, two taps per
-// 16x16x32 MFMA (K = 2 taps x 16 channels; lane group g takes tap 2c + g / 2, half g % 2). The
-// weight fragments of a channel tile for the group (14 chunks) sit in registers; the next tile's
-// are loaded while this one's MFMAs run.
+// Brick convolution (the operation and arguments above). A block computes a 4 x 4 x 16 voxel
+// brick (16 M tiles of 16 voxels along z) for one 16-channel output tile. Wave w takes i-planes
+// 2 (w / 2) + {0, 1} and j-rows 2 (w % 2) + {0, 1} of the brick, its M tile m being plane
+// 2 (w / 2) + m / 2, row 2 (w % 2) + m % 2, so a 2x2x2 pool window lies in two neighbouring
+// lanes. Input channels go in groups of 16: the group's halo brick (6 x 6 x 18 voxels) is
+// staged in LDS as two 8-channel halves ([half][voxel][8], so the 8 lanes of an LDS access read
+// 128 contiguous bytes), and the 27 taps run from LDS, two taps per 16x16x32 MFMA (K = 2 taps x
+// 16 channels; lane group g takes tap 2c + g / 2, half g % 2). The weight fragments of the
+// channel tile for the group (14 chunks) sit in registers. The weights are the MFMA's A operand
+// and the voxels its B operand, so lane (g, r) ends with output channels 4 g .. 4 g + 3 of
+// voxel r of each M tile: one 8-byte store per tile.
 // Grid: x = blocks walking the bricks (block x takes bricks x, x + gridDim.x, ...: the next
 // brick's halo is loaded while the current one's MFMAs run, and with a single input group per
-// brick the weight fragments are loaded once), y = groups of NP channel tiles, z = splits of
-// the input-channel groups (every split has at least one group). With one split the block
-// applies bias, accumulation and ReLU itself; with several, each writes FP32 partial sums
-// part[z][v][co] and k_conv_finish adds them in split order (a completion counter with
-// device-scope fences in this kernel measured twice as slow: the fences write back and
-// invalidate L2). up1: x1 is at half resolution
-// (dimensions d / 2) and is read as its nearest-neighbour upsampling, so the decoder's upsampled
-// input is never written.
+// brick the weight fragments are loaded once), y = output-channel tiles, z = splits of the
+// input-channel groups (every split has at least one group). With one split the block applies
+// bias, accumulation and ReLU itself (and the fused outputs below); with several, each writes
+// FP32 partial sums part[z][v][co] and k_conv_finish adds them in split order (a completion
+// counter with device-scope fences in this kernel measured twice as slow: the fences write back
+// and invalidate L2). up1: x1 is at half resolution (dimensions d / 2) and is read as its
+// nearest-neighbour upsampling, so the decoder's upsampled input is never written.
 // ---------------------------------------------------------------------------------------
 constexpr int kBrickHalo = 6 * 6 * 18;
 // the second channel half starts 64 bytes past a multiple of 128 B: the 8 lanes of a 128-byte
 // LDS store phase (4 voxels x 2 halves) then hit disjoint banks
 constexpr int kBrickHalf = kBrickHalo + 4;
 
-template <int NP>
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+
+// Outputs fused into an unsplit brick convolution's epilogue (null pointers: none). Both read
+// the layer's BF16 outputs as stored:
+//   pool: their 2x2x2 average pool (dimensions d / 2, all even; cout channels): the window's
+//         eight BF16 values summed in FP32 in (di, dj, dk) order, dk fastest, times 1/8, rounded
+//         to BF16 (the same sum, bit for bit, as k_conv_finish_pool's);
+//   wh, bh, yh: the pointwise head (16 -> 16 channels, W[co][ci] BF16, FP32 bias, no
+//         activation; cout = 16) on a 16x16x16 MFMA, written to yh.
+struct ConvFuse {
+  uint16_t* pool;
+  const uint16_t* wh;
+  const float* bh;
+  uint16_t* yh;
+};
+
+__device__ __forceinline__ float bf_lo(uint32_t u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t u) { return __uint_as_float(u & 0xFFFF0000u); }
+__device__ __forceinline__ uint32_t bf_pack(float a, float b) {
+  return (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16);
+}
+
 __global__ __launch_bounds__(256) void k_conv_brick(
     const uint16_t* __restrict__ x1, int c1, const uint16_t* __restrict__ x2, int c2,
     const uint16_t* __restrict__ W, const float* __restrict__ bias, uint16_t* __restrict__ y1,
     uint16_t* __restrict__ y2, int ysplit, int relu, int accum, VoxDims d, int cout,
-    float* __restrict__ part, int up1) {
+    float* __restrict__ part, int up1, ConvFuse fz) {
   __shared__ uint4 sx[2 * kBrickHalf];  // [half][voxel] 8 channels each
   const int py1 = up1 ? d.py / 2 : d.py, pz1 = up1 ? d.pz / 2 : d.pz;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, r = lane & 15;
+  const int wi = 2 * (w >> 1), wj = 2 * (w & 1);  // the wave's first plane and row in the brick
   const int nbk = (d.pz + 15) / 16, nbj = (d.py + 3) / 4;
   const int nbricks = ((d.px + 3) / 4) * nbj * nbk;
   const int cin = c1 + c2, ngr = cin / 16;
   const int gpk = (ngr + gridDim.z - 1) / gridDim.z;
   const int gbeg = blockIdx.z * gpk, gend = min(ngr, gbeg + gpk);
   const int per = gend - gbeg;  // input-channel groups per brick for this split
-  const int nt0 = blockIdx.y * NP;
+  const int co0 = blockIdx.y * 16 + 4 * g;  // the lane's four output channels
   // the block's output channels are fixed: their biases are read once (a read per output value
   // would wait on the next brick's halo loads, which share the load counter)
-  float bco[NP];
+  float bco[4], bh[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int n = 0; n < NP; ++n) bco[n] = bias ? bias[(nt0 + n) * 16 + r] : 0.f;
+  for (int e = 0; e < 4; ++e) bco[e] = bias ? bias[co0 + e] : 0.f;
+  s16x4 wh = {0, 0, 0, 0};  // head A fragment: W[r][4 g .. 4 g + 3]
+  if (fz.yh) {
+    wh = *(const s16x4*)(fz.wh + 16 * r + 4 * g);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) bh[e] = fz.bh[4 * g + e];
+  }
   // work items (brick, group), brick-major; block x takes bricks x, x + gridDim.x, ...
   const int my_bricks = blockIdx.x < nbricks ? (nbricks - 1 - blockIdx.x) / gridDim.x + 1 : 0;
   const int items = per > 0 ? my_bricks * per : 0;
@@ -118,9 +148,9 @@ __global__ __launch_bounds__(256) void k_conv_brick(
                  : uint4{0u, 0u, 0u, 0u};
     }
   };
-  // weight fragments of channel tile n (of this block) for input-channel group gr, 14 tap pairs
-  auto load_b = [&](int gr, int n, Frag8 (&bf)[14]) {
-    const int co = (nt0 + n) * 16 + r;
+  // weight fragments of the channel tile for input-channel group gr, 14 tap pairs
+  auto load_b = [&](int gr, Frag8 (&bf)[14]) {
+    const int co = blockIdx.y * 16 + r;
     const int ch = 16 * gr + 8 * (g & 1);
 #pragma unroll
     for (int c = 0; c < 14; ++c) {
@@ -129,20 +159,18 @@ __global__ __launch_bounds__(256) void k_conv_brick(
                          : uint4{0u, 0u, 0u, 0u};
     }
   };
-  f32x4 acc[4][NP];
+  f32x4 acc[4];
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
-#pragma unroll
-    for (int n = 0; n < NP; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int m = 0; m < 4; ++m) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
   uint4 hv[6];
-  Frag8 bA[14], bB[14];
+  Frag8 bw[14];
   if (items > 0) {
     load_halo(0, hv);
-    load_b(gbeg, 0, bA);  // with one group per brick the weights stay for every brick
+    load_b(gbeg, bw);  // with one group per brick the weights stay for every brick
   }
   for (int t = 0; t < items; ++t) {
     const int gr = gbeg + t % per;
-    if (per > 1 && t > 0) load_b(gr, 0, bA);
+    if (per > 1 && t > 0) load_b(gr, bw);
     __syncthreads();  // the previous item's fragment reads are done
 #pragma unroll
     for (int u = 0; u < 6; ++u) {
@@ -151,79 +179,128 @@ __global__ __launch_bounds__(256) void k_conv_brick(
     }
     __syncthreads();
     if (t + 1 < items) load_halo(t + 1, hv);  // in flight during this item's MFMAs
-    // A fragments of tap pair c for the four j-rows (LDS offset of the lane's voxel and tap)
+    // B fragments of tap pair c for the wave's four M tiles (LDS offset of the lane's voxel and
+    // tap)
     auto read_a = [&](int c, Frag8 (&a)[4]) {
       const int tap = 2 * c + (g >> 1);
       const int tt = tap < 27 ? tap : 26;  // the padding tap reads a valid slot (times 0)
       const int di = tt / 9, dj = (tt / 3) % 3, dk = tt % 3;
-      const int base = (g & 1) * kBrickHalf + ((w + di) * 6 + dj) * 18 + r + dk;
+      const int base = (g & 1) * kBrickHalf + ((wi + di) * 6 + wj + dj) * 18 + r + dk;
 #pragma unroll
-      for (int m = 0; m < 4; ++m) a[m].u = sx[base + m * 18];
+      for (int m = 0; m < 4; ++m) a[m].u = sx[base + ((m >> 1) * 6 + (m & 1)) * 18];
     };
+    // the next tap pair's four fragments are read while this pair's MFMAs run (the scheduling
+    // groups keep that order: four LDS reads, then four MFMAs)
+    Frag8 a0[4], a1[4];
+    read_a(0, a0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
 #pragma unroll
-    for (int n = 0; n < NP; ++n) {
-      Frag8(&bc)[14] = (n & 1) ? bB : bA;
-      if (n + 1 < NP) load_b(gr, n + 1, (n & 1) ? bA : bB);
-      // the next tap pair's four A fragments are read while this pair's MFMAs run
-      // (the scheduling groups keep that order: four LDS reads, then four MFMAs)
-      Frag8 a0[4], a1[4];
-      read_a(0, a0);
-      __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
-#pragma unroll
-      for (int c = 0; c < 14; ++c) {
-        Frag8(&cur)[4] = (c & 1) ? a1 : a0;
-        if (c + 1 < 14) {
-          read_a(c + 1, (c & 1) ? a0 : a1);
-          __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
-        }
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[m].v, bc[c].v, acc[m][n], 0, 0, 0);
-        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+    for (int c = 0; c < 14; ++c) {
+      Frag8(&cur)[4] = (c & 1) ? a1 : a0;
+      if (c + 1 < 14) {
+        read_a(c + 1, (c & 1) ? a0 : a1);
+        __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
       }
-    }
-    if ((t + 1) % per) continue;  // more groups of this brick follow
-    // the brick's last group: D (lane rows 4 g + e = voxel k0 + 4 g + e of row (i0 + w, j0 + m),
-    // column r) out, accumulators cleared for the next brick
-    int i0, j0, k0;
-    brick_of(t, i0, j0, k0);
-    auto put = [&](int64_t v, int co, float val) {
-      uint16_t* dst =
-          co < ysplit ? y1 + v * ysplit + co : y2 + v * (cout - ysplit) + (co - ysplit);
-      if (accum) val += bf2f(*dst);
-      if (relu) val = fmaxf(val, 0.f);
-      *dst = f2bf(val);
-    };
-    if (!part) {
 #pragma unroll
       for (int m = 0; m < 4; ++m)
+        acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[c].v, cur[m].v, acc[m], 0, 0, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+    }
+    if ((t + 1) % per) continue;  // more groups of this brick follow
+    // the brick's last group: lane (g, r) holds channels co0 .. co0 + 3 of voxel k0 + r of each
+    // M tile; accumulators cleared for the next brick
+    int i0, j0, k0;
+    brick_of(t, i0, j0, k0);
+    const int kk = k0 + r;
+    auto voxel = [&](int m, bool& ok) {
+      const int ip = i0 + wi + (m >> 1), jr = j0 + wj + (m & 1);
+      ok = ip < d.px && jr < d.py && kk < d.pz;  // past the grid: nothing stored
+      return ((int64_t)ip * d.py + jr) * d.pz + kk;
+    };
+    if (part) {  // split: this split's FP32 partial sums (k_conv_finish adds them)
 #pragma unroll
-        for (int n = 0; n < NP; ++n) {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int kk = k0 + 4 * g + e;
-            if (kk >= d.pz || i0 + w >= d.px || j0 + m >= d.py) continue;  // past the grid
-            put(((int64_t)(i0 + w) * d.py + (j0 + m)) * d.pz + kk, (nt0 + n) * 16 + r,
-                acc[m][n][e] + bco[n]);
-          }
-          acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
-        }
+      for (int m = 0; m < 4; ++m) {
+        bool ok;
+        const int64_t v = voxel(m, ok);
+        if (ok) *(f32x4*)(part + ((int64_t)blockIdx.z * d.nv + v) * cout + co0) = acc[m];
+        acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
       continue;
     }
-    // split: this split's FP32 partial sums (k_conv_finish adds them)
+    uint2 o[4];  // the stored BF16 outputs, per M tile
+    bool ok0 = false;
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+    for (int m = 0; m < 4; ++m) {
+      bool ok;
+      const int64_t v = voxel(m, ok);
+      if (m == 0) ok0 = ok;
+      uint16_t* dst =
+          co0 < ysplit ? y1 + v * ysplit + co0 : y2 + v * (cout - ysplit) + (co0 - ysplit);
+      float val[4];
 #pragma unroll
-      for (int n = 0; n < NP; ++n) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int kk = k0 + 4 * g + e;
-          if (kk >= d.pz || i0 + w >= d.px || j0 + m >= d.py) continue;
-          const int64_t v = ((int64_t)(i0 + w) * d.py + (j0 + m)) * d.pz + kk;
-          part[((int64_t)blockIdx.z * d.nv + v) * cout + (nt0 + n) * 16 + r] = acc[m][n][e];
-        }
-        acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int e = 0; e < 4; ++e) val[e] = acc[m][e] + bco[e];
+      if (accum && ok) {
+        const uint2 old = *(const uint2*)dst;
+        val[0] += bf_lo(old.x);
+        val[1] += bf_hi(old.x);
+        val[2] += bf_lo(old.y);
+        val[3] += bf_hi(old.y);
       }
+      if (relu) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) val[e] = fmaxf(val[e], 0.f);
+      }
+      o[m] = uint2{bf_pack(val[0], val[1]), bf_pack(val[2], val[3])};
+      if (ok) *(uint2*)dst = o[m];
+      acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    if (fz.pool) {
+      // window of the even lane r (k even) and lane r + 1 (k odd), summed in (di, dj, dk) order,
+      // dk fastest
+      uint2 nb[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {  // quad_perm [1, 0, 3, 2]: the partner lane's values
+        nb[m].x = (uint32_t)__builtin_amdgcn_mov_dpp((int)o[m].x, 0xB1, 0xF, 0xF, false);
+        nb[m].y = (uint32_t)__builtin_amdgcn_mov_dpp((int)o[m].y, 0xB1, 0xF, 0xF, false);
+      }
+      if (!(r & 1) && ok0) {
+        float sp[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          sp[0] += bf_lo(o[m].x);
+          sp[0] += bf_lo(nb[m].x);
+          sp[1] += bf_hi(o[m].x);
+          sp[1] += bf_hi(nb[m].x);
+          sp[2] += bf_lo(o[m].y);
+          sp[2] += bf_lo(nb[m].y);
+          sp[3] += bf_hi(o[m].y);
+          sp[3] += bf_hi(nb[m].y);
+        }
+        const int64_t pv = ((int64_t)((i0 + wi) >> 1) * (d.py >> 1) + ((j0 + wj) >> 1)) *
+                               (d.pz >> 1) + (kk >> 1);
+        *(uint2*)(fz.pool + pv * cout + co0) =
+            uint2{bf_pack(0.125f * sp[0], 0.125f * sp[1]), bf_pack(0.125f * sp[2], 0.125f * sp[3])};
+      }
+    }
+    if (fz.yh) {
+      // head: A = W (lane (g, r): row r, columns 4 g ..), B = the outputs (lane (g, r): channels
+      // 4 g .. of voxel r); D lane (g, r): head channels 4 g + e of voxel r
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        bool ok;
+        const int64_t v = voxel(m, ok);
+        union {
+          uint2 u;
+          s16x4 s;
+        } b;
+        b.u = o[m];
+        const f32x4 yv =
+            __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wh, b.s, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        if (ok)
+          *(uint2*)(fz.yh + v * 16 + 4 * g) =
+              uint2{bf_pack(yv[0] + bh[0], yv[1] + bh[1]), bf_pack(yv[2] + bh[2], yv[3] + bh[3])};
+      }
+    }
   }
 }
 
@@ -243,6 +320,35 @@ __global__ void k_conv_finish(const float* __restrict__ part, int ks, VoxDims d,
   if (accum) val += bf2f(*dst);
   if (relu) val = fmaxf(val, 0.f);
   *dst = f2bf(val);
+}
+
+// k_conv_finish for a layer whose 2x2x2 average pool is also wanted (ysplit = cout, no accum;
+// dimensions even): one thread per (pooled voxel, channel) finishes the window's eight outputs
+// as k_conv_finish does, stores them, and stores their pool (the BF16 values summed in FP32 in
+// (di, dj, dk) order, dk fastest, times 1/8, rounded to BF16).
+__global__ void k_conv_finish_pool(const float* __restrict__ part, int ks, VoxDims d, int cout,
+                                   const float* __restrict__ bias, uint16_t* __restrict__ y,
+                                   int relu, uint16_t* __restrict__ pool) {
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int pxc = d.px >> 1, pyc = d.py >> 1, pzc = d.pz >> 1;
+  if (q >= (int64_t)pxc * pyc * pzc * cout) return;
+  const int co = (int)(q % cout);
+  const int64_t pv = q / cout;
+  const int pk = (int)(pv % pzc), pj = (int)((pv / pzc) % pyc), pi = (int)(pv / ((int64_t)pzc * pyc));
+  float s = 0.f;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int64_t v = ((int64_t)(2 * pi + (a >> 2)) * d.py + (2 * pj + ((a >> 1) & 1))) * d.pz +
+                      (2 * pk + (a & 1));
+    float val = 0.f;
+    for (int z = 0; z < ks; ++z) val += part[((int64_t)z * d.nv + v) * cout + co];
+    if (bias) val += bias[co];
+    if (relu) val = fmaxf(val, 0.f);
+    const uint16_t o = f2bf(val);
+    y[v * cout + co] = o;
+    s += bf2f(o);
+  }
+  pool[pv * cout + co] = f2bf(0.125f * s);
 }
 
 // Pointwise 16 -> 16 channel layer (the network head and its data gradient), one thread per
@@ -554,36 +660,8 @@ __global__ void k_relu_bwd(uint16_t* __restrict__ dy, const uint16_t* __restrict
   if (q < n && !(bf2f(y[q]) > 0.f)) dy[q] = 0;
 }
 
-// 2x2x2 average pooling (fine dims d, coarse dims d/2), channels C: one thread per 8 channels of
-// a coarse voxel (16-byte loads and store).
-__global__ void k_pool(const uint16_t* __restrict__ x, VoxDims d, int C,
-                       uint16_t* __restrict__ y, VoxDims dc) {
-  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int C8 = C / 8;
-  if (q >= dc.nv * C8) return;
-  const int c = 8 * (int)(q % C8);
-  const int64_t v = q / C8;
-  int i, j, k;
-  vox_decode(v, dc, i, j, k);
-  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    const int64_t f = ((int64_t)(2 * i + (a >> 2)) * d.py + (2 * j + ((a >> 1) & 1))) * d.pz +
-                      (2 * k + (a & 1));
-    const uint4 u = *(const uint4*)(x + f * C + c);
-    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      s[2 * t] += bf2f((uint16_t)(w[t] & 0xFFFFu));
-      s[2 * t + 1] += bf2f((uint16_t)(w[t] >> 16));
-    }
-  }
-  uint32_t o[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t)
-    o[t] = (uint32_t)f2bf(0.125f * s[2 * t]) | ((uint32_t)f2bf(0.125f * s[2 * t + 1]) << 16);
-  *(uint4*)(y + v * C + c) = uint4{o[0], o[1], o[2], o[3]};
-}
+// Backward of the 2x2x2 average pool (fused into the forward convolutions' epilogues): each fine
+// voxel gets 1/8 of its coarse voxel's gradient.
 __global__ void k_pool_bwd(const uint16_t* __restrict__ dy, VoxDims dc, int C,
                            uint16_t* __restrict__ dx, VoxDims d, int accum) {
   const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;

@@ -204,12 +204,14 @@ inline void refresh_bf16(amt_unet* n) {
 
 // y = conv(x1 ++ x2) with the layer's weights; nt = output channels / 16. 3x3x3 layers run as
 // brick convolutions (k_conv_brick, split as brick_cfg says, partial sums in n->part added by
-// k_conv_finish); the
-// pointwise head (16 -> 16 channels, one input tensor, one output tensor) as k_pointwise16.
+// k_conv_finish); the pointwise head (16 -> 16 channels, one input tensor, one output tensor) as
+// k_pointwise16. fz: outputs fused into the layer (its pool, or the head applied to it), in the
+// brick epilogue when the launch is unsplit, else (the pool) in k_conv_finish_pool; the caller
+// passes a pool only with ysplit = cout and no accum, a head only with cout = 16.
 inline void conv_launch(amt_unet* n, int taps, int nt, const uint16_t* x1, int c1,
                         const uint16_t* x2, int c2, const uint16_t* W, const float* bias,
                         uint16_t* y1, uint16_t* y2, int ysplit, int relu, int accum,
-                        const VoxDims& d, int up1 = 0) {
+                        const VoxDims& d, int up1 = 0, amt::ConvFuse fz = amt::ConvFuse{}) {
   hipStream_t st = n->stream;
   const int cout = 16 * nt;
   if (taps == 27) {
@@ -218,16 +220,18 @@ inline void conv_launch(amt_unet* n, int taps, int nt, const uint16_t* x1, int c
     // two resident blocks per CU (the kernel's occupancy) walk the bricks, one brick at least
     const int64_t gx = std::max<int64_t>(1, std::min<int64_t>(c.bricks, 512 / (c.ny * c.ks)));
     const dim3 grid((unsigned)gx, (unsigned)c.ny, (unsigned)c.ks), blk(256);
-#define AMT_BRICK_CASE(NP)                                                                    \
-  if (c.np == NP)                                                                             \
-    hipLaunchKernelGGL((amt::k_conv_brick<NP>), grid, blk, 0, st, x1, c1, x2, c2, W, bias, y1, \
-                       y2, ysplit, relu, accum, d, cout, part, up1);
-    AMT_BRICK_CASE(1)
-#undef AMT_BRICK_CASE
-    if (part)
+    hipLaunchKernelGGL(amt::k_conv_brick, grid, blk, 0, st, x1, c1, x2, c2, W, bias, y1, y2,
+                       ysplit, relu, accum, d, cout, part, up1, part ? amt::ConvFuse{} : fz);
+    if (!part) return;
+    if (fz.pool) {
+      const int64_t cnt = (d.nv / 8) * cout;
+      hipLaunchKernelGGL(amt::k_conv_finish_pool, dim3((unsigned)ceil_div(cnt, 256)), dim3(256),
+                         0, st, part, c.ks, d, cout, bias, y1, relu, fz.pool);
+    } else {
       hipLaunchKernelGGL(amt::k_conv_finish, dim3((unsigned)ceil_div(d.nv * cout, 256)),
                          dim3(256), 0, st, part, c.ks, d, cout, bias, y1, y2, ysplit, relu,
                          accum);
+    }
     return;
   }
   // the pointwise head (16 -> 16) and its data gradient; the only 1-tap layer
@@ -238,21 +242,26 @@ inline void conv_launch(amt_unet* n, int taps, int nt, const uint16_t* x1, int c
                      x1, W, bias, y1, relu, accum, d.nv);
 }
 
-// up1: in1 is the coarse tensor whose upsampling the layer reads (not written out).
-inline void forward_layer(amt_unet* n, int li, int in1, int in2, int out, int up1 = 0) {
+// up1: in1 is the coarse tensor whose upsampling the layer reads (not written out). pool: the
+// tensor that receives the output's 2x2x2 average pool (-1: none). head: the layer after this
+// one is the pointwise head, applied in this layer's epilogue (its output written to OUT).
+inline void forward_layer(amt_unet* n, int li, int in1, int in2, int out, int up1 = 0,
+                          int pool = -1, bool head = false) {
   const auto& L = n->L[li];
   const VoxDims& d = n->d[n->act_l[out]];
+  amt::ConvFuse fz{};
+  if (pool >= 0) fz.pool = n->act[pool];
+  if (head) {
+    const auto& H = n->L[li + 1];
+    fz.wh = n->wb + H.w_off;
+    fz.bh = n->master + H.b_off;
+    fz.yh = n->act[amt_unet::OUT];
+  }
   conv_launch(n, L.taps, L.cout / 16, n->act[in1], L.c1, in2 >= 0 ? n->act[in2] : nullptr,
               L.c2, n->wb + L.w_off, n->master + L.b_off, n->act[out], nullptr, L.cout, L.relu,
-              0, d, up1);
+              0, d, up1, fz);
 }
 
-inline void pool(amt_unet* n, int in, int out) {
-  const VoxDims& dc = n->d[n->act_l[out]];
-  const int C = n->act_c[in];
-  hipLaunchKernelGGL(amt::k_pool, dim3(ceil_div(dc.nv * C / 8, 256)), dim3(256), 0, n->stream,
-                     n->act[in], n->d[n->act_l[in]], C, n->act[out], dc);
-}
 inline void upsample(amt_unet* n, int in, int out) {
   const VoxDims& d = n->d[n->act_l[out]];
   const int C = n->act_c[in];
@@ -267,14 +276,11 @@ inline void upsample(amt_unet* n, int in, int out) {
 void forward(amt_unet* n, bool infer) {
   using U = amt_unet;
   forward_layer(n, 0, U::X0, -1, U::A0a);
-  forward_layer(n, 1, U::A0a, -1, U::S0);
-  pool(n, U::S0, U::P1);
+  forward_layer(n, 1, U::A0a, -1, U::S0, 0, U::P1);
   forward_layer(n, 2, U::P1, -1, U::A1a);
-  forward_layer(n, 3, U::A1a, -1, U::S1);
-  pool(n, U::S1, U::P2);
+  forward_layer(n, 3, U::A1a, -1, U::S1, 0, U::P2);
   forward_layer(n, 4, U::P2, -1, U::A2a);
-  forward_layer(n, 5, U::A2a, -1, U::S2);
-  pool(n, U::S2, U::P3);
+  forward_layer(n, 5, U::A2a, -1, U::S2, 0, U::P3);
   forward_layer(n, 6, U::P3, -1, U::B3a);
   forward_layer(n, 7, U::B3a, -1, U::B3);
   if (infer) {
@@ -297,8 +303,7 @@ void forward(amt_unet* n, bool infer) {
     upsample(n, U::C1, U::U0);
     forward_layer(n, 12, U::U0, U::S0, U::C0a);
   }
-  forward_layer(n, 13, U::C0a, -1, U::C0);
-  forward_layer(n, 14, U::C0, -1, U::OUT);
+  forward_layer(n, 13, U::C0a, -1, U::C0, 0, -1, true);  // and the head (layer 14)
 }
 
 // Backward of one conv layer. dy = grad of its output (post-ReLU; the ReLU mask is applied

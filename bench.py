@@ -202,6 +202,25 @@ def run_ai(n_train=192, n_test=16, shape=(24, 16, 16), oc_iters=40, steps=1000, 
                 compliance_ratios_trained=[round(r["compliance_ratio"], 3) for r in res_t])
 
 
+def run_unet_conv(shape=(128, 64, 64), reps=20, rounds=3):
+    """The U-Net inference forward on its own (14 brick convolutions, 3 poolings, the head) on
+    the 128x64x64 grid, the 120x60x60 workload's node grid padded to multiples of 8: the best of
+    `rounds` HIP-event timings of `reps` forwards, and its convolution FLOP rate."""
+    import am_ml_suite_amd as A
+    net = A.UNet(*shape, seed=3)
+    n = shape[0] * shape[1] * shape[2]
+    net.infer(np.random.default_rng(0).standard_normal((n, 16)).astype(np.float32) * 0.1)
+    best_ms, best_tf = None, None
+    for _ in range(rounds):
+        ms, tf = net.benchmark(reps)
+        if best_ms is None or ms < best_ms:
+            best_ms, best_tf = ms, tf
+    log("  [unet conv] %dx%dx%d forward %.4f ms, %.1f TFLOP/s (%.1f%% of dense BF16 peak)"
+        % (shape + (best_ms, best_tf, 100 * best_tf / BF16_PEAK_TFLOPS)))
+    return dict(grid="%dx%dx%d" % shape, forward_ms=best_ms, conv_tflops=best_tf,
+                frac_bf16_peak=best_tf / BF16_PEAK_TFLOPS, forwards_timed=reps * rounds)
+
+
 def run_learned(shape=(96, 48, 48), train_steps=400, samples=64):
     """The learned preconditioner on 96x48x48 bracket-like problems (698,691 DOFs): residual
     samples of a Jacobi-PCG solve on seed 4, GPU training, then on the held-out seed-5 problem
@@ -454,6 +473,10 @@ def main():
         learned = dict(warm_start=run_learned_warm())
         log("learned: U-Net preconditioner on 96x48x48")
         learned["fcg_96x48x48"] = run_learned(train_steps=2000)
+    unet_conv = None
+    if "unet_conv" in paths:
+        log("unet_conv: U-Net inference forward")
+        unet_conv = run_unet_conv()
     ai_line = None
     if "ai" in paths:
         log("ai: density predictor")
@@ -479,6 +502,8 @@ def main():
             line["learned"] = learned
         if ai_line is not None:
             line["ai"] = ai_line
+        if unet_conv is not None:
+            line["unet_conv"] = unet_conv
         print(json.dumps(line), flush=True)
         return
 
@@ -570,6 +595,8 @@ def main():
         line["learned"] = learned
     if ai_line is not None:
         line["ai"] = ai_line
+    if unet_conv is not None:
+        line["unet_conv"] = unet_conv
     line.update(extra)
     print(json.dumps(line), flush=True)
 
