@@ -313,8 +313,15 @@ __global__ void k_conv_finish(const float* __restrict__ part, int ks, VoxDims d,
   if (q >= d.nv * cout) return;
   const int co = (int)(q % cout);
   const int64_t v = q / cout;
+  // up to 8 splits with every load issued at once, then any further splits (in split order)
+  float pz[8];
+#pragma unroll
+  for (int z = 0; z < 8; ++z) pz[z] = z < ks ? part[(int64_t)z * d.nv * cout + q] : 0.f;
   float val = 0.f;
-  for (int z = 0; z < ks; ++z) val += part[(int64_t)z * d.nv * cout + q];
+#pragma unroll
+  for (int z = 0; z < 8; ++z)
+    if (z < ks) val += pz[z];
+  for (int z = 8; z < ks; ++z) val += part[(int64_t)z * d.nv * cout + q];
   if (bias) val += bias[co];
   uint16_t* dst = co < ysplit ? y1 + v * ysplit + co : y2 + v * (cout - ysplit) + (co - ysplit);
   if (accum) val += bf2f(*dst);
@@ -335,17 +342,36 @@ __global__ void k_conv_finish_pool(const float* __restrict__ part, int ks, VoxDi
   const int co = (int)(q % cout);
   const int64_t pv = q / cout;
   const int pk = (int)(pv % pzc), pj = (int)((pv / pzc) % pyc), pi = (int)(pv / ((int64_t)pzc * pyc));
+  int64_t v[8];
+  float val[8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    v[a] = ((int64_t)(2 * pi + (a >> 2)) * d.py + (2 * pj + ((a >> 1) & 1))) * d.pz +
+           (2 * pk + (a & 1));
+    val[a] = 0.f;
+  }
+  // up to 8 splits with every load issued at once, then any further splits (in split order)
+  float pz[8][8];
+#pragma unroll
+  for (int z = 0; z < 8; ++z)
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+      pz[z][a] = z < ks ? part[((int64_t)z * d.nv + v[a]) * cout + co] : 0.f;
+#pragma unroll
+  for (int z = 0; z < 8; ++z)
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+      if (z < ks) val[a] += pz[z][a];
+  for (int z = 8; z < ks; ++z)
+#pragma unroll
+    for (int a = 0; a < 8; ++a) val[a] += part[((int64_t)z * d.nv + v[a]) * cout + co];
   float s = 0.f;
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
-    const int64_t v = ((int64_t)(2 * pi + (a >> 2)) * d.py + (2 * pj + ((a >> 1) & 1))) * d.pz +
-                      (2 * pk + (a & 1));
-    float val = 0.f;
-    for (int z = 0; z < ks; ++z) val += part[((int64_t)z * d.nv + v) * cout + co];
-    if (bias) val += bias[co];
-    if (relu) val = fmaxf(val, 0.f);
-    const uint16_t o = f2bf(val);
-    y[v * cout + co] = o;
+    if (bias) val[a] += bias[co];
+    if (relu) val[a] = fmaxf(val[a], 0.f);
+    const uint16_t o = f2bf(val[a]);
+    y[v[a] * cout + co] = o;
     s += bf2f(o);
   }
   pool[pv * cout + co] = f2bf(0.125f * s);
