@@ -28,7 +28,7 @@ EXPORTS = [
     "amt_unet_activation", "amt_unet_get_gain", "amt_unet_set_gain", "amt_attach_unet",
     "amt_batch_create", "amt_batch_destroy", "amt_batch_size", "amt_batch_set_loads",
     "amt_batch_set_density", "amt_batch_get_density", "amt_batch_get_displacement",
-    "amt_batch_run", "amt_unet_collect", "amt_unet_train",
+    "amt_batch_run", "amt_batch_operator_info", "amt_unet_collect", "amt_unet_train",
     "amt_simp_create", "amt_simp_destroy", "amt_simp_add_load_case", "amt_simp_set_density",
     "amt_simp_get_density", "amt_simp_step", "amt_simp_filter", "amt_simp_get_displacement",
     "amt_oc_update_noalloc", "amt_operator_info", "amt_operator_timings",
@@ -162,6 +162,7 @@ def load():
     L.amt_batch_get_density.argtypes = [P, C.c_int, _dp, C.c_int64]
     L.amt_batch_get_displacement.argtypes = [P, C.c_int, _dp, C.c_int64]
     L.amt_batch_run.argtypes = [P, C.c_int, C.c_void_p, C.POINTER(C.c_double)]
+    L.amt_batch_operator_info.argtypes = [P, C.POINTER(C.c_int), _dp, C.c_int]
     L.amt_unet_set_gain.argtypes = [P, C.c_double]
     _i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
     L.amt_simp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,

@@ -38,10 +38,16 @@ using namespace amt;
 namespace {
 
 constexpr int kBChunk = 16;        // PCG iterations per chunk graph (even: parity returns)
-constexpr int kBMgWorkers = 4;     // host threads of the multigrid batch (one HW queue each)
+constexpr int kBMgWorkers = 8;     // host threads of the multigrid batch
 constexpr int kBPersist = 2048;    // blocks of a persistent launch (8 per CU)
 constexpr int kBOcLevels = 6;
 constexpr int kBOcCand = (1 << kBOcLevels) - 1;
+// Matvec candidates of the lockstep solver: 0 the LDS gather tile (BK_TILE items); 1 + 3 s + l
+// the element-column streaming tile of shape kBShape[s] (threads along j x k) with kBPlanes[l]
+// node planes per tile. Chosen per batch by timing (amt_config.operator_tile = -1) or forced.
+constexpr int kBShapes = 3, kBPlaneCounts = 3, kBCands = 1 + kBShapes * kBPlaneCounts;
+constexpr int kBShapeTJ[kBShapes] = {16, 8, 4}, kBShapeTK[kBShapes] = {16, 32, 64};
+constexpr int kBPlanes[kBPlaneCounts] = {2, 4, 8};
 
 int bfail(int code, const std::string& msg) { return amt::set_global_error(code, msg); }
 
@@ -90,6 +96,19 @@ struct amt_batch {
          *compl_ = nullptr;
   // partials
   double *part_tile = nullptr, *part_dof = nullptr, *part_elem = nullptr, *part_node = nullptr;
+  // matvec candidates: the chosen one (-1 until the first run tunes it), the tuner's times,
+  // per candidate the items (count, owners, per-problem stream grids) and per problem the
+  // range of its partials (item0, nitem); the Hadamard blocks per Ke0 table
+  int op_cand = -1;
+  double op_us[kBCands] = {};
+  int nS[kBCands] = {};
+  int* ipS[kBCands] = {};
+  BStream* DS[kBCands] = {};
+  int* ar[kBCands] = {};
+  bool had_ok = false;
+  HadK<double>* HK64 = nullptr;
+  HadK<float>* HK32 = nullptr;
+  double* part_apply = nullptr;  // first-half partials p.q, the largest candidate's item count
   hipStream_t stream = nullptr;
   hipGraphExec_t g_pre = nullptr, g_start0 = nullptr, g_start1 = nullptr, g_chunk = nullptr,
                  g_post = nullptr;
@@ -175,20 +194,47 @@ void enqueue_start(amt_batch* b, int first) {
                      dim3(kBlock), 0, b->stream, (T*)b->p0, T(0), b->n_dof);
 }
 
+// Items of matvec candidate c.
+int cand_items(const amt_batch* b, int c) { return c == 0 ? b->n_items[BK_TILE] : b->nS[c]; }
+
+// First half of an iteration of parity par on matvec candidate c.
+template <typename T>
+void launch_apply(amt_batch* b, int c, int par) {
+  const int nd = b->n_items[BK_DOF], n = cand_items(b, c);
+  const T* s = std::is_same<T, float>::value ? (const T*)b->s32 : (const T*)b->s64;
+  T* pp[2] = {(T*)b->p0, (T*)b->p1};
+  if (c == 0) {
+    const T* Ke = std::is_same<T, float>::value ? (const T*)b->Ke32 : (const T*)b->Ke64;
+    hipLaunchKernelGGL((k_b_apply<T>), dim3(grid_of(n)), dim3(kBlock), 0, b->stream, par, b->D,
+                       b->ip[BK_TILE], n, (const T*)b->z, pp[par], pp[par ^ 1], (T*)b->q, s, Ke,
+                       b->fixcnt, b->pen, b->S, b->part_dof, nd, b->part_apply);
+    return;
+  }
+  const HadK<T>* HK = std::is_same<T, float>::value ? (const HadK<T>*)b->HK32
+                                                     : (const HadK<T>*)b->HK64;
+#define AMT_B_APPLY_S(TJ, TK)                                                                 \
+  hipLaunchKernelGGL((k_b_apply_s<T, TJ, TK>), dim3(grid_of(n)), dim3(kBlock), 0, b->stream, \
+                     par, b->D, b->DS[c], b->ipS[c], n, (const T*)b->z, pp[par], pp[par ^ 1], \
+                     (T*)b->q, s, HK, b->fixcnt, b->pen, b->S, b->part_dof, nd, b->part_apply)
+  switch ((c - 1) / kBPlaneCounts) {
+    case 0: AMT_B_APPLY_S(16, 16); break;
+    case 1: AMT_B_APPLY_S(8, 32); break;
+    default: AMT_B_APPLY_S(4, 64); break;
+  }
+#undef AMT_B_APPLY_S
+}
+
 template <typename T>
 void enqueue_chunk(amt_batch* b) {
-  const int nt = b->n_items[BK_TILE], nd = b->n_items[BK_DOF];
-  const T* s = std::is_same<T, float>::value ? (const T*)b->s32 : (const T*)b->s64;
-  const T* Ke = std::is_same<T, float>::value ? (const T*)b->Ke32 : (const T*)b->Ke64;
+  const int nd = b->n_items[BK_DOF], c = b->op_cand;
   T* pp[2] = {(T*)b->p0, (T*)b->p1};
   for (int t = 0; t < kBChunk; ++t) {
     const int par = t & 1;
-    hipLaunchKernelGGL((k_b_apply<T>), dim3(grid_of(nt)), dim3(kBlock), 0, b->stream, par, b->D,
-                       b->ip[BK_TILE], nt, (const T*)b->z, pp[par], pp[par ^ 1], (T*)b->q, s, Ke,
-                       b->fixcnt, b->pen, b->S, b->part_dof, nd, b->part_tile);
+    launch_apply<T>(b, c, par);
     hipLaunchKernelGGL((k_b_update<T>), dim3(grid_of(nd)), dim3(kBlock), 0, b->stream, par ^ 1,
                        b->D, b->ip[BK_DOF], nd, b->u, b->r, (T*)b->z, pp[par ^ 1],
-                       (const T*)b->q, b->dinv, b->S, b->part_tile, nt, b->part_dof);
+                       (const T*)b->q, b->dinv, b->S, b->part_apply, b->ar[c],
+                       cand_items(b, c), b->part_dof);
   }
   hipLaunchKernelGGL(k_b_check, dim3(1), dim3(kBlock), 0, b->stream, 0, b->D, b->np, b->S,
                      b->part_dof, nd, b->flags);
@@ -263,12 +309,54 @@ int read_flags(amt_batch* b, std::vector<BState>& f, int* active) {
   return AMT_OK;
 }
 
+// The matvec tuner: each candidate's first-half launch timed on this batch (one launch to warm
+// up, then the mean of three), every problem in a fresh state so that every tile runs; the
+// fastest is kept. The state slots it writes are rewritten by the next solve's start.
+template <typename T>
+int tune_apply(amt_batch* b) {
+  BState fresh;
+  std::memset(&fresh, 0, sizeof(fresh));
+  fresh.fresh = 1;
+  fresh.maxit = 1;
+  std::vector<BState> st(2 * (size_t)b->np, fresh);
+  B_HIP(hipMemcpyAsync(b->S, st.data(), sizeof(BState) * st.size(), hipMemcpyHostToDevice,
+                       b->stream));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  B_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return bfail(AMT_EHIP, "hipEventCreate failed");
+  }
+  int rc = AMT_OK, best = 0;
+  for (int c = 0; c < kBCands && rc == AMT_OK; ++c) {
+    b->op_us[c] = 0.0;
+    if (c > 0 && !b->had_ok) continue;
+    launch_apply<T>(b, c, 0);
+    (void)hipEventRecord(e0, b->stream);
+    for (int r = 0; r < 3; ++r) launch_apply<T>(b, c, 0);
+    (void)hipEventRecord(e1, b->stream);
+    float ms = 0.f;
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = bfail(AMT_EHIP, "matvec tuning failed");
+      break;
+    }
+    b->op_us[c] = 1e3 * ms / 3.0;
+    if (b->op_us[c] < b->op_us[best]) best = c;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc == AMT_OK) b->op_cand = best;
+  return rc;
+}
+
 template <typename T>
 int run_impl(amt_batch* b, int iterations, amt_iteration_result* hist, double* elapsed) {
   for (int i = 0; i < b->np; ++i)
     if (!b->loads_set[i])
       return bfail(AMT_EINVAL, "loads not set for problem " + std::to_string(i));
   if (!b->design_ready) B_CHECK(build_design(b));
+  if (b->op_cand < 0) B_CHECK(tune_apply<T>(b));  // before the chunk graph is captured
   B_CHECK(capture(b, &b->g_pre, [&] { enqueue_pre<T>(b); }));
   B_CHECK(capture(b, &b->g_start0, [&] { enqueue_start<T>(b, 1); }));
   B_CHECK(capture(b, &b->g_start1, [&] { enqueue_replace<T>(b); }));
@@ -415,6 +503,11 @@ int amt_batch_create(const amt_config* cfgs, int n, amt_batch** out) {
     *out = b;
     return AMT_OK;
   }
+  for (int i = 0; i < n; ++i)
+    if (cfgs[i].operator_tile < -1 || cfgs[i].operator_tile >= kBCands ||
+        cfgs[i].operator_tile != cfgs[0].operator_tile)
+      return bfail(AMT_EINVAL, "batch: operator_tile must be -1 (tuned) or a candidate 0.." +
+                                   std::to_string(kBCands - 1) + ", the same for all problems");
   amt_batch* b = new amt_batch();
   b->np = n;
   b->mixed = cfgs[0].precision == AMT_PREC_MIXED_FP32;
@@ -498,6 +591,42 @@ int amt_batch_create(const amt_config* cfgs, int n, amt_batch** out) {
     delete b;
     return bfail(AMT_EINVAL, "batch too large");
   }
+  // matvec candidates: the Hadamard blocks of every Ke0 table (the streaming candidates need
+  // them), then per candidate each problem's items and the range of its partials
+  std::vector<HadK<double>> hk64(ke_key.size() / 3);
+  std::vector<HadK<float>> hk32(ke_key.size() / 3);
+  b->had_ok = true;
+  for (size_t t = 0; t < hk64.size(); ++t)
+    b->had_ok = b->had_ok && hadamard_blocks<double>(&ke64[576 * t], hk64[t]) &&
+                hadamard_blocks<float>(&ke64[576 * t], hk32[t]);
+  if (cfgs[0].operator_tile > 0 && !b->had_ok) {
+    delete b;
+    return bfail(AMT_ENOTSUP, "batch: the streaming operator needs an isotropic Ke0");
+  }
+  std::vector<int> ipc[kBCands], arc[kBCands];
+  std::vector<BStream> dsc[kBCands];
+  int max_items = b->n_items[BK_TILE];
+  for (int i = 0; i < n; ++i)
+    arc[0].insert(arc[0].end(), {b->hp[i].item0[BK_TILE], b->hp[i].nitem[BK_TILE]});
+  for (int c = 1; c < kBCands; ++c) {
+    const int sh = (c - 1) / kBPlaneCounts, L = kBPlanes[(c - 1) % kBPlaneCounts];
+    for (int i = 0; i < n; ++i) {
+      BStream B;
+      B.sg = make_stream_grid(b->hp[i].g, kBShapeTJ[sh], kBShapeTK[sh], L);
+      B.item0 = b->nS[c];
+      B.nitem = B.sg.n_tiles;
+      if ((int64_t)b->nS[c] + B.nitem > (int64_t)INT32_MAX / 64) {
+        delete b;
+        return bfail(AMT_EINVAL, "batch too large");
+      }
+      b->nS[c] += B.nitem;
+      ipc[c].insert(ipc[c].end(), (size_t)B.nitem, i);
+      dsc[c].push_back(B);
+      arc[c].insert(arc[c].end(), {B.item0, B.nitem});
+    }
+    max_items = std::max(max_items, b->nS[c]);
+  }
+  b->op_cand = cfgs[0].operator_tile;
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     delete b;
     return bfail(AMT_EHIP, "stream creation failed");
@@ -548,6 +677,16 @@ int amt_batch_create(const amt_config* cfgs, int n, amt_batch** out) {
   A(balloc(b, &b->part_dof, 2 * (size_t)b->n_items[BK_DOF]));
   A(balloc(b, &b->part_elem, (size_t)kBOcCand * b->n_items[BK_ELEM]));
   A(balloc(b, &b->part_node, b->n_items[BK_NODE]));
+  A(balloc(b, &b->part_apply, (size_t)max_items));
+  A(balloc(b, &b->HK64, hk64.size()));
+  A(balloc(b, &b->HK32, hk32.size()));
+  for (int c = 0; c < kBCands; ++c) {
+    A(balloc(b, &b->ar[c], arc[c].size()));
+    if (c > 0) {
+      A(balloc(b, &b->ipS[c], ipc[c].size()));
+      A(balloc(b, &b->DS[c], dsc[c].size()));
+    }
+  }
   if (rc != AMT_OK) return bail(rc);
   auto up = [&](void* dst, const void* src, size_t bytes) {
     if (rc == AMT_OK && bytes &&
@@ -560,6 +699,15 @@ int amt_batch_create(const amt_config* cfgs, int n, amt_batch** out) {
   up(b->Ke32, ke32.data(), sizeof(float) * ke32.size());
   up(b->st, st_all.data(), sizeof(StencilEntry) * st_all.size());
   up(b->sw, sw_all.data(), sizeof(double) * sw_all.size());
+  up(b->HK64, hk64.data(), sizeof(HadK<double>) * hk64.size());
+  up(b->HK32, hk32.data(), sizeof(HadK<float>) * hk32.size());
+  for (int c = 0; c < kBCands; ++c) {
+    up(b->ar[c], arc[c].data(), sizeof(int) * arc[c].size());
+    if (c > 0) {
+      up(b->ipS[c], ipc[c].data(), sizeof(int) * ipc[c].size());
+      up(b->DS[c], dsc[c].data(), sizeof(BStream) * dsc[c].size());
+    }
+  }
   // initial density: each problem's volume fraction (topology_optimizer.cpp:403-405)
   {
     std::vector<double> rho(ne);
@@ -670,6 +818,15 @@ int amt_batch_get_displacement(amt_batch* b, int i, double* u, int64_t n_dof) {
   if (!u || n_dof != 3 * P.g.n_nodes) return bfail(AMT_EINVAL, "displacement buffer size mismatch");
   B_HIP(hipStreamSynchronize(b->stream));
   B_HIP(hipMemcpy(u, b->u + 3 * P.n_off, sizeof(double) * n_dof, hipMemcpyDeviceToHost));
+  return AMT_OK;
+}
+
+int amt_batch_operator_info(amt_batch* b, int* candidate, double* us, int n) {
+  if (!b) return bfail(AMT_EINVAL, "bad argument");
+  if (!b->mg.empty()) return bfail(AMT_ENOTSUP, "a multigrid batch has no lockstep matvec");
+  if (candidate) *candidate = b->op_cand;
+  if (us)
+    for (int c = 0; c < n && c < kBCands; ++c) us[c] = b->op_us[c];
   return AMT_OK;
 }
 

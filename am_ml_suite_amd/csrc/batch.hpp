@@ -16,6 +16,7 @@
 #pragma once
 
 #include "device.hpp"
+#include "stream_op.hpp"
 
 namespace amt {
 
@@ -330,22 +331,93 @@ __global__ __launch_bounds__(kBlock) void k_b_apply(
   }
 }
 
+// A problem's items of one element-column streaming operator candidate (stream_op.hpp): its
+// stream grid (tile shape and plane count of the candidate) and the first of its tiles in the
+// candidate's item numbering.
+struct BStream {
+  StreamGrid sg;
+  int item0, nitem;
+};
+
+// First half of an iteration on the element-column streaming operator (tile TJ x TK threads,
+// each item one stream tile of a problem): the state bookkeeping, outputs and partials of
+// k_b_apply, with q = K p from the Hadamard block form HK[P.ke].
+template <typename T, int TJ, int TK>
+__global__ __launch_bounds__(kBlock) void k_b_apply_s(
+    int par, const BProb* __restrict__ D, const BStream* __restrict__ DS,
+    const int* __restrict__ ip, int n, const T* __restrict__ z, const T* __restrict__ pold,
+    T* __restrict__ pnew, T* __restrict__ q, const T* __restrict__ s,
+    const HadK<T>* __restrict__ HK, const uint8_t* __restrict__ fixcnt,
+    const double* __restrict__ pen, BState* S, const double* __restrict__ upart, int ustride,
+    double* __restrict__ apart) {
+  __shared__ StreamSmem<T, TJ, TK> sm;
+  struct Raw {
+    T a, b;
+  };
+  struct Own {
+    T f[3];
+  };
+  AMT_ITEMS(w, n) {
+    const int pi = ip[w];
+    const BProb P = D[pi];
+    const BStream BS = DS[pi];
+    const BState cur = S[2 * pi + par];
+    double beta_d;
+    const BState nx =
+        b_step_state(cur, par, upart + P.item0[BK_DOF], P.nitem[BK_DOF], ustride, beta_d);
+    if (w == BS.item0 && threadIdx.x == 0) S[2 * pi + (par ^ 1)] = nx;
+    if (nx.done) continue;  // uniform over the block
+    const int64_t off = 3 * P.n_off;
+    const T beta = (T)beta_d, pp = (T)pen[pi];
+    const T* zb = z + off;
+    const T* pb = pold + off;
+    const uint8_t* fb = fixcnt + off;
+    T* pn = pnew + off;
+    T* qb = q + off;
+    const HadK<T> K = HK[P.ke];
+    double acc[1] = {0.0};
+    stream_tile_at<T, TJ, TK>(
+        sm, P.g, BS.sg, K, s + P.e_off, w - BS.item0,
+        [&](int64_t d) { return Raw{zb[d], pb[d]}; },
+        [&](const Raw& v) { return fma(beta, v.b, v.a); },
+        [&](int64_t nd) {
+          Own o = {{T(0), T(0), T(0)}};
+          if (nd >= 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) o.f[c] = (T)fb[3 * nd + c];
+          }
+          return o;
+        },
+        [&](int64_t nd, T (&yv)[3], const T (&pv)[3], const Own& o) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const T qv = fma(o.f[c] * pp, pv[c], yv[c]);
+            pn[3 * nd + c] = pv[c];
+            qb[3 * nd + c] = qv;
+            acc[0] = fma((double)pv[c], (double)qv, acc[0]);
+          }
+        });
+    store_partial_at<1>(acc, apart, n, w);  // its barriers also end this item's LDS use
+  }
+}
+
 // Second half: alpha = rz / p.q; u += alpha p; r -= alpha q (FP64); z = (T)(D^-1 r); DOF-item
-// partials |r|^2, r.z. S[2p + par] is the state the first half wrote.
+// partials |r|^2, r.z. S[2p + par] is the state the first half wrote; the first half's partials
+// of problem p are apart[ar[2p] .. ar[2p] + ar[2p + 1]) (stride astride).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_update(
     int par, const BProb* __restrict__ D, const int* __restrict__ ip, int n,
     double* __restrict__ u, double* __restrict__ r, T* __restrict__ z,
     const T* __restrict__ p, const T* __restrict__ q, const double* __restrict__ dinv,
-    const BState* __restrict__ S, const double* __restrict__ apart, int astride,
-    double* __restrict__ upart) {
+    const BState* __restrict__ S, const double* __restrict__ apart, const int* __restrict__ ar,
+    int astride, double* __restrict__ upart) {
   AMT_ITEMS(w, n) {
     const int pi = ip[w];
     const BProb P = D[pi];
     const BState st = S[2 * pi + par];
     if (st.done) continue;
     double pq[1];
-    reduce_partials<1>(apart + P.item0[BK_TILE], P.nitem[BK_TILE], astride, pq);
+    reduce_partials<1>(apart + ar[2 * pi], ar[2 * pi + 1], astride, pq);
     const double a = st.rz / pq[0];
     const int64_t nd = 3 * P.g.n_nodes, off = 3 * P.n_off;
     const int64_t base = (int64_t)(w - P.item0[BK_DOF]) * kBDofItem;

@@ -210,30 +210,24 @@ struct StreamSmem {
   T slot[12][TJ * TK];                         // [(oy + 2 oz) * 3 + c][thread]
 };
 
-// Runs one tile. ld(q) issues the loads of DOF q's raw inputs and returns them; val(raw)
-// forms the operand value from them (the caller's transform, e.g. p = z + beta p_old), applied
-// when the value is stored to LDS one step later. pre(n) issues the owned node n's epilogue
-// loads at the start of its step; fin(n, y, xo, pre) gets y = K x on the node's three rows
-// (without penalty) and xo = x at the node. All threads of the block call it (barriers);
-// has = false runs no planes.
+// Runs tile t of the grid (t outside [0, n_tiles): nothing). ld(q) issues the loads of DOF q's
+// raw inputs and returns them; val(raw) forms the operand value from them (the caller's
+// transform, e.g. p = z + beta p_old), applied when the value is stored to LDS one step later.
+// pre(n) issues the owned node n's epilogue loads at the start of its step; fin(n, y, xo, pre)
+// gets y = K x on the node's three rows (without penalty) and xo = x at the node. All threads
+// of the block call it with the same t (barriers).
 template <typename T, int TJ, int TK, typename LD, typename VAL, typename PRE, typename FIN>
-__device__ __forceinline__ void stream_tile(StreamSmem<T, TJ, TK>& sm, const GridDims& g,
-                                            const StreamGrid& sg, const HadK<T>& K,
-                                            const T* __restrict__ s, bool has, LD ld, VAL val,
-                                            PRE pre, FIN fin) {
+__device__ __forceinline__ void stream_tile_at(StreamSmem<T, TJ, TK>& sm, const GridDims& g,
+                                               const StreamGrid& sg, const HadK<T>& K,
+                                               const T* __restrict__ s, int t, LD ld, VAL val,
+                                               PRE pre, FIN fin) {
   static_assert(TJ * TK == kBlock, "one element column per thread");
   constexpr int RS = kPlaneRow<T, TK>;
-  int i0 = 0, j0 = 0, k0 = 0;
-  if (has) {
-    const int b = blockIdx.x;
-    const int t = (b & 7) * sg.per_xcd + (b >> 3);
-    has = t < sg.n_tiles;
-    const int is = t % sg.ni, jk = t / sg.ni;
-    i0 = is * sg.L;
-    j0 = (jk / sg.nk) * (TJ - 1);
-    k0 = (jk % sg.nk) * (TK - 1);
-  }
-  if (!has) return;  // block-uniform: the whole block skips every barrier below
+  if (t < 0 || t >= sg.n_tiles) return;  // block-uniform: the whole block skips every barrier
+  const int is = t % sg.ni, jk = t / sg.ni;
+  const int i0 = is * sg.L;
+  const int j0 = (jk / sg.nk) * (TJ - 1);
+  const int k0 = (jk % sg.nk) * (TK - 1);
   const int tid = threadIdx.x, tj = tid / TK, tk = tid % TK;
   const int j = j0 - 1 + tj, k = k0 - 1 + tk;
   // the extra node of the plane: row TJ (TK + 1 nodes) and column TK (TJ nodes)
@@ -328,6 +322,18 @@ __device__ __forceinline__ void stream_tile(StreamSmem<T, TJ, TK>& sm, const Gri
       fin(n, y, xo, pv);
     }
   }
+}
+
+// The tile of this block: block b runs on XCD b % 8 and takes tile (b % 8) * per_xcd + b / 8.
+// has = false runs no planes.
+template <typename T, int TJ, int TK, typename LD, typename VAL, typename PRE, typename FIN>
+__device__ __forceinline__ void stream_tile(StreamSmem<T, TJ, TK>& sm, const GridDims& g,
+                                            const StreamGrid& sg, const HadK<T>& K,
+                                            const T* __restrict__ s, bool has, LD ld, VAL val,
+                                            PRE pre, FIN fin) {
+  const int b = blockIdx.x;
+  stream_tile_at(sm, g, sg, K, s, has ? (b & 7) * sg.per_xcd + (b >> 3) : -1, ld, val, pre,
+                 fin);
 }
 
 // Launch shapes the streaming kernels are instantiated for (threads along j x along k).

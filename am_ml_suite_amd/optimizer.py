@@ -497,7 +497,7 @@ class BatchOptimizer:
       - Jacobi-PCG: all problems in lockstep, every phase one persistent launch over all of
         them, the OC step replayed from hipGraphs;
       - preconditioner="multigrid": one single-problem handle and stream per problem, run by
-        four host threads so the problems' launches overlap on the device."""
+        eight host threads so the problems' launches overlap on the device."""
 
     def __init__(self, problems):
         self._L = _ffi.load()
@@ -559,6 +559,27 @@ class BatchOptimizer:
                                           C.byref(el)))
         return ([[_convert(hist[it * self.n + t]) for t in range(self.n)]
                  for it in range(int(iterations))], el.value)
+
+    # lockstep matvec candidates (amt_batch_operator_info): 0 the gather tile, 1 + 3 s + l the
+    # streaming tile of shape BATCH_SHAPES[s] with BATCH_PLANES[l] node planes
+    BATCH_SHAPES = ((16, 16), (8, 32), (4, 64))
+    BATCH_PLANES = (2, 4, 8)
+
+    def operator_info(self):
+        """(chosen matvec candidate, {candidate: measured launch time in us}) of a lockstep
+        batch; the candidate is -1 before the first run."""
+        c = C.c_int()
+        us = np.zeros(16)
+        self._check(self._L.amt_batch_operator_info(self._h, C.byref(c), us, us.size))
+        return c.value, {i: float(t) for i, t in enumerate(us) if t > 0}
+
+    @classmethod
+    def operator_shape(cls, cand):
+        """(tile description, node planes) of lockstep matvec candidate cand."""
+        if cand == 0:
+            return "4x4x16 node gather tile", 0
+        s, l = divmod(cand - 1, len(cls.BATCH_PLANES))
+        return "%dx%d streaming tile" % cls.BATCH_SHAPES[s], cls.BATCH_PLANES[l]
 
 
 # notebook binding names (cell 23 lines 9-12)

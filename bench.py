@@ -365,11 +365,20 @@ def run_batch(warmup, steps, solver="jacobi"):
     n_dof = sum(p.n_dofs for p in ps)
     flat = [r for row in hist for r in row]
     log("  [batch] %d OC iterations of %d problems in %.3fs" % (steps, len(ps), el))
+    matvec = None
+    if solver == "jacobi":  # the lockstep matvec the tuner picked, with every candidate's time
+        cand, us = B.operator_info()
+        tile, planes = A.BatchOptimizer.operator_shape(cand)
+        matvec = dict(candidate=cand, tile=tile, planes=planes,
+                      tuner_us={str(c): t for c, t in sorted(us.items())})
+        log("  [batch] matvec: candidate %d (%s, %d planes), %.1f us; gather tile %.1f us"
+            % (cand, tile, planes, us.get(cand, 0.0), us.get(0, 0.0)))
     return dict(workload="batch_problems_n64_seed3", n_problems=len(ps), n_dofs_total=n_dof,
+                matvec=matvec,
                 n_elements_total=sum(p.n_elements for p in ps),
                 solver=("lockstep persistent launches, Jacobi-PCG, fp32 matvec + fp64"
                         if solver == "jacobi" else
-                        "per-problem multigrid FCG (fp32 V-cycle, fp64 CG), 4 host threads"),
+                        "per-problem multigrid FCG (fp32 V-cycle, fp64 CG), 8 host threads"),
                 steps=steps, warmup=warmup, time_s=el, oc_iter_per_s=steps / el,
                 problem_oc_iter_per_s=steps * len(ps) / el, dof_oc_iter_per_s=n_dof * steps / el,
                 max_pcg_iters=max(r.solver_iterations for r in flat),

@@ -76,7 +76,10 @@ typedef struct amt_config {
                             element-column streaming tile of shape s (0: 16x16, 1: 8x32, 2: 4x64
                             threads along j x k) with 4, 8 or 16 (l = 0, 1, 2) node planes per
                             block; 10 + 2 s + l the same shapes with 2 or 3 (l = 0, 1) planes.
-                            All streaming shapes give the same operator values. */
+                            All streaming shapes give the same operator values. In a lockstep
+                            batch (amt_batch_create): the PCG matvec, -1 tuned at the first
+                            run, 0 the gather tile, 1 + 3 s + l shape s with 2, 4 or 8 planes
+                            (amt_batch_operator_info); the same value for every problem. */
   int mg_fused;          /* multigrid V-cycle on the levels of at most 16,384 nodes: 1 (default)
                             two brick-tiled launches per level (restriction + pre-smoothing,
                             prolongation + post-smoothing) and one for the dense level; 0 one
@@ -269,6 +272,13 @@ int amt_batch_get_displacement(amt_batch* b, int i, double* u, int64_t n_dof);
  * solver_iterations counts PCG iterations, outer_iterations the residual replacements,
  * and the phase times are the batch's. elapsed_s (optional): wall time of the call. */
 int amt_batch_run(amt_batch* b, int iterations, amt_iteration_result* hist, double* elapsed_s);
+/* The lockstep PCG matvec: *candidate (0 the LDS gather tile; 1 + 3 s + l the element-column
+ * streaming tile of shape s (0: 16x16, 1: 8x32, 2: 4x64 threads along j x k) with 2, 4 or 8
+ * (l = 0, 1, 2) node planes), chosen at the first amt_batch_run as the fastest of the launches
+ * timed there unless amt_config.operator_tile forced it (-1 before); us[c] (up to n entries,
+ * optional) the measured launch time of candidate c in microseconds, 0 where it did not run.
+ * A multigrid batch has no lockstep matvec (AMT_ENOTSUP). No counterpart in the reference. */
+int amt_batch_operator_info(amt_batch* b, int* candidate, double* us, int n);
 
 /* ---- The Python SIMPOptimizer flavour (src/am/numerical/topopt.py:84-411) ----
  * The suite's second SIMP loop, on the same matrix-free operator. It differs from

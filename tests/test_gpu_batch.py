@@ -67,16 +67,66 @@ def test_batch_mixed_matches_single_problem_path():
             assert hist[it][i].solver_converged and hist[it][i].true_rel_residual <= 1e-8
 
 
-def test_batch_problem_order_and_isolation():
+@pytest.mark.parametrize("cand", [0, 5])
+def test_batch_problem_order_and_isolation(cand):
     """A problem's results do not depend on its neighbours: the same problem alone and in the
-    middle of a batch gives bit-identical compliance and density (FP64 mode)."""
+    middle of a batch gives bit-identical compliance and density (FP64 mode), on the gather-tile
+    matvec (0) and on a streaming one (5: 8x32 threads, 4 planes)."""
     probs = _small_set()
-    alone = _batch([probs[2]], precision="fp64")
+    alone = _batch([probs[2]], precision="fp64", operator_tile=cand)
     h1, _ = alone.run(3)
-    B = _batch(probs, precision="fp64")
+    B = _batch(probs, precision="fp64", operator_tile=cand)
     h2, _ = B.run(3)
+    assert alone.operator_info()[0] == B.operator_info()[0] == cand
     assert [h1[it][0].compliance for it in range(3)] == [h2[it][2].compliance for it in range(3)]
     assert np.array_equal(alone.get_density(0), B.get_density(2))
+
+
+@pytest.mark.parametrize("cand", [1, 5, 9])
+def test_batch_streaming_matvec_matches_oracle(cand):
+    """FP64 batch on a streaming-tile matvec (1: 16x16 threads, 2 planes; 5: 8x32, 4; 9: 4x64,
+    8), problems smaller than one tile along some axis: every problem's per-iteration
+    compliance within 1e-8 relative of the oracle over 4 OC iterations, density max-abs
+    <= 1e-4, every solve converged on the FP64 true residual."""
+    probs = _small_set()
+    B = _batch(probs, precision="fp64", warm_start=False, operator_tile=cand)
+    hist, _ = B.run(4)
+    assert B.operator_info()[0] == cand
+    for i, p in enumerate(probs):
+        pr = p.params
+        ocfg = O.make_config(p.nx, p.ny, p.nz, max_iterations=4, pcg_tol=1e-8,
+                             pcg_maxit=100000, ref_compat=0,
+                             volume_fraction=pr["volume_fraction"],
+                             filter_radius=pr["filter_radius"], move_limit=pr["move_limit"],
+                             rho_min=pr["rho_min"], E_min=pr["E_min"])
+        rho_ref, ref, _, _ = O.run(ocfg, p.rho0, p.F, p.fixed)
+        c = np.array([hist[it][i].compliance for it in range(4)])
+        cr = np.array([h["compliance"] for h in ref])
+        assert np.abs(c / cr - 1).max() <= 1e-8, (p.name, c, cr)
+        assert np.abs(B.get_density(i) - rho_ref).max() <= 1e-4, p.name
+        assert all(hist[it][i].solver_converged for it in range(4))
+        assert all(hist[it][i].true_rel_residual <= 1e-8 for it in range(4))
+
+
+def test_batch_matvec_tuner():
+    """Mixed batch with the tuned matvec: every candidate is timed at the first run, the fastest
+    is kept, and the solves converge as with a forced candidate (compliance within 1e-6 of the
+    gather-tile run)."""
+    probs = A.batch_problems(n=8, seed=5, lo=12, hi=40)
+    B = _batch(probs, precision="mixed_fp32", warm_start=True)
+    assert B.operator_info()[0] == -1
+    hist, _ = B.run(2)
+    cand, us = B.operator_info()
+    print("tuned matvec:", cand, A.BatchOptimizer.operator_shape(cand),
+          {c: round(t, 1) for c, t in us.items()})
+    assert sorted(us) == list(range(10))
+    assert us[cand] == min(us.values())
+    G = _batch(probs, precision="mixed_fp32", warm_start=True, operator_tile=0)
+    hg, _ = G.run(2)
+    for i in range(len(probs)):
+        for it in range(2):
+            assert hist[it][i].solver_converged and hist[it][i].true_rel_residual <= 1e-8
+            assert hist[it][i].compliance == pytest.approx(hg[it][i].compliance, rel=1e-6)
 
 
 def test_batch_generator_set_runs():
@@ -110,7 +160,7 @@ def test_batch_errors():
 
 
 def test_multigrid_batch_matches_single_problem_runs():
-    """Multigrid mode (one handle and stream per problem, four host threads): every problem's
+    """Multigrid mode (one handle and stream per problem, eight host threads): every problem's
     compliance history and density are bitwise those of the same problem run alone, whatever
     order the threads take the problems in."""
     ps = A.batch_problems(n=6, seed=8, lo=10, hi=22)
