@@ -50,7 +50,7 @@ using namespace amt;
 
 namespace {
 
-constexpr int kOcLevels = 6;  // bisection levels evaluated per OC sweep launch
+constexpr int kOcLevels = 3;  // bisection levels evaluated per OC sweep launch
 constexpr int kOcBlocks = 512;  // OC sweep grid (partials per candidate)
 constexpr int kOcCand = (1 << kOcLevels) - 1;
 constexpr int kMaxReduceBlocks = 1024;
@@ -2185,7 +2185,7 @@ int amt_profile_read(amt_problem* p, int64_t* launches, double* total_ms) {
 // device address that hipGetSymbolAddress returns, looked up once per device and kept.
 namespace {
 constexpr int kNaStride = kOcBlocks;                           // partials per value
-constexpr size_t kNaSweep = (size_t)kOcCand * kNaStride;       // 63 candidate volumes (>= 4
+constexpr size_t kNaSweep = (size_t)kOcCand * kNaStride;       // candidate volumes (>= 4
                                                                // bound values)
 constexpr size_t kNaFinal = 2 * (size_t)kNaStride;             // max |drho|, volume (count)
 constexpr size_t kNaState = kNaSweep + kNaFinal;               // OcState offset, in doubles
