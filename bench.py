@@ -495,11 +495,12 @@ def main():
         log("simp_py: the Python SIMPOptimizer flavour")
         simp_py = run_simp_py(5, not args.no_cpu_baseline)
     batch = None
-    if "batch" in paths:
+    if "batch" in paths or "batch_lockstep" in paths:
         log("batch: 64 generated problems in lockstep")
         batch = run_batch(1, 3)
-        log("batch: the same 64 problems, multigrid per problem")
-        batch["multigrid"] = run_batch(1, 3, "multigrid")
+        if "batch" in paths:  # batch_lockstep: the lockstep engine alone (one host thread)
+            log("batch: the same 64 problems, multigrid per problem")
+            batch["multigrid"] = run_batch(1, 3, "multigrid")
 
     if not out:  # only the auxiliary workloads were asked for
         line = {"metric": "auxiliary workloads", "batch": batch}
