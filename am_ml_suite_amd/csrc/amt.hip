@@ -8,6 +8,7 @@
 //   k_dot (compliance), k_sens                         ("sensitivity")
 //   k_filter, k_oc_sweep x ~16, k_oc_final             ("update")
 // and one small device->host copy of scalars ends it.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -286,8 +287,7 @@ void dot_dev(amt_problem* p, const double* a, const double* b, double* out);
 int bank_push(amt_problem* p, const double* r);
 int learned_warm_start(amt_problem* p);
 int residual64(amt_problem* p, const double* x, double* rout, double* rr_host);
-void prof_begin(amt_problem* p, hipEvent_t* a);
-void prof_end(amt_problem* p, hipEvent_t a);
+void prof_pick(amt_problem* p, hipEvent_t* e0, hipEvent_t* e1);
 void prof_harvest(amt_problem* p);
 
 // ---- operator coefficients for the current density ("assembly") ----
@@ -513,6 +513,26 @@ amt_problem::OpSel make_op_sel(const amt_problem* p, int cand) {
     }                                                                                          \
   } while (0)
 
+// AMT_STREAM_LAUNCH with the kernel's start and end timestamped into events E0, E1 by the
+// GPU (hipExtLaunchKernelGGL), for the sampled operator launches.
+#define AMT_STREAM_LAUNCH_EV(SEL, KER, E0, E1, ...)                                             \
+  do {                                                                                         \
+    switch ((SEL).shape) {                                                                     \
+      case 0:                                                                                  \
+        hipExtLaunchKernelGGL((KER<16, 16>), dim3((SEL).nb), dim3(kBlock), 0, p->stream, E0,   \
+                              E1, 0, __VA_ARGS__);                                             \
+        break;                                                                                 \
+      case 1:                                                                                  \
+        hipExtLaunchKernelGGL((KER<8, 32>), dim3((SEL).nb), dim3(kBlock), 0, p->stream, E0,    \
+                              E1, 0, __VA_ARGS__);                                             \
+        break;                                                                                 \
+      default:                                                                                 \
+        hipExtLaunchKernelGGL((KER<4, 64>), dim3((SEL).nb), dim3(kBlock), 0, p->stream, E0,    \
+                              E1, 0, __VA_ARGS__);                                             \
+        break;                                                                                 \
+    }                                                                                          \
+  } while (0)
+
 // y = K x (+ penalty with P) in FP64 / FP32 with the selection's launch.
 void launch_apply64(amt_problem* p, const amt_problem::OpSel& s, const double* x, double* y,
                     const double* P) {
@@ -557,9 +577,21 @@ unsigned launch_residual(amt_problem* p, const double* x, double* rout) {
   return s.nb;
 }
 
-// p_new = z + beta p_old, q = K p_new, partials p.q into R_APPLY (flexible CG).
+// p_new = z + beta p_old, q = K p_new, partials p.q into R_APPLY (flexible CG). With events
+// e0, e1 the GPU timestamps the kernel's start and end into them.
 void launch_fcg_apply(amt_problem* p, const amt_problem::OpSel& s, const double* z,
-                      const double* pold, double* pnew, const FcgState* st) {
+                      const double* pold, double* pnew, const FcgState* st,
+                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+  if (e0) {
+    if (s.shape < 0)
+      hipExtLaunchKernelGGL(k_fcg_apply, dim3(s.nb), dim3(kBlock), 0, p->stream, e0, e1, 0, z,
+                            pold, pnew, p->q, p->s64, p->Ke64, p->fixcnt, p->scal + S_P, p->g,
+                            p->tg, st, p->part(R_APPLY));
+    else
+      AMT_STREAM_LAUNCH_EV(s, k_fcg_apply_s, e0, e1, z, pold, pnew, p->q, p->s64, p->had64,
+                           p->fixcnt, p->scal + S_P, p->g, s.sg, st, p->part(R_APPLY));
+    return;
+  }
   if (s.shape < 0)
     hipLaunchKernelGGL(k_fcg_apply, dim3(s.nb), dim3(kBlock), 0, p->stream, z, pold, pnew, p->q,
                        p->s64, p->Ke64, p->fixcnt, p->scal + S_P, p->g, p->tg, st,
@@ -863,13 +895,13 @@ void fcg_enqueue_chunk(amt_problem* p, bool eager_prof) {
   // can reduce all the operator partials itself (nba^2 doubles of L2 reads per launch)
   const bool fused = p->cfg.precision != AMT_PREC_MIXED_BF16 && p->op_apply.nb <= kFusedUpdateTiles;
   for (int t = 0; t < kMgChunk; ++t) {
-    hipEvent_t ev = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     if (eager_prof && p->prof_on && p->ev_used + 2 <= (int)p->ev_pool.size()) {
-      ev = p->ev_pool[p->ev_used];
-      (void)hipEventRecord(ev, p->stream);
+      e0 = p->ev_pool[p->ev_used];
+      e1 = p->ev_pool[p->ev_used + 1];
+      p->ev_used += 2;
     }
-    launch_fcg_apply(p, p->op_apply, p->z, p->p[t & 1], p->p[(t + 1) & 1], p->fcg);
-    if (ev) prof_end(p, ev);
+    launch_fcg_apply(p, p->op_apply, p->z, p->p[t & 1], p->p[(t + 1) & 1], p->fcg, e0, e1);
     if (fused) {
       // the update runs inside the level-0 pre-smoothing; r alternates between two buffers
       mg_vcycle(p, p->rr[(t + 1) & 1], p->z, p->q, p->rr[t & 1], p->p[(t + 1) & 1]);
@@ -1005,17 +1037,15 @@ void dot_dev(amt_problem* p, const double* a, const double* b, double* out) {
                      (int)nred(p->n_dof), p->stride_red, out);
 }
 
-void prof_begin(amt_problem* p, hipEvent_t* a) {
-  *a = nullptr;
+// Events for the next operator launch when it is sampled (every prof_every-th), else null: the
+// launch then goes through hipExtLaunchKernelGGL, which timestamps the kernel's start and end.
+void prof_pick(amt_problem* p, hipEvent_t* e0, hipEvent_t* e1) {
+  *e0 = *e1 = nullptr;
   if (!p->prof_on) return;
   if (++p->prof_tick % p->prof_every) return;
   if (p->ev_used + 2 > (int)p->ev_pool.size()) return;
-  *a = p->ev_pool[p->ev_used];
-  (void)hipEventRecord(*a, p->stream);
-}
-void prof_end(amt_problem* p, hipEvent_t a) {
-  if (!a) return;
-  (void)hipEventRecord(p->ev_pool[p->ev_used + 1], p->stream);
+  *e0 = p->ev_pool[p->ev_used];
+  *e1 = p->ev_pool[p->ev_used + 1];
   p->ev_used += 2;
 }
 void prof_harvest(amt_problem* p) {
@@ -1052,26 +1082,27 @@ int pcg_run(amt_problem* p, const double* b, TX* x, T* r, T* z, T** pp, T* q, co
   int t = 0;
   while (!out->done) {
     for (int c = 0; c < kChunk; ++c, ++t) {
-      hipEvent_t ev;
-      prof_begin(p, &ev);
+      // a sampled launch (events e0, e1) is timestamped by the GPU at the kernel's start and end
+      hipEvent_t e0, e1;
+      prof_pick(p, &e0, &e1);
       if constexpr (std::is_same<T, float>::value) {
         if (mfma)
-          hipLaunchKernelGGL(k_pcg_apply_mfma, dim3(nba), dim3(kBlock), 0, p->stream, t, z,
-                             pp[t & 1], pp[(t + 1) & 1], q, s, p->Kh16, p->Kl16, p->fixcnt,
-                             p->scal + S_P, g, p->tg, p->pcg, (const double*)p->part(R_UPD),
-                             nbu, p->part(R_APPLY), p->stride_red);
+          hipExtLaunchKernelGGL(k_pcg_apply_mfma, dim3(nba), dim3(kBlock), 0, p->stream, e0, e1,
+                                0, t, z, pp[t & 1], pp[(t + 1) & 1], q, s, p->Kh16, p->Kl16,
+                                p->fixcnt, p->scal + S_P, g, p->tg, p->pcg,
+                                (const double*)p->part(R_UPD), nbu, p->part(R_APPLY),
+                                p->stride_red);
         else
-          hipLaunchKernelGGL((k_pcg_apply<T, TK>), dim3(nba), dim3(kBlock), 0, p->stream, t, z,
-                             pp[t & 1], pp[(t + 1) & 1], q, s, Ke, p->fixcnt, p->scal + S_P, g,
-                             p->tg, p->pcg, (const double*)p->part(R_UPD), nbu,
-                             p->part(R_APPLY), p->stride_red);
+          hipExtLaunchKernelGGL((k_pcg_apply<T, TK>), dim3(nba), dim3(kBlock), 0, p->stream, e0,
+                                e1, 0, t, z, pp[t & 1], pp[(t + 1) & 1], q, s, Ke, p->fixcnt,
+                                p->scal + S_P, g, p->tg, p->pcg, (const double*)p->part(R_UPD),
+                                nbu, p->part(R_APPLY), p->stride_red);
       } else {
-        hipLaunchKernelGGL((k_pcg_apply<T, TK>), dim3(nba), dim3(kBlock), 0, p->stream, t, z,
-                           pp[t & 1], pp[(t + 1) & 1], q, s, Ke, p->fixcnt, p->scal + S_P, g,
-                           p->tg, p->pcg, (const double*)p->part(R_UPD), nbu, p->part(R_APPLY),
-                           p->stride_red);
+        hipExtLaunchKernelGGL((k_pcg_apply<T, TK>), dim3(nba), dim3(kBlock), 0, p->stream, e0, e1,
+                              0, t, z, pp[t & 1], pp[(t + 1) & 1], q, s, Ke, p->fixcnt,
+                              p->scal + S_P, g, p->tg, p->pcg, (const double*)p->part(R_UPD), nbu,
+                              p->part(R_APPLY), p->stride_red);
       }
-      prof_end(p, ev);
       hipLaunchKernelGGL((k_pcg_update<T, TX>), dim3(nbu), dim3(kBlock), 0, p->stream, t, x, r,
                          z, pp[(t + 1) & 1], q, dinv, p->n_dof, p->pcg,
                          (const double*)p->part(R_APPLY), nba, p->part(R_UPD), p->stride_red);

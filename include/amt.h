@@ -371,8 +371,9 @@ int amt_operator_info(amt_problem* p, int kind, int* candidate, int* tj, int* tk
  * (as amt_config.operator_tile), up to n entries; 0 where it did not run. */
 int amt_operator_timings(amt_problem* p, int kind, double* us, int n);
 
-/* Kernel timing of the operator (K x) launches, measured with HIP events on the handle's
- * stream: enable, then read the number of timed launches and their summed time. */
+/* Kernel timing of the operator (K x) launches on the handle's stream: sampled launches go
+ * through hipExtLaunchKernelGGL, whose events the GPU timestamps at the kernel's start and end.
+ * Enable, then read the number of timed launches and their summed time. */
 int amt_profile_enable(amt_problem* p, int on);
 int amt_profile_read(amt_problem* p, int64_t* launches, double* total_ms);
 
