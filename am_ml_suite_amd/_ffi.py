@@ -22,7 +22,7 @@ EXPORTS = [
     "amt_run_iteration", "amt_run", "amt_apply_operator", "amt_apply_operator_prec",
     "amt_get_diagonal", "amt_solve", "amt_apply_multigrid",
     "amt_sensitivities", "amt_filter", "amt_filter_neighbors", "amt_oc_update", "amt_sync",
-    "amt_device_info", "amt_profile_enable", "amt_profile_read", "amt_element_stiffness",
+    "amt_device_info", "amt_profile_enable", "amt_profile_read", "amt_profile_read_kind", "amt_element_stiffness",
     "amt_unet_create", "amt_unet_create_for", "amt_unet_destroy", "amt_unet_num_params",
     "amt_unet_get_params", "amt_unet_set_params", "amt_unet_forward", "amt_unet_backward",
     "amt_unet_activation", "amt_unet_get_gain", "amt_unet_set_gain", "amt_attach_unet",
@@ -137,6 +137,7 @@ def load():
     L.amt_device_info.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int)]
     L.amt_profile_enable.argtypes = [P, C.c_int]
     L.amt_profile_read.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    L.amt_profile_read_kind.argtypes = [P, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     L.amt_element_stiffness.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
     _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     L.amt_unet_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(P)]

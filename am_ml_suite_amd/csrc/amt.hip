@@ -218,8 +218,9 @@ struct amt_problem {
   bool prof_on = false;
   std::vector<hipEvent_t> ev_pool;
   int ev_used = 0;
-  int64_t prof_launches = 0;
-  double prof_ms = 0.0;
+  std::vector<int> ev_kind;  // per used event pair: 0 operator, 1 level-0 pre-smoothing + update
+  int64_t prof_launches = 0, prof_launches_upd = 0;
+  double prof_ms = 0.0, prof_ms_upd = 0.0;
   int prof_every = 8, prof_tick = 0;
 
   std::vector<void*> allocs;
@@ -288,6 +289,7 @@ int bank_push(amt_problem* p, const double* r);
 int learned_warm_start(amt_problem* p);
 int residual64(amt_problem* p, const double* x, double* rout, double* rr_host);
 void prof_pick(amt_problem* p, hipEvent_t* e0, hipEvent_t* e1);
+void prof_take(amt_problem* p, int kind, hipEvent_t* e0, hipEvent_t* e1);
 void prof_harvest(amt_problem* p);
 
 // ---- operator coefficients for the current density ("assembly") ----
@@ -603,8 +605,21 @@ void launch_fcg_apply(amt_problem* p, const amt_problem::OpSel& s, const double*
 
 void launch_mg0_pre_upd(amt_problem* p, const amt_problem::OpSel& s, const double* r_prev,
                         double* rnew, const double* q, double* x, const double* pnew,
-                        const FcgState* st, int nba) {
+                        const FcgState* st, int nba, hipEvent_t e0 = nullptr,
+                        hipEvent_t e1 = nullptr) {
   auto& L0 = p->mg[0];
+  if (e0) {  // timestamped by the GPU at the kernel's start and end
+    if (s.shape < 0)
+      hipExtLaunchKernelGGL(k_mg0_pre_upd, dim3(s.nb), dim3(kBlock), 0, p->stream, e0, e1, 0,
+                            r_prev, rnew, q, x, pnew, p->g, p->tg, p->s32, p->Ke32, p->fixcnt,
+                            p->scal + S_P, L0.wd, L0.x, L0.t, st,
+                            (const double*)p->part(R_APPLY), nba, p->part(R_UPD));
+    else
+      AMT_STREAM_LAUNCH_EV(s, k_mg0_pre_upd_s, e0, e1, r_prev, rnew, q, x, pnew, p->g, s.sg,
+                           p->s32, p->had32, p->fixcnt, p->scal + S_P, L0.wd, L0.x, L0.t, st,
+                           (const double*)p->part(R_APPLY), nba, p->part(R_UPD));
+    return;
+  }
   if (s.shape < 0)
     hipLaunchKernelGGL(k_mg0_pre_upd, dim3(s.nb), dim3(kBlock), 0, p->stream, r_prev, rnew, q, x,
                        pnew, p->g, p->tg, p->s32, p->Ke32, p->fixcnt, p->scal + S_P, L0.wd,
@@ -748,14 +763,15 @@ int tune_ops(amt_problem* p) {
 // (k_mg0_pre_upd): r = r_prev - alpha q is formed there, x += alpha p_new, and the |r|^2
 // partials go to R_UPD, one per tile block.
 void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q,
-               const double* r_prev = nullptr, const double* p_new = nullptr) {
+               const double* r_prev = nullptr, const double* p_new = nullptr,
+               hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   auto& mg = p->mg;
   const int* done = (const int*)((const char*)p->fcg + offsetof(FcgState, done));
   const size_t L = mg.size();
   const bool mf = p->cfg.precision == AMT_PREC_MIXED_BF16;
   if (r_prev)
     launch_mg0_pre_upd(p, p->op_upd, r_prev, (double*)r, q, p->u, p_new, p->fcg,
-                       (int)p->op_apply.nb);
+                       (int)p->op_apply.nb, e0, e1);
   else if (mf)
     hipLaunchKernelGGL(k_mg0_pre_mfma, dim3(p->nb_tile), dim3(kBlock), 0, p->stream, r, p->g,
                        p->tg, p->s32, p->Kh16, p->Kl16, p->fixcnt, p->scal + S_P, mg[0].wd,
@@ -896,15 +912,13 @@ void fcg_enqueue_chunk(amt_problem* p, bool eager_prof) {
   const bool fused = p->cfg.precision != AMT_PREC_MIXED_BF16 && p->op_apply.nb <= kFusedUpdateTiles;
   for (int t = 0; t < kMgChunk; ++t) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (eager_prof && p->prof_on && p->ev_used + 2 <= (int)p->ev_pool.size()) {
-      e0 = p->ev_pool[p->ev_used];
-      e1 = p->ev_pool[p->ev_used + 1];
-      p->ev_used += 2;
-    }
+    if (eager_prof) prof_take(p, 0, &e0, &e1);
     launch_fcg_apply(p, p->op_apply, p->z, p->p[t & 1], p->p[(t + 1) & 1], p->fcg, e0, e1);
     if (fused) {
       // the update runs inside the level-0 pre-smoothing; r alternates between two buffers
-      mg_vcycle(p, p->rr[(t + 1) & 1], p->z, p->q, p->rr[t & 1], p->p[(t + 1) & 1]);
+      hipEvent_t u0 = nullptr, u1 = nullptr;
+      if (eager_prof) prof_take(p, 1, &u0, &u1);
+      mg_vcycle(p, p->rr[(t + 1) & 1], p->z, p->q, p->rr[t & 1], p->p[(t + 1) & 1], u0, u1);
     } else {
       hipLaunchKernelGGL(k_fcg_update, dim3(nbu), dim3(kBlock), 0, p->stream, p->u, p->r,
                          p->p[(t + 1) & 1], p->q, p->n_dof, p->fcg,
@@ -1037,16 +1051,23 @@ void dot_dev(amt_problem* p, const double* a, const double* b, double* out) {
                      (int)nred(p->n_dof), p->stride_red, out);
 }
 
-// Events for the next operator launch when it is sampled (every prof_every-th), else null: the
-// launch then goes through hipExtLaunchKernelGGL, which timestamps the kernel's start and end.
+// Two events of the pool for a timed launch of the given kind (0 operator, 1 level-0
+// pre-smoothing with the update) while profiling is on and the pool lasts, else null; the
+// launch goes through hipExtLaunchKernelGGL, which timestamps the kernel's start and end.
+void prof_take(amt_problem* p, int kind, hipEvent_t* e0, hipEvent_t* e1) {
+  *e0 = *e1 = nullptr;
+  if (!p->prof_on || p->ev_used + 2 > (int)p->ev_pool.size()) return;
+  *e0 = p->ev_pool[p->ev_used];
+  *e1 = p->ev_pool[p->ev_used + 1];
+  p->ev_kind[p->ev_used / 2] = kind;
+  p->ev_used += 2;
+}
+// Events for the next operator launch when it is sampled (every prof_every-th), else null.
 void prof_pick(amt_problem* p, hipEvent_t* e0, hipEvent_t* e1) {
   *e0 = *e1 = nullptr;
   if (!p->prof_on) return;
   if (++p->prof_tick % p->prof_every) return;
-  if (p->ev_used + 2 > (int)p->ev_pool.size()) return;
-  *e0 = p->ev_pool[p->ev_used];
-  *e1 = p->ev_pool[p->ev_used + 1];
-  p->ev_used += 2;
+  prof_take(p, 0, e0, e1);
 }
 void prof_harvest(amt_problem* p) {
   if (!p->prof_on || p->ev_used == 0) return;
@@ -1054,8 +1075,13 @@ void prof_harvest(amt_problem* p) {
   for (int i = 0; i < p->ev_used; i += 2) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, p->ev_pool[i], p->ev_pool[i + 1]) == hipSuccess) {
-      p->prof_ms += ms;
-      p->prof_launches += 1;
+      if (p->ev_kind[i / 2] == 1) {
+        p->prof_ms_upd += ms;
+        p->prof_launches_upd += 1;
+      } else {
+        p->prof_ms += ms;
+        p->prof_launches += 1;
+      }
     }
   }
   p->ev_used = 0;
@@ -2190,21 +2216,27 @@ int amt_operator_timings(amt_problem* p, int kind, double* us, int n) {
 int amt_profile_enable(amt_problem* p, int on) {
   if (!p) return AMT_EINVAL;
   p->prof_on = on != 0;
-  p->prof_ms = 0.0;
-  p->prof_launches = 0;
+  p->prof_ms = p->prof_ms_upd = 0.0;
+  p->prof_launches = p->prof_launches_upd = 0;
   p->prof_tick = 0;
   if (p->prof_on && p->ev_pool.empty()) {
     p->ev_pool.resize(2 * 4096);
+    p->ev_kind.assign(4096, 0);
     for (auto& e : p->ev_pool) HIP_TRY(p, hipEventCreate(&e));
   }
   return AMT_OK;
 }
 
 int amt_profile_read(amt_problem* p, int64_t* launches, double* total_ms) {
+  return amt_profile_read_kind(p, 0, launches, total_ms);
+}
+
+int amt_profile_read_kind(amt_problem* p, int kind, int64_t* launches, double* total_ms) {
   if (!p) return AMT_EINVAL;
+  if (kind < 0 || kind > 1) return fail(p, AMT_ERANGE, "profile kind must be 0 or 1");
   prof_harvest(p);
-  if (launches) *launches = p->prof_launches;
-  if (total_ms) *total_ms = p->prof_ms;
+  if (launches) *launches = kind ? p->prof_launches_upd : p->prof_launches;
+  if (total_ms) *total_ms = kind ? p->prof_ms_upd : p->prof_ms;
   return AMT_OK;
 }
 

@@ -364,9 +364,11 @@ class TopologyOptimizer:
     def profile_enable(self, on=True):
         self._check(self._L.amt_profile_enable(self._h, int(bool(on))))
 
-    def profile_read(self):
+    def profile_read(self, kind=0):
+        """(timed launches, their total ms) of kind 0 (operator) or 1 (level-0 pre-smoothing
+        with the flexible-CG update)."""
         n, ms = C.c_int64(), C.c_double()
-        self._check(self._L.amt_profile_read(self._h, C.byref(n), C.byref(ms)))
+        self._check(self._L.amt_profile_read_kind(self._h, int(kind), C.byref(n), C.byref(ms)))
         return n.value, ms.value
 
     def operator_info(self, kind=0):

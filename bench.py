@@ -40,6 +40,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch of the fast path's level-0 pre-smoothing with the flexible-CG update, by
+# candidate, from the same kind of counter runs (profiles/fcg_apply_bracket120x60x60_hbm_traffic.md)
+PRE_UPD_TRAFFIC_BYTES = {0: 96.60e6, 1: 139.87e6, 2: 120.51e6, 3: 100.03e6, 4: 119.36e6,
+                         5: 99.72e6, 6: 93.19e6, 7: 110.80e6, 8: 92.94e6, 9: 87.62e6,
+                         10: 136.59e6, 11: 141.31e6, 12: 109.70e6, 13: 127.14e6, 14: 97.46e6,
+                         15: 111.52e6}
 # HBM bytes per launch of the fast path's FP64 operator on this workload, by operator candidate
 # (amt_config.operator_tile), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a plain
 # 256 MiB copy (profiles/fcg_apply_bracket120x60x60_hbm_traffic.md)
@@ -85,10 +91,13 @@ def run_path(problem, warmup, steps, precision, warm_start, profile, precond="ja
     opt.sync()
     dt = time.perf_counter() - t0
     prof = opt.profile_read() if profile else (0, 0.0)
+    prof_upd = opt.profile_read(1) if profile and precond == "multigrid" else (0, 0.0)
     dens = opt.get_density()
     op = opt.operator_info(0) if precond == "multigrid" else None
+    op_upd = opt.operator_info(1) if precond == "multigrid" else None
     opt.close()
-    return dict(time=dt, results=res, density=dens, prof=prof, op=op)
+    return dict(time=dt, results=res, density=dens, prof=prof, op=op, prof_upd=prof_upd,
+                op_upd=op_upd)
 
 
 def run_large(steps):
@@ -524,31 +533,47 @@ def main():
     timed = m["results"][args.warmup:]
     pcg_per_oc = float(np.mean([r.solver_iterations for r in timed]))
 
-    # dominant kernel: the fused PCG operator launch (p = z + beta p_old gathered, q = K p)
-    vb = 8
-    bytes_per_launch = n_dof * (4 * vb + 1) + n_e * vb
-    flops_per_launch = 1152.0 * n_n
-    launches, ms = m["prof"]
-    if launches:
+    def op_name(base, op, tile):
+        if op["candidate"] == 0:
+            return "%s (%s)" % (base, tile)
+        return ("%s_s<%d, %d> (element-column streaming tile, %d node planes)"
+                % (base, op["tj"], op["tk"], op["planes"]))
+
+    def roof(prof, bytes_per_launch, flops_per_launch, kernel, traffic):
+        launches, ms = prof
+        if not launches:
+            return None
         avg_s = ms / launches / 1e3
         achieved = bytes_per_launch / avg_s / 1e9
-        kernel, traffic = "k_pcg_apply<double>", None
-        if main_key == "fast":
-            op = m["op"]
-            if op["candidate"] == 0:
-                kernel = "k_fcg_apply (4x4x16 node gather tile)"
-            else:
-                kernel = ("k_fcg_apply_s<%d, %d> (element-column streaming tile, %d node planes)"
-                          % (op["tj"], op["tk"], op["planes"]))
-            traffic = FCG_APPLY_TRAFFIC_BYTES.get(op["candidate"])
-        roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=achieved / HBM_PEAK_GBS, traffic=traffic, kernel=kernel,
-                        avg_launch_us=avg_s * 1e6, timed_launches=launches,
-                        algorithmic_bytes_per_launch=bytes_per_launch,
-                        flops_per_launch=flops_per_launch,
-                        achieved_tflops=flops_per_launch / avg_s / 1e12)
-    else:
-        roofline = None
+        return dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=achieved / HBM_PEAK_GBS, traffic=traffic, kernel=kernel,
+                    avg_launch_us=avg_s * 1e6, timed_launches=launches,
+                    algorithmic_bytes_per_launch=bytes_per_launch,
+                    flops_per_launch=flops_per_launch,
+                    achieved_tflops=flops_per_launch / avg_s / 1e12)
+
+    # The operator launch: p = z + beta p_old formed as the tile loads it, q = K p (FP64), p.q
+    # partials; 33 B/DOF (read z, p_old; write p, q; the penalty count) + the 8-byte element
+    # scale. flops: the dense 24x24 element product's 1152 per node.
+    flops_op = 1152.0 * n_n
+    roof_op = roof(m["prof"], n_dof * (4 * 8 + 1) + n_e * 8, flops_op,
+                   "k_pcg_apply<double>" if main_key != "fast" else
+                   op_name("k_fcg_apply", m["op"], "4x4x16 node gather tile"),
+                   FCG_APPLY_TRAFFIC_BYTES.get(m["op"]["candidate"]) if main_key == "fast"
+                   else None)
+    roofline, kernels = roof_op, None
+    if main_key == "fast":
+        # The level-0 pre-smoothing with the flexible-CG update (one launch per FCG iteration,
+        # like the operator): alpha from the operator's partials, r' = r - alpha q and
+        # x += alpha p (FP64), x0 = wd r' and t = r' - K x0 (FP32 operator); 61 B/DOF (read r,
+        # q, x, p, wd, the penalty count; write r', x, x0, t) + the 4-byte element scale.
+        roof_upd = roof(m["prof_upd"], n_dof * 61 + n_e * 4, flops_op,
+                        op_name("k_mg0_pre_upd", m["op_upd"], "4x4x16 node gather tile"),
+                        PRE_UPD_TRAFFIC_BYTES.get(m["op_upd"]["candidate"]))
+        kernels = dict(fcg_operator=roof_op, pre_smoothing_update=roof_upd)
+        # the dominant kernel: the longer of the two launches of every FCG iteration
+        if roof_upd and (roof_op is None or roof_upd["avg_launch_us"] > roof_op["avg_launch_us"]):
+            roofline = roof_upd
 
     extra = {}
     if "plain" in out:
@@ -587,7 +612,7 @@ def main():
                               + (", fine-level smoothing on MFMA (split BF16)"
                                  if args.fast_smoother == "mfma" else "")
                               if main_key == "fast" else "FP64 Jacobi-PCG")},
-        "roofline": roofline, "cpu_baseline": cpu, "batch": batch,
+        "roofline": roofline, "roofline_kernels": kernels, "cpu_baseline": cpu, "batch": batch,
     }
     if main_key == "fast":
         line["metric_note"] = (

@@ -376,6 +376,11 @@ int amt_operator_timings(amt_problem* p, int kind, double* us, int n);
  * Enable, then read the number of timed launches and their summed time. */
 int amt_profile_enable(amt_problem* p, int on);
 int amt_profile_read(amt_problem* p, int64_t* launches, double* total_ms);
+/* As amt_profile_read for kind 0 (the operator launches: every 8th of the Jacobi-PCG solver's,
+ * the flexible-CG operator's of the first chunk of every multigrid solve) or kind 1 (the
+ * level-0 pre-smoothing with the flexible-CG update, first chunk of every multigrid solve).
+ * No counterpart in the reference. */
+int amt_profile_read_kind(amt_problem* p, int kind, int64_t* launches, double* total_ms);
 
 #ifdef __cplusplus
 }
