@@ -621,7 +621,8 @@ def main():
             "previous two displacements. The learned "
             "part (U-Net warm start trained on the GPU) is timed on the same workload in "
             "learned.warm_start, against this path without it (speedup_learned_vs_multigrid), "
-            "and is not in value")
+            "and is not in value. roofline is the longer of the two level-0 launches of an FCG "
+            "iteration (roofline_kernels has both)")
     if large is not None:
         line["large"] = large
     if simp_py is not None:
