@@ -40,7 +40,7 @@ namespace {
 constexpr int kBChunk = 16;        // PCG iterations per chunk graph (even: parity returns)
 constexpr int kBMgWorkers = 8;     // host threads of the multigrid batch
 constexpr int kBPersist = 2048;    // blocks of a persistent launch (8 per CU)
-constexpr int kBOcLevels = 6;
+constexpr int kBOcLevels = 3;  // bisection levels per OC sweep (7 candidates)
 constexpr int kBOcCand = (1 << kBOcLevels) - 1;
 // Matvec candidates of the lockstep solver: 0 the LDS gather tile (BK_TILE items); 1 + 3 s + l
 // the element-column streaming tile of shape kBShape[s] (threads along j x k) with kBPlanes[l]
