@@ -440,7 +440,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--paths", default="plain,fast,learned,batch,simp_py")
+    ap.add_argument("--paths", default="plain,fast,learned,batch,simp_py,large,unet_conv")
     ap.add_argument("--large-steps", type=int, default=100)
     ap.add_argument("--fast-smoother", choices=("fp32", "mfma"), default="fp32",
                     help="fine-level smoothing of the fast path's V-cycle: the FP32 vector "
