@@ -69,7 +69,7 @@ constexpr float kMgTheta = 1.9f;  // ...capped per row at theta / sum_j |A_ij| (
 // whose spectrum lies in (0, 1] (Gershgorin); the polynomial damps [1 / kChebRatio, 1].
 // Step s (from 1): d' = a_s d + b_s M^-1 (rhs - A x), x' = x + d' (the three-term recurrence
 // of the Chebyshev iteration, with a_1 = 0).
-constexpr double kChebRatio = 20.0;
+constexpr double kChebRatio = 100.0;
 struct ChebCoef {
   double a, b;
 };
