@@ -60,7 +60,8 @@ constexpr int kChunk = 32;  // PCG iterations launched between two host polls
 // (k_mg0_pre_upd): 2048 blocks each reading 2048 partials is 33 MB of L2 reads per launch; at
 // 256x128x128 (19,305 tiles) it would be 3 GB, so larger grids keep k_fcg_update.
 constexpr unsigned kFusedUpdateTiles = 2048;
-constexpr int kMgChunk = 4;  // flexible-CG iterations per captured graph (even)
+constexpr int kMgChunk = 2;  // flexible-CG iterations per captured graph (even)
+constexpr int kNnChunk = 4;      // learned FCG: iterations between host polls (even)
 constexpr int kNnPatience = 16;  // learned FCG: chunks allowed without a 10% |r| drop
 constexpr double kNnGainMomentum = 0.05;  // running means behind the correction's weight
 constexpr float kMgOmega = 0.6f;  // damped-Jacobi weight of the V-cycle smoother...
@@ -1333,7 +1334,7 @@ int solve_unet(amt_problem* p, amt_solve_stats* st) {
   double rr_mark = hs.rr;
   int t = 0, fell_back = 0, chunk = 0, mark_chunk = 0;
   while (!hs.done) {
-    for (int c = 0; c < kMgChunk; ++c, ++t) {
+    for (int c = 0; c < kNnChunk; ++c, ++t) {
       launch_fcg_apply(p, p->op_apply, p->z, p->p[t & 1], p->p[(t + 1) & 1], p->fcg);
       hipLaunchKernelGGL(k_fcg_update, dim3(nbu), dim3(kBlock), 0, p->stream, p->u, p->r,
                          p->p[(t + 1) & 1], p->q, p->n_dof, p->fcg,
