@@ -194,6 +194,10 @@ struct amt_problem {
   IterScalars* iter_scal = nullptr;
   // one captured chunk of kMgChunk flexible-CG iterations, replayed per chunk
   hipGraph_t fcg_graph = nullptr;
+  // the post-solve launches of an OC iteration, captured once
+  hipGraph_t post_graph = nullptr;
+  hipGraphExec_t post_exec = nullptr;
+  bool post_graph_failed = false;
   hipGraphExec_t fcg_exec = nullptr;
   bool fcg_graph_failed = false;
   int mg_last_iters = 0;  // iterations of the previous solve: sizes the first launch batch
@@ -1507,16 +1511,11 @@ int build_design(amt_problem* p) {
   return AMT_OK;
 }
 
-// OC update on the device density with p->dcf; writes state into p->oc.
-int oc_update(amt_problem* p, OcState* hs) {
-  if (!p->design_ready) AMT_CHECK(build_design(p));
-  OcState init;
-  std::memset(&init, 0, sizeof(init));
-  init.l1 = 1e-20;
-  init.l2 = 1e20;
-  init.vf = p->cfg.volume_fraction;
-  init.n_design = p->n_design > 0 ? p->n_design : (double)p->n_e;  // :222-225 fallback
-  HIP_TRY(p, hipMemcpyAsync(p->oc, &init, sizeof(OcState), hipMemcpyHostToDevice, p->stream));
+// The OC update's launches on the device density with p->dcf (state into p->oc); every
+// argument is fixed for the handle, so the sequence can be captured once.
+void enqueue_oc(amt_problem* p) {
+  hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(64), 0, p->stream, p->oc,
+                     p->cfg.volume_fraction, (const double*)(p->scal + S_NDESIGN), p->n_e);
   const unsigned nb = std::min<unsigned>(nred(p->n_e), kOcBlocks);
   hipLaunchKernelGGL(k_oc_bounds, dim3(nb), dim3(kBlock), 0, p->stream, p->rho, p->dcf,
                      p->design, p->n_e, p->cfg.move_limit, p->cfg.rho_min, p->part(R_OC),
@@ -1536,6 +1535,12 @@ int oc_update(amt_problem* p, OcState* hs) {
                      p->part(R_MISC), p->stride_red);
   hipLaunchKernelGGL(k_oc_final_reduce, dim3(1), dim3(kBlock), 0, p->stream, p->oc,
                      (const double*)p->part(R_MISC), (int)nb, p->stride_red);
+}
+
+// OC update on the device density with p->dcf; writes state into p->oc.
+int oc_update(amt_problem* p, OcState* hs) {
+  if (!p->design_ready) AMT_CHECK(build_design(p));
+  enqueue_oc(p);
   AMT_CHECK(check_launch(p, "oc"));
   p->coeffs_ready = false;
   if (hs) {
@@ -1543,6 +1548,44 @@ int oc_update(amt_problem* p, OcState* hs) {
     HIP_TRY(p, hipStreamSynchronize(p->stream));
   }
   return AMT_OK;
+}
+
+// After the solve: compliance with the unmodified forces (topology_optimizer.cpp:338),
+// sensitivities, filter, the OC update and the end-of-iteration scalars for one host read.
+void enqueue_post_solve(amt_problem* p) {
+  dot_dev(p, p->u, p->F, p->scal + S_COMPL);
+  hipLaunchKernelGGL(k_sens, dim3(nblk(p->n_e)), dim3(kBlock), 0, p->stream, p->u, p->rho,
+                     p->Ke64, p->g, p->cfg.penalty, p->dc);
+  launch_filter(p);
+  enqueue_oc(p);
+  hipLaunchKernelGGL(k_iter_scalars, dim3(1), dim3(64), 0, p->stream, p->oc, p->scal + S_COMPL,
+                     p->scal + S_RR, p->iter_scal);
+}
+
+// The post-solve launches (~80, most of them short OC sweeps and walks) captured once per
+// handle and replayed, so the host does not enqueue them one by one. Returns false if no graph
+// could be built (the launches are then enqueued eagerly).
+bool post_graph_ready(amt_problem* p) {
+  if (p->post_exec) return true;
+  if (p->post_graph_failed) return false;
+  p->post_graph_failed = true;
+  if (hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  enqueue_post_solve(p);
+  hipGraph_t g = nullptr;
+  const hipError_t e1 = hipStreamEndCapture(p->stream, &g);
+  if (e1 == hipSuccess && g &&
+      hipGraphInstantiate(&p->post_exec, g, nullptr, nullptr, 0) == hipSuccess) {
+    p->post_graph = g;
+    p->post_graph_failed = false;
+    return true;
+  }
+  (void)hipGetLastError();
+  if (g) (void)hipGraphDestroy(g);
+  p->post_exec = nullptr;
+  return false;
 }
 
 int run_iteration(amt_problem* p, amt_iteration_result* out) {
@@ -1561,18 +1604,25 @@ int run_iteration(amt_problem* p, amt_iteration_result* out) {
   p->defer_true_rr = false;
   AMT_CHECK(rc_solve);
   const double t2 = now_s();
-  // compliance with the unmodified forces (topology_optimizer.cpp:338)
-  dot_dev(p, p->u, p->F, p->scal + S_COMPL);
-  hipLaunchKernelGGL(k_sens, dim3(nblk(p->n_e)), dim3(kBlock), 0, p->stream, p->u, p->rho,
-                     p->Ke64, p->g, p->cfg.penalty, p->dc);
-  AMT_CHECK(check_launch(p, "sensitivities"));
-  if (p->cfg.verbose) HIP_TRY(p, hipStreamSynchronize(p->stream));
-  const double t3 = now_s();
-  launch_filter(p);
-  AMT_CHECK(oc_update(p, nullptr));
-  // one host read per OC iteration after the solve: OC state, compliance, true residual
-  hipLaunchKernelGGL(k_iter_scalars, dim3(1), dim3(64), 0, p->stream, p->oc, p->scal + S_COMPL,
-                     p->scal + S_RR, p->iter_scal);
+  double t3 = t2;
+  if (p->cfg.use_graphs && !p->cfg.verbose && post_graph_ready(p)) {
+    HIP_TRY(p, hipGraphLaunch(p->post_exec, p->stream));
+  } else {
+    // compliance with the unmodified forces (topology_optimizer.cpp:338)
+    dot_dev(p, p->u, p->F, p->scal + S_COMPL);
+    hipLaunchKernelGGL(k_sens, dim3(nblk(p->n_e)), dim3(kBlock), 0, p->stream, p->u, p->rho,
+                       p->Ke64, p->g, p->cfg.penalty, p->dc);
+    AMT_CHECK(check_launch(p, "sensitivities"));
+    if (p->cfg.verbose) HIP_TRY(p, hipStreamSynchronize(p->stream));
+    t3 = now_s();
+    launch_filter(p);
+    enqueue_oc(p);
+    // one host read per OC iteration after the solve: OC state, compliance, true residual
+    hipLaunchKernelGGL(k_iter_scalars, dim3(1), dim3(64), 0, p->stream, p->oc,
+                       p->scal + S_COMPL, p->scal + S_RR, p->iter_scal);
+  }
+  AMT_CHECK(check_launch(p, "post-solve"));
+  p->coeffs_ready = false;
   IterScalars sc;
   AMT_CHECK(copy_out(p, &sc, p->iter_scal, sizeof(IterScalars)));
   const OcState& hs = sc.oc;
@@ -1833,6 +1883,8 @@ int amt_destroy(amt_problem* p) {
   for (auto e : p->ev_pool) (void)hipEventDestroy(e);
   if (p->fcg_exec) (void)hipGraphExecDestroy(p->fcg_exec);
   if (p->fcg_graph) (void)hipGraphDestroy(p->fcg_graph);
+  if (p->post_exec) (void)hipGraphExecDestroy(p->post_exec);
+  if (p->post_graph) (void)hipGraphDestroy(p->post_graph);
 
   for (void* a : p->allocs) (void)hipFree(a);
   if (p->stream) (void)hipStreamDestroy(p->stream);

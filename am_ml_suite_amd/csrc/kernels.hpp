@@ -568,6 +568,20 @@ __global__ __launch_bounds__(kBlock) void k_oc_sweep(const double* __restrict__ 
   store_partials_many<NC>(vol, partials, stride);
 }
 
+// Bisection state at the start of an OC update (topology_optimizer.cpp:215-225): [1e-20, 1e20],
+// the volume target, and the design-element count (all elements when the mask is empty).
+__global__ void k_oc_init(OcState* st, double vf, const double* __restrict__ n_design,
+                          int64_t n_e) {
+  if (threadIdx.x || blockIdx.x) return;
+  OcState s;
+  memset(&s, 0, sizeof(s));
+  s.l1 = 1e-20;
+  s.l2 = 1e20;
+  s.vf = vf;
+  s.n_design = *n_design > 0 ? *n_design : (double)n_e;
+  *st = s;
+}
+
 // End-of-iteration scalars gathered for one host read: OC state, compliance, |b - K u|^2.
 struct IterScalars {
   OcState oc;
