@@ -42,17 +42,17 @@ BF16_PEAK_TFLOPS = 2500.0
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the fast path's level-0 pre-smoothing with the flexible-CG update, by
 # candidate, from the same kind of counter runs (profiles/fcg_apply_bracket120x60x60_hbm_traffic.md)
-PRE_UPD_TRAFFIC_BYTES = {0: 96.60e6, 1: 139.87e6, 2: 120.51e6, 3: 100.03e6, 4: 119.36e6,
-                         5: 99.72e6, 6: 93.19e6, 7: 110.80e6, 8: 92.94e6, 9: 87.62e6,
-                         10: 136.59e6, 11: 141.31e6, 12: 109.70e6, 13: 127.14e6, 14: 97.46e6,
-                         15: 111.52e6}
+PRE_UPD_TRAFFIC_BYTES = {0: 101.49e6, 1: 108.65e6, 2: 88.75e6, 3: 84.40e6, 4: 103.86e6,
+                         5: 87.92e6, 6: 84.37e6, 7: 102.79e6, 8: 87.92e6, 9: 84.38e6,
+                         10: 87.24e6, 11: 95.91e6, 12: 90.85e6, 13: 106.27e6, 14: 87.54e6,
+                         15: 98.60e6}
 # HBM bytes per launch of the fast path's FP64 operator on this workload, by operator candidate
 # (amt_config.operator_tile), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a plain
 # 256 MiB copy (profiles/fcg_apply_bracket120x60x60_hbm_traffic.md)
-FCG_APPLY_TRAFFIC_BYTES = {0: 49.39e6, 1: 73.98e6, 2: 66.39e6, 3: 62.07e6, 4: 66.81e6,
-                          5: 62.17e6, 6: 58.51e6, 7: 60.65e6, 8: 58.30e6, 9: 55.18e6,
-                          10: 66.42e6, 11: 70.08e6, 12: 56.91e6, 13: 63.91e6, 14: 51.37e6,
-                          15: 55.39e6}
+FCG_APPLY_TRAFFIC_BYTES = {0: 49.39e6, 1: 51.18e6, 2: 51.32e6, 3: 49.03e6, 4: 51.41e6,
+                          5: 51.54e6, 6: 49.04e6, 7: 50.81e6, 8: 51.38e6, 9: 49.03e6,
+                          10: 49.26e6, 11: 49.41e6, 12: 49.14e6, 13: 50.71e6, 14: 49.14e6,
+                          15: 49.64e6}
 
 
 def log(msg):
