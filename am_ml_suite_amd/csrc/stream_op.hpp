@@ -207,10 +207,17 @@ inline unsigned stream_blocks(const StreamGrid& s) { return 8u * (unsigned)s.per
 template <typename T, int TK>
 constexpr int kPlaneRow = (sizeof(T) == 4 && TK == 16) ? 48 : TK + 1;
 
+// FP32 tiles take one barrier per plane step: three node-plane buffers and two slot buffers,
+// each node's sum finished one step later. FP64 tiles keep two barriers per step and two plane
+// buffers, so that four blocks of the largest shape fit the LDS of a CU.
+template <typename T>
+constexpr bool kOneBarrier = sizeof(T) == 4;
+
 template <typename T, int TJ, int TK>
 struct StreamSmem {
-  T plane[2][3][(TJ + 1) * kPlaneRow<T, TK>];  // [parity][component][row * stride + col]
-  T slot[12][TJ * TK];                         // [(oy + 2 oz) * 3 + c][thread]
+  // [buffer][component][row * stride + col]
+  T plane[kOneBarrier<T> ? 3 : 2][3][(TJ + 1) * kPlaneRow<T, TK>];
+  T slot[kOneBarrier<T> ? 2 : 1][12][TJ * TK];  // [buffer][(oy + 2 oz) * 3 + c][thread]
 };
 
 // Runs tile t of the grid (t outside [0, n_tiles): nothing). ld(q) issues the loads of DOF q's
@@ -269,6 +276,21 @@ __device__ __forceinline__ void stream_tile_at(StreamSmem<T, TJ, TK>& sm, const 
     return (el_col && e >= 0 && e < g.nx) ? s[((int64_t)e * g.ny + j) * g.nz + k] : T(0);
   };
 
+  // node (e, j, k) is corner (oy, oz) of the element columns (j - oy, k - oz): its four partial
+  // sums from slot buffer sb, in a fixed order
+  using Pre = decltype(pre((int64_t)-1));
+  auto finish = [&](int sb, int64_t n, const T (&xo)[3], const Pre& pv) {
+    T y[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      y[c] = sm.slot[sb][0 * 3 + c][tid];
+      y[c] += sm.slot[sb][1 * 3 + c][tid - TK];
+      y[c] += sm.slot[sb][2 * 3 + c][tid - 1];
+      y[c] += sm.slot[sb][3 * 3 + c][tid - TK - 1];
+    }
+    fin(n, y, xo, pv);
+  };
+
   issue(i0 - 1);
   store_plane(0);
   issue(i0);
@@ -278,12 +300,18 @@ __device__ __forceinline__ void stream_tile_at(StreamSmem<T, TJ, TK>& sm, const 
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int c = 0; c < 3; ++c) prev1[a][c] = T(0);
+  // one-barrier mode: the node of the previous step, finished after this step's barrier
+  Pre pv_h{};
+  T xo_h[3] = {T(0), T(0), T(0)};
+  int64_t n_h = 0;
+  bool out_h = false;
 
-  int par = 0;  // parity buffer holding node plane e
-  for (int e = i0 - 1; e <= elast; ++e, par ^= 1) {
-    store_plane(par ^ 1);          // node plane e + 1
+  int pb = 0;  // buffer holding node plane e
+  for (int e = i0 - 1, st = 0; e <= elast; ++e, ++st) {
+    const int pn = kOneBarrier<T> ? (pb == 2 ? 0 : pb + 1) : (pb ^ 1);
+    store_plane(pn);  // node plane e + 1
     // Vector memory operations complete in issue order for the wait counter: the epilogue's
-    // loads (waited for at the end of this step) and the next scale go out before the plane
+    // loads (waited for when the node is finished) and the next scale go out before the plane
     // two steps ahead, so waiting for them does not wait for that prefetch.
     const bool outp = e >= i0 && own_col;
     const int64_t n = ((int64_t)e * g.nny + j) * g.nnz + k;
@@ -291,40 +319,47 @@ __device__ __forceinline__ void stream_tile_at(StreamSmem<T, TJ, TK>& sm, const 
     const T se = sv;
     if (e + 1 <= elast) sv = scale(e + 1);
     if (e + 2 <= elast + 1) issue(e + 2);
+    // Two-barrier mode: plane e + 1 is stored. One-barrier mode also: the previous step's slot
+    // buffer is written, and every thread is past its reads of the plane buffer that the next
+    // step overwrites (three buffers) and of the slot buffer that this step overwrites (two).
     __syncthreads();
+    if (kOneBarrier<T> && out_h) finish((st - 1) & 1, n_h, xo_h, pv_h);
     T u[3][8], xo[3];
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
       const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
       const int o = (tj + by) * RS + tk + bz;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) u[c][b] = sm.plane[par ^ bx][c][o];
+      for (int c = 0; c < 3; ++c) u[c][b] = sm.plane[bx ? pn : pb][c][o];
     }
     // corner 0 is the thread's own node of plane e; taken here, before the barrier after which
-    // a faster thread may overwrite this plane buffer with plane e + 2
+    // a faster thread may overwrite this plane buffer
 #pragma unroll
     for (int c = 0; c < 3; ++c) xo[c] = u[c][0];
     had_apply(u, K, se);
+    const int sb = kOneBarrier<T> ? (st & 1) : 0;
 #pragma unroll
     for (int a = 0; a < 4; ++a)  // a = oy + 2 oz: corners b = 2a (ox = 0) and 2a + 1 (ox = 1)
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        sm.slot[a * 3 + c][tid] = u[c][2 * a] + prev1[a][c];
+        sm.slot[sb][a * 3 + c][tid] = u[c][2 * a] + prev1[a][c];
         prev1[a][c] = u[c][2 * a + 1];
       }
-    __syncthreads();
-    if (outp) {
-      // node (e, j, k) is corner (oy, oz) of the element column (j - oy, k - oz)
-      T y[3];
+    if constexpr (kOneBarrier<T>) {
+      out_h = outp;
+      n_h = n;
+      pv_h = pv;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        y[c] = sm.slot[0 * 3 + c][tid];
-        y[c] += sm.slot[1 * 3 + c][tid - TK];
-        y[c] += sm.slot[2 * 3 + c][tid - 1];
-        y[c] += sm.slot[3 * 3 + c][tid - TK - 1];
-      }
-      fin(n, y, xo, pv);
+      for (int c = 0; c < 3; ++c) xo_h[c] = xo[c];
+    } else {
+      __syncthreads();
+      if (outp) finish(0, n, xo, pv);
     }
+    pb = pn;
+  }
+  if constexpr (kOneBarrier<T>) {
+    __syncthreads();
+    if (out_h) finish((elast - i0 + 1) & 1, n_h, xo_h, pv_h);
   }
 }
 
