@@ -19,8 +19,9 @@
 //   - forms its element's 24 inputs from two node planes and applies s_e Ke0;
 //   - adds the element's ox = 0 corners to the previous element's ox = 1 corners, kept in
 //     registers: the sum along x never leaves the thread;
-//   - exchanges the 12 partial sums (4 corners (oy, oz) x 3 components) through LDS, and each
-//     owned node adds its four in a fixed order.
+//   - keeps the partial sums of its own node (corner (0, 0)) in registers, exchanges the other 9
+//     (3 corners (oy, oz) x 3 components) through LDS, and each owned node adds its four in a
+//     fixed order.
 // A tile owns (TJ - 1) x (TK - 1) node columns and L node planes; neighbouring tiles overlap by
 // one element column. Per node this is ~3 LDS values written and ~27 read, instead of the 27
 // neighbours x 8 elements of a gather tile, and every node's eight contributions are summed in
@@ -217,7 +218,9 @@ template <typename T, int TJ, int TK>
 struct StreamSmem {
   // [buffer][component][row * stride + col]
   T plane[kOneBarrier<T> ? 3 : 2][3][(TJ + 1) * kPlaneRow<T, TK>];
-  T slot[kOneBarrier<T> ? 2 : 1][12][TJ * TK];  // [buffer][(oy + 2 oz) * 3 + c][thread]
+  // corners (oy, oz) != (0, 0) of each element column; corner (0, 0) is the thread's own node and
+  // stays in registers
+  T slot[kOneBarrier<T> ? 2 : 1][9][TJ * TK];  // [buffer][(oy + 2 oz - 1) * 3 + c][thread]
 };
 
 // Runs tile t of the grid (t outside [0, n_tiles): nothing). ld(q) issues the loads of DOF q's
@@ -279,14 +282,14 @@ __device__ __forceinline__ void stream_tile_at(StreamSmem<T, TJ, TK>& sm, const 
   // node (e, j, k) is corner (oy, oz) of the element columns (j - oy, k - oz): its four partial
   // sums from slot buffer sb, in a fixed order
   using Pre = decltype(pre((int64_t)-1));
-  auto finish = [&](int sb, int64_t n, const T (&xo)[3], const Pre& pv) {
+  auto finish = [&](int sb, int64_t n, const T (&own)[3], const T (&xo)[3], const Pre& pv) {
     T y[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      y[c] = sm.slot[sb][0 * 3 + c][tid];
-      y[c] += sm.slot[sb][1 * 3 + c][tid - TK];
-      y[c] += sm.slot[sb][2 * 3 + c][tid - 1];
-      y[c] += sm.slot[sb][3 * 3 + c][tid - TK - 1];
+      y[c] = own[c];
+      y[c] += sm.slot[sb][0 * 3 + c][tid - TK];
+      y[c] += sm.slot[sb][1 * 3 + c][tid - 1];
+      y[c] += sm.slot[sb][2 * 3 + c][tid - TK - 1];
     }
     fin(n, y, xo, pv);
   };
@@ -302,7 +305,7 @@ __device__ __forceinline__ void stream_tile_at(StreamSmem<T, TJ, TK>& sm, const 
     for (int c = 0; c < 3; ++c) prev1[a][c] = T(0);
   // one-barrier mode: the node of the previous step, finished after this step's barrier
   Pre pv_h{};
-  T xo_h[3] = {T(0), T(0), T(0)};
+  T xo_h[3] = {T(0), T(0), T(0)}, own_h[3] = {T(0), T(0), T(0)};
   int64_t n_h = 0;
   bool out_h = false;
 
@@ -323,7 +326,7 @@ __device__ __forceinline__ void stream_tile_at(StreamSmem<T, TJ, TK>& sm, const 
     // buffer is written, and every thread is past its reads of the plane buffer that the next
     // step overwrites (three buffers) and of the slot buffer that this step overwrites (two).
     __syncthreads();
-    if (kOneBarrier<T> && out_h) finish((st - 1) & 1, n_h, xo_h, pv_h);
+    if (kOneBarrier<T> && out_h) finish((st - 1) & 1, n_h, own_h, xo_h, pv_h);
     T u[3][8], xo[3];
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -338,11 +341,16 @@ __device__ __forceinline__ void stream_tile_at(StreamSmem<T, TJ, TK>& sm, const 
     for (int c = 0; c < 3; ++c) xo[c] = u[c][0];
     had_apply(u, K, se);
     const int sb = kOneBarrier<T> ? (st & 1) : 0;
+    T own[3];
 #pragma unroll
     for (int a = 0; a < 4; ++a)  // a = oy + 2 oz: corners b = 2a (ox = 0) and 2a + 1 (ox = 1)
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        sm.slot[sb][a * 3 + c][tid] = u[c][2 * a] + prev1[a][c];
+        const T v = u[c][2 * a] + prev1[a][c];
+        if (a == 0)
+          own[c] = v;
+        else
+          sm.slot[sb][(a - 1) * 3 + c][tid] = v;
         prev1[a][c] = u[c][2 * a + 1];
       }
     if constexpr (kOneBarrier<T>) {
@@ -350,16 +358,19 @@ __device__ __forceinline__ void stream_tile_at(StreamSmem<T, TJ, TK>& sm, const 
       n_h = n;
       pv_h = pv;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) xo_h[c] = xo[c];
+      for (int c = 0; c < 3; ++c) {
+        xo_h[c] = xo[c];
+        own_h[c] = own[c];
+      }
     } else {
       __syncthreads();
-      if (outp) finish(0, n, xo, pv);
+      if (outp) finish(0, n, own, xo, pv);
     }
     pb = pn;
   }
   if constexpr (kOneBarrier<T>) {
     __syncthreads();
-    if (out_h) finish((elast - i0 + 1) & 1, n_h, xo_h, pv_h);
+    if (out_h) finish((elast - i0 + 1) & 1, n_h, own_h, xo_h, pv_h);
   }
 }
 
