@@ -68,7 +68,7 @@ class SIMPConfig:
     use_graphs: bool = True
     operator_tile: int = -1           # level-0 operator launch: -1 autotune, else a candidate
     mg_fused: bool = True             # fused V-cycle kernels on the small levels
-    mg_coarse_cheb: int = 3           # Chebyshev degree of the smoother on levels >= 2 (0: Jacobi)
+    mg_coarse_cheb: int = 3           # Chebyshev degree on levels >= 2 (+1 on the two coarsest stencil levels of large grids; 0: Jacobi)
 
 
 @dataclass

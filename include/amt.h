@@ -87,7 +87,9 @@ typedef struct amt_config {
                             another order. */
   int mg_coarse_cheb;    /* smoother of the multigrid levels from 2 on: 0 damped Jacobi (as on
                             levels 0 and 1), k = 2..4 the degree-k Chebyshev polynomial in
-                            M^-1 A, M = diag(sum_j |A_ij|) (default 3) */
+                            M^-1 A, M = diag(sum_j |A_ij|) (default 3); on grids of more than
+                            2^18 elements the two stencil levels above the dense one take
+                            degree k + 1 */
 } amt_config;
 
 /* IterationResult (topology_optimizer.hpp:65-74) plus solver detail. */
