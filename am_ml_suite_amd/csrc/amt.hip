@@ -70,7 +70,7 @@ constexpr float kMgTheta = 1.9f;  // ...capped per row at theta / sum_j |A_ij| (
 // whose spectrum lies in (0, 1] (Gershgorin); the polynomial damps [1 / kChebRatio, 1].
 // Step s (from 1): d' = a_s d + b_s M^-1 (rhs - A x), x' = x + d' (the three-term recurrence
 // of the Chebyshev iteration, with a_1 = 0).
-constexpr double kChebRatio = 100.0;
+constexpr double kChebRatio = 160.0;
 // On grids of more than kChebExtraMinElements elements the two stencil levels above the dense
 // one take kChebCoarseExtra Chebyshev steps more than mg_coarse_cheb: on the 120x60x60 bracket
 // degree 4 there instead of 3 cut the FCG iterations per OC iteration from 33.1 to 29.6 for

@@ -470,7 +470,7 @@ def test_multigrid_fused_preconditioner_matches_one_step_kernels(shape, cheb):
     |r.M s - s.M r| / (|r| |M s|) of the fused form is no larger than the one-step form's (x2, or
     1e-6), and its linearity defect |M(r + 2s) - Mr - 2Ms| / |M(r + 2s)| stays at the one-step
     form's order (x4, or 2e-6: ~30 FP32 ulps, the rounding that the Chebyshev coefficients of
-    the interval [1/100, 1] amplify); both come from the FP32 arithmetic of the smoothers, whose
+    the interval [1/160, 1] amplify); both come from the FP32 arithmetic of the smoothers, whose
     weights span the 1e8 penalty and the 1e-9 void stiffness."""
     p = A.bracket_like_problem(*shape)
     rng = np.random.default_rng(7)
