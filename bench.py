@@ -42,17 +42,15 @@ BF16_PEAK_TFLOPS = 2500.0
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the fast path's level-0 pre-smoothing with the flexible-CG update, by
 # candidate, from the same kind of counter runs (profiles/fcg_apply_bracket120x60x60_hbm_traffic.md)
-PRE_UPD_TRAFFIC_BYTES = {0: 101.49e6, 1: 108.65e6, 2: 88.75e6, 3: 84.40e6, 4: 103.86e6,
-                         5: 87.92e6, 6: 84.37e6, 7: 102.79e6, 8: 87.92e6, 9: 84.38e6,
-                         10: 87.24e6, 11: 95.91e6, 12: 90.85e6, 13: 106.27e6, 14: 87.54e6,
-                         15: 98.60e6}
+PRE_UPD_TRAFFIC_BYTES = {0: 101.34e6, 1: 107.04e6, 2: 89.50e6, 3: 84.41e6, 4: 100.79e6, 5: 87.98e6,
+                         6: 84.38e6, 7: 100.07e6, 8: 87.94e6, 9: 84.39e6, 10: 86.62e6, 11: 93.69e6,
+                         12: 88.60e6, 13: 102.36e6, 14: 86.58e6, 15: 95.60e6}
 # HBM bytes per launch of the fast path's FP64 operator on this workload, by operator candidate
 # (amt_config.operator_tile), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a plain
 # 256 MiB copy (profiles/fcg_apply_bracket120x60x60_hbm_traffic.md)
-FCG_APPLY_TRAFFIC_BYTES = {0: 49.39e6, 1: 51.18e6, 2: 51.32e6, 3: 49.03e6, 4: 51.41e6,
-                          5: 51.54e6, 6: 49.04e6, 7: 50.81e6, 8: 51.38e6, 9: 49.03e6,
-                          10: 49.26e6, 11: 49.41e6, 12: 49.14e6, 13: 50.71e6, 14: 49.14e6,
-                          15: 49.64e6}
+FCG_APPLY_TRAFFIC_BYTES = {0: 49.39e6, 1: 51.40e6, 2: 51.35e6, 3: 49.04e6, 4: 51.60e6, 5: 51.52e6,
+                          6: 49.04e6, 7: 51.15e6, 8: 51.43e6, 9: 49.04e6, 10: 49.28e6, 11: 49.46e6,
+                          12: 49.14e6, 13: 50.51e6, 14: 49.14e6, 15: 49.68e6}
 
 
 def log(msg):
