@@ -14,7 +14,7 @@
 //
 // Streaming tile. A block of TJ x TK = 256 threads owns a column of element positions
 // (j, k) each and walks the element planes i = i0 - 1 .. i0 + L - 1. Per plane it
-//   - writes the next node plane (TJ + 1) x (TK + 1) into LDS (double-buffered by parity),
+//   - writes the next node plane (TJ + 1) x (TK + 1) into LDS (two or three plane buffers),
 //     from values loaded one step earlier (so the load latency hides behind a step's work);
 //   - forms its element's 24 inputs from two node planes and applies s_e Ke0;
 //   - adds the element's ox = 0 corners to the previous element's ox = 1 corners, kept in
