@@ -78,6 +78,12 @@ constexpr double kChebRatio = 160.0;
 // ~17 iterations either way, and the extra launches only cost them (64-problem set, 16^3-64^3:
 // 15.0 against 13.9 OC iter/s).
 constexpr int kChebCoarseExtra = 1;
+// Damped-Jacobi stencil levels of more than this many nodes sweep half-precision blocks
+// (k_mg_stencil_f16). A sweep streams the level's stored stencil (504 B per node in FP32): at
+// 256x128x128 level 1 (545k nodes, 275 MB) that comes from HBM, and halving it gave 19.6 ->
+// 20.1 OC iter/s; at 120x60x60 (59k nodes, 30 MB) the stencil stays in the caches, the sweep is
+// bound by its load instructions, and the scale loads made it slower (137.0 -> 135.1).
+constexpr int64_t kHalfStencilMinNodes = 1 << 18;
 constexpr int64_t kChebExtraMinElements = 1 << 18;
 struct ChebCoef {
   double a, b;
@@ -183,6 +189,10 @@ struct amt_problem {
     MgGrid g;
     MgBrick brick{};     // 4x4x4-node bricks of the fused kernels (levels >= 1)
     float* A = nullptr;  // 27-point block stencil (levels >= 1)
+    // half-precision blocks with per-node scales for the one-launch-per-sweep damped-Jacobi
+    // levels of more than kHalfStencilMinNodes nodes (k_mg_stencil_f16; level 1 in practice)
+    _Float16* Ah = nullptr;
+    float* Ahs = nullptr;
     float *wd = nullptr, *rhs = nullptr, *x = nullptr, *t = nullptr;
     // Chebyshev levels: 1 / row sums and five iterate / direction buffers
     float* mi = nullptr;
@@ -434,6 +444,11 @@ int mg_build_hierarchy(amt_problem* p, const double* Ke) {
       AMT_CHECK(dalloc(p, &p->mg[l].mi, nd));
       for (auto& b : p->mg[l].cx) AMT_CHECK(dalloc(p, &b, nd));
     }
+  for (int l = 1; l <= nl - 2; ++l)
+    if (!p->mg[l].mi && p->mg[l].g.n_nodes > kHalfStencilMinNodes) {
+      AMT_CHECK(dalloc(p, &p->mg[l].Ah, (size_t)9 * kStencilBlocks * p->mg[l].g.n_nodes));
+      AMT_CHECK(dalloc(p, &p->mg[l].Ahs, (size_t)p->mg[l].g.n_nodes));
+    }
   if (p->cfg.mg_fused && nl >= 3 && p->mg_nc <= kDenseMax) {
     for (int l = 1; l <= nl - 2; ++l)
       if (p->mg[l].g.n_nodes <= kFuseMaxNodes) {
@@ -467,6 +482,9 @@ int mg_setup(amt_problem* p) {
     hipLaunchKernelGGL(k_mg_stencil_wdinv, dim3(nblk(3 * mg[l].g.n_nodes)), dim3(kBlock), 0,
                        p->stream, mg[l].A, mg[l].g, kMgOmega, kMgTheta, mg[l].wd, mg[l].mi,
                        (float)cheb_coef(1).b);
+    if (mg[l].Ah)
+      hipLaunchKernelGGL(k_mg_stencil_f16, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0,
+                         p->stream, mg[l].A, mg[l].g, mg[l].Ah, mg[l].Ahs);
   }
   // coarsest level: dense inverse on the device (no host round trip, the OC iteration stays
   // asynchronous)
@@ -831,8 +849,13 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q,
                          mg[l - 1].t, mg[l - 1].g, mg[l].rhs, mg[l].g, mg[l].wd,
                          ch ? mg[l].cx[0] : mg[l].x);
       if (!ch) {
-        hipLaunchKernelGGL(k_mg_pre, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0,
-                           p->stream, mg[l].A, mg[l].g, mg[l].rhs, mg[l].x, mg[l].t);
+        if (mg[l].Ah)
+          hipLaunchKernelGGL(k_mg_pre_h, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0,
+                             p->stream, mg[l].Ah, mg[l].Ahs, mg[l].g, mg[l].rhs, mg[l].x,
+                             mg[l].t);
+        else
+          hipLaunchKernelGGL(k_mg_pre, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0,
+                             p->stream, mg[l].A, mg[l].g, mg[l].rhs, mg[l].x, mg[l].t);
         continue;
       }
       xpre[l] = cheb_steps(l, 2, 0, 0);  // x1 = d1 in buffer 0
@@ -879,8 +902,14 @@ void mg_vcycle(amt_problem* p, const double* r, double* z, const double* q,
       } else {
         hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(mg[l].g.n_nodes)), dim3(kBlock), 0, p->stream,
                            res, mg[l + 1].g, mg[l].x, mg[l].g, nullptr);
-        hipLaunchKernelGGL(k_mg_post, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0,
-                           p->stream, mg[l].A, mg[l].g, mg[l].wd, mg[l].rhs, mg[l].x, mg[l].t);
+        if (mg[l].Ah)
+          hipLaunchKernelGGL(k_mg_post_h, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0,
+                             p->stream, mg[l].Ah, mg[l].Ahs, mg[l].g, mg[l].wd, mg[l].rhs,
+                             mg[l].x, mg[l].t);
+        else
+          hipLaunchKernelGGL(k_mg_post, dim3(mg_row_grid(mg[l].g.n_nodes)), dim3(kBlock), 0,
+                             p->stream, mg[l].A, mg[l].g, mg[l].wd, mg[l].rhs, mg[l].x,
+                             mg[l].t);
       }
       res = mg[l].t;
       continue;

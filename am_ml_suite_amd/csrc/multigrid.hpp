@@ -445,6 +445,54 @@ __device__ __forceinline__ void mg_post_row(const float* __restrict__ A, const M
   out[q] = x[q] + wd[q] * (rhs[q] - y);
 }
 
+// Half-precision copy of a stencil level's blocks for its smoothing sweeps: every stored entry
+// divided by the largest |entry| of its node (S[node]), so each node's 126 values lie in
+// [-1, 1] whatever the density around it (void regions sit ~1e-9 below solid ones), and the
+// sweep multiplies each block's sum by its node's scale. It halves the bytes of the level's
+// sweeps, which stream the stored stencil once per sweep; the coarsening reads the FP32 blocks.
+__global__ __launch_bounds__(kBlock) void k_mg_stencil_f16(const float* __restrict__ A, MgGrid g,
+                                                           _Float16* __restrict__ Ah,
+                                                           float* __restrict__ S) {
+  const int64_t N = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (N >= g.n_nodes) return;
+  const int64_t nn = g.n_nodes;
+  float mx = 0.f;
+  for (int e = 0; e < 126; ++e) mx = fmaxf(mx, fabsf(A[e * nn + N]));
+  const float sc = mx > 0.f ? mx : 1.f, inv = 1.f / sc;
+  for (int e = 0; e < 126; ++e) Ah[e * nn + N] = (_Float16)(A[e * nn + N] * inv);
+  S[N] = sc;
+}
+
+// stencil_row with the half-precision blocks: each block's three products summed, then scaled
+// by the scale of the node that stores the block.
+__device__ __forceinline__ float stencil_row_h(const _Float16* __restrict__ Ah,
+                                               const float* __restrict__ S, const MgGrid& g,
+                                               int64_t n, int i, int j, int k, int r,
+                                               const float (&nb)[27][3]) {
+  const int64_t sj = g.nnz, si = (int64_t)g.nny * g.nnz, nn = g.n_nodes;
+  float y = 0.f, yo = 0.f;  // yo: the blocks stored at n
+#pragma unroll
+  for (int d = 0; d < 27; ++d) {
+    const int di = d / 9 - 1, dj = (d / 3) % 3 - 1, dk = d % 3 - 1;
+    float b = 0.f;
+    if (d >= 13) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        b = fmaf((float)Ah[(int64_t)((d - 13) * 9 + 3 * r + c) * nn + n], nb[d][c], b);
+      yo += b;
+    } else {
+      const bool ok = (i + di) >= 0 && (i + di) < g.nnx && (j + dj) >= 0 &&
+                      (j + dj) < g.nny && (k + dk) >= 0 && (k + dk) < g.nnz;
+      const int64_t m = ok ? n + di * si + dj * sj + dk : n;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        b = fmaf((float)Ah[(int64_t)((13 - d) * 9 + 3 * c + r) * nn + m], nb[d][c], b);
+      y = fmaf(S[m], b, y);
+    }
+  }
+  return fmaf(S[n], yo, y);
+}
+
 // Restriction sum (P^T r_f)(N) at coarse node N = (I, J, K), r_f read through f(fine DOF):
 // the 27 fine nodes 2N + a in the order (x, y, z), weight 1 on the node itself and 1/2 per axis
 // beside it. A fine node outside the grid is read at the clamped position with weight 0, so
@@ -700,6 +748,42 @@ __global__ __launch_bounds__(kBlock) void k_mg_pre(const float* __restrict__ A, 
   int r;
   if (!mg_row_block(g, n, r)) return;
   mg_pre_row(A, g, rhs, x0, t, n, r);
+}
+
+// k_mg_pre / k_mg_post on the half-precision blocks (k_mg_stencil_f16).
+__global__ __launch_bounds__(kBlock) void k_mg_pre_h(const _Float16* __restrict__ Ah,
+                                                     const float* __restrict__ S, MgGrid g,
+                                                     const float* __restrict__ rhs,
+                                                     const float* __restrict__ x0,
+                                                     float* __restrict__ t) {
+  int64_t n;
+  int r;
+  if (!mg_row_block(g, n, r)) return;
+  int i, j, k;
+  mg_decode(n, g, i, j, k);
+  float nb[27][3];
+  mg_gather(g, n, i, j, k, [&](int64_t q) { return x0[q]; }, nb);
+  const float y = stencil_row_h(Ah, S, g, n, i, j, k, r, nb);
+  const int64_t q = 3 * n + r;
+  t[q] = rhs[q] - y;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mg_post_h(const _Float16* __restrict__ Ah,
+                                                      const float* __restrict__ S, MgGrid g,
+                                                      const float* __restrict__ wd,
+                                                      const float* __restrict__ rhs,
+                                                      const float* __restrict__ x,
+                                                      float* __restrict__ out) {
+  int64_t n;
+  int r;
+  if (!mg_row_block(g, n, r)) return;
+  int i, j, k;
+  mg_decode(n, g, i, j, k);
+  float nb[27][3];
+  mg_gather(g, n, i, j, k, [&](int64_t q) { return x[q]; }, nb);
+  const float y = stencil_row_h(Ah, S, g, n, i, j, k, r, nb);
+  const int64_t q = 3 * n + r;
+  out[q] = x[q] + wd[q] * (rhs[q] - y);
 }
 
 __global__ __launch_bounds__(kBlock) void k_mg_post(const float* __restrict__ A, MgGrid g,
